@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the DUGMA registration hot path on one MI355X.
+
+A step is one fused energy + pose-adjoint evaluation (dugma_energy_grad: the
+optimiser's gradient callback) over the 6,000 x 6,000 3-D full-covariance
+workload (structured-light noise, 10% outliers), inputs resident in HBM.
+The value is pair evaluations per second of that whole call.  Beside it the
+line reports:
+  plain      the same gradient by central finite differences (13 forward
+             launches per gradient), same GPU;
+  large      the fused call at 200,000 x 200,000 isotropic;
+  roofline   achieved FP32 FLOP/s of the all-pairs kernel (HIP events on the
+             context's stream) against MI355X's 157.3 TFLOP/s;
+  cpu_baseline  the FP64 oracle (OpenMP, all host cores) on the same
+             workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector, AMD public spec
+# canonical algorithmic FLOP per pair (FMA = 2, exp/rsq/rcp = 1)
+FLOP_3D_FULL_FUSED = 107
+FLOP_3D_FULL_FWD = 59
+FLOP_3D_ISO_FUSED = 36
+FLOP_3D_ISO_FWD = 17
+
+
+class Hip:
+    """HIP events through the runtime's C API (timing on a given stream)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.lib.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.lib.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+    def timed(self, stream, fn, reps):
+        """Milliseconds of `reps` calls of fn, bracketed by device syncs and
+        timed with events on `stream`."""
+        e0, e1 = self.event(), self.event()
+        self.sync()
+        self.lib.hipEventRecord(e0, ctypes.c_void_p(stream))
+        for _ in range(reps):
+            fn()
+        self.lib.hipEventRecord(e1, ctypes.c_void_p(stream))
+        self.lib.hipEventSynchronize(e1)
+        self.sync()
+        ms = ctypes.c_float()
+        self.lib.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        self.lib.hipEventDestroy(e0)
+        self.lib.hipEventDestroy(e1)
+        return float(ms.value)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def start_pose(d, deg=2.0, dt=0.003, seed=0):
+    from dugma_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    return W.random_rotation(rng, deg) @ d["R_G"], np.ravel(d["t_G"]) + rng.uniform(-dt, dt, size=3)
+
+
+def bench_fused(hip, dg, d, steps, warmup, label):
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = start_pose(d)
+        for _ in range(warmup):
+            dg.energy_grad(h, R, t)
+        h.set_timing(True)
+        ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps)
+        kms, launches = h.pair_time()
+        h.set_timing(False)
+        nT, nS, iso = h.energy_info()
+        log("%s: %.3f ms/step, pair kernel %.3f ms/launch (%d launches), iso=%s"
+            % (label, ms / steps, kms / max(launches, 1), launches, iso))
+        return dict(ms_per_step=ms / steps, kernel_ms=kms / max(launches, 1), pairs=nT * nS, iso=iso,
+                    h=None)
+    finally:
+        dg.fr_m(h)
+
+
+def bench_plain(hip, dg, d, steps, step_h=1e-4):
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = start_pose(d)
+        dg.energy_grad_fd(h, R, t, step_h)
+        ms = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, step_h), steps)
+        nT, nS, _ = h.energy_info()
+        return dict(ms_per_gradient=ms / steps, pairs=nT * nS)
+    finally:
+        dg.fr_m(h)
+
+
+def cpu_baseline(d, budget_s=12.0):
+    from dugma_amd import workloads as W
+    from oracle import oracle as O
+    Y, SY, X, SX = d["X"].T, W.unblocks(d["X_C"]), d["Y"].T, W.unblocks(d["Y_C"])
+    R, t = start_pose(d)
+    O.energy(Y[:200], SY[:200], X, SX, R, t, grads=True)  # warm-up
+    times = []
+    t_start = time.time()
+    while len(times) < 5 and (time.time() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        O.energy(Y, SY, X, SX, R, t, grads=True)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    pairs = Y.shape[0] * X.shape[0]
+    return dict(value=pairs / med / 1e9, unit="Gpair-evals/s", cores=O.num_threads(), kind="port",
+                sample="full %dx%d workload, energy + pose gradient in FP64 (oracle, OpenMP), "
+                       "median of %d runs of %.2f s" % (Y.shape[0], X.shape[0], len(times), med))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-large", action="store_true", help="skip the 200K x 200K measurement")
+    ap.add_argument("--no-plain", action="store_true", help="skip the finite-difference measurement")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import dugma_amd as dg
+    from dugma_amd import workloads as W
+    hip = Hip()
+
+    log("generating 6000 x 6000 workload")
+    d2 = W.config2(n=6000, m=6000)
+    fused = bench_fused(hip, dg, d2, args.steps, args.warmup, "fused 6000x6000 full")
+    value = fused["pairs"] / (fused["ms_per_step"] / 1e3) / 1e9
+    achieved = fused["pairs"] * FLOP_3D_FULL_FUSED / (fused["kernel_ms"] / 1e3) / 1e12
+    out = {
+        "metric": "Gpair-evals/s energy+adjoint (fused vs plain FD), batched registrations/s",
+        "value": round(value, 3),
+        "unit": "Gpair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fused["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "3-D 6,000 x 6,000 full-covariance structured-light clouds, 10% outliers; "
+                               "fused energy + 6-DoF pose adjoint per step (per-pair FP32, FP64 sums)",
+                   "target_points": 6000, "source_points": 6000, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED},
+    }
+    if not args.no_plain:
+        plain = bench_plain(hip, dg, d2, max(3, args.steps // 10))
+        pv = plain["pairs"] / (plain["ms_per_gradient"] / 1e3) / 1e9
+        out["plain_fd"] = {"value": round(pv, 3), "unit": "Gpair-evals/s",
+                           "ms_per_gradient": round(plain["ms_per_gradient"], 4),
+                           "scheme": "6-DoF central differences, h=1e-4, 13 forward launches",
+                           "fused_speedup": round(value / pv, 2)}
+        log("plain FD: %.3f ms/gradient, fused speed-up %.2fx" % (plain["ms_per_gradient"], value / pv))
+    if not args.no_large:
+        log("generating 200000 x 200000 workload")
+        d3 = W.config3()
+        big = bench_fused(hip, dg, d3, 3, 1, "fused 200Kx200K iso")
+        bv = big["pairs"] / (big["ms_per_step"] / 1e3) / 1e9
+        ba = big["pairs"] * FLOP_3D_ISO_FUSED / (big["kernel_ms"] / 1e3) / 1e12
+        out["large"] = {"workload": "3-D 200,000 x 200,000 isotropic (8-NN sigma), fused energy + pose adjoint",
+                        "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
+                        "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
+                        "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED}
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(d2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,206 @@
+"""dugma_amd -- MI355X (gfx950) implementation of DUGMA's registration hot path.
+
+Python mirror of the reference's MEX entry points (DUGMA.m:44-126):
+
+    ok, h = init(X, X_C, Y, Y_C)              # init.cpp:36-83
+    sig = update_sig(h, R, t, N, M, D)        # update_sig.cpp:8-66
+    coef = compi(h, R, t, N, M, D, sig)       # compi.cpp:8-74
+    ok = fr_m(h)                              # fr_m.cpp:8-35
+
+X is the fixed target cloud (D x N, MATLAB layout), Y the moving source cloud
+(D x M), X_C / Y_C the D x (D*N) covariance blocks; R, t the current pose.
+The MEX functions also hand back six device handles; here the one context
+object ``h`` stands for all six.
+
+The convolved-uncertainty energy  E = sum_ij N(X_i - R Y_j - t; CX_i + R CY_j R^T)
+and its adjoints are exposed through ``energy``, ``energy_grad``,
+``energy_grad_points`` and ``energy_grad_fd``; exact k nearest neighbours
+through ``knn``.  Everything runs in libdugma_hip.so; nothing here computes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DugmaError
+
+COV_AUTO, COV_FULL, COV_ISO = 0, 1, 2
+KNN_TARGET_SELF, KNN_SOURCE_SELF, KNN_TARGET_TO_SOURCE = 0, 1, 2
+_COV = {"auto": COV_AUTO, "full": COV_FULL, "iso": COV_ISO}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _cm(a, shape=None):
+    """Column-major FP64 copy (MATLAB memory order)."""
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    if shape is not None and a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+def _cov_mode(mode):
+    return _COV[mode] if isinstance(mode, str) else int(mode)
+
+
+class Context:
+    """The device state created by ``init`` (the reference's six handles)."""
+
+    def __init__(self, ptr, N, M, D):
+        self._ptr = ctypes.c_void_p(ptr)
+        self.N, self.M, self.D = N, M, D
+
+    @property
+    def ptr(self):
+        if not self._ptr or not self._ptr.value:
+            raise DugmaError(-5, "context already freed")
+        return self._ptr
+
+    def stream(self):
+        """hipStream_t the context launches on (as an integer)."""
+        return _lib.load().dugma_ctx_stream(self.ptr)
+
+    def set_timing(self, on=True):
+        _lib.check(_lib.load().dugma_ctx_set_timing(self.ptr, int(bool(on))))
+
+    def pair_time(self):
+        """(milliseconds, launches) of the all-pairs kernels since set_timing(True)."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        _lib.check(_lib.load().dugma_ctx_pair_time(self.ptr, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def energy_info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().dugma_ctx_energy_info(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, bool(c.value)
+
+    def read_handle(self, which):
+        """Host copy of reference handle ``which`` (0..5: X, inv X_C, Y, inv Y_C, g_X, g_Y)."""
+        N, M, D = self.N, self.M, self.D
+        n = [N * D, N * D * D, M * D, M * D * D, N, M][which]
+        out = np.empty(n, np.float32)
+        _lib.check(_lib.load().dugma_read_handle(self.ptr, int(which),
+                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def free(self):
+        if self._ptr and self._ptr.value:
+            ok = ctypes.c_double()
+            _lib.check(_lib.load().dugma_fr_m(self._ptr, ctypes.byref(ok)))
+            self._ptr = ctypes.c_void_p(0)
+            return ok.value
+        return 1.0
+
+
+def init(X, X_C, Y, Y_C):
+    """[ok, h] = init(X, X_C, Y, Y_C): X (D,N), X_C (D,D*N), Y (D,M), Y_C (D,D*M)."""
+    X = _cm(X)
+    D, N = X.shape
+    Y = _cm(Y)
+    M = Y.shape[1]
+    if Y.shape[0] != D:
+        raise ValueError("X and Y must have the same dimension")
+    X_C = _cm(X_C, (D, D * N))
+    Y_C = _cm(Y_C, (D, D * M))
+    out = ctypes.c_void_p()
+    ok = ctypes.c_double()
+    _lib.check(_lib.load().dugma_init(_dp(X), _dp(X_C), _dp(Y), _dp(Y_C), N, M, D,
+                                      ctypes.byref(out), ctypes.byref(ok)))
+    return ok.value, Context(out.value, N, M, D)
+
+
+def update_sig(h, R, t, N=None, M=None, D=None):
+    """sig = update_sig(h, R, t, N, M, D) (FP32 value returned as a float)."""
+    R = _cm(R, (h.D, h.D))
+    t = _cm(np.ravel(t))
+    sig = ctypes.c_double()
+    _lib.check(_lib.load().dugma_update_sig(h.ptr, _dp(R), _dp(t), float(N or h.N),
+                                            float(M or h.M), float(D or h.D), ctypes.byref(sig)))
+    return sig.value
+
+
+def compi(h, R, t, N=None, M=None, D=None, sig=1.0):
+    """coef = compi(h, R, t, N, M, D, sig): 637 (3-D) or 45 (2-D) coefficients."""
+    R = _cm(R, (h.D, h.D))
+    t = _cm(np.ravel(t))
+    coef = np.zeros(637 if h.D == 3 else 45)
+    _lib.check(_lib.load().dugma_compi(h.ptr, _dp(R), _dp(t), float(N or h.N), float(M or h.M),
+                                       float(D or h.D), float(sig), _dp(coef)))
+    return coef
+
+
+def fr_m(h):
+    """ok = fr_m(h)."""
+    return h.free()
+
+
+def energy(h, R, t, cov="auto"):
+    R = _cm(R, (h.D, h.D))
+    t = _cm(np.ravel(t))
+    E = ctypes.c_double()
+    _lib.check(_lib.load().dugma_energy(h.ptr, _dp(R), _dp(t), _cov_mode(cov), ctypes.byref(E)))
+    return E.value
+
+
+def energy_grad(h, R, t, cov="auto"):
+    """(E, gpose): gpose = [dE/dw (3), dE/dt (3)] (3-D) or [dE/dtheta, dE/dt (2)]."""
+    R = _cm(R, (h.D, h.D))
+    t = _cm(np.ravel(t))
+    E = ctypes.c_double()
+    g = np.zeros(6 if h.D == 3 else 3)
+    _lib.check(_lib.load().dugma_energy_grad(h.ptr, _dp(R), _dp(t), _cov_mode(cov),
+                                             ctypes.byref(E), _dp(g)))
+    return E.value, g
+
+
+def energy_grad_fd(h, R, t, step=1e-4, cov="auto"):
+    R = _cm(R, (h.D, h.D))
+    t = _cm(np.ravel(t))
+    E = ctypes.c_double()
+    g = np.zeros(6 if h.D == 3 else 3)
+    _lib.check(_lib.load().dugma_energy_grad_fd(h.ptr, _dp(R), _dp(t), _cov_mode(cov), float(step),
+                                                ctypes.byref(E), _dp(g)))
+    return E.value, g
+
+
+def energy_grad_points(h, R, t, cov="auto"):
+    """Energy with per-point gradients.  Returns a dict of numpy arrays:
+    gX (N,D), gXC (N,D,D), gY (M,D), gYC (M,D,D), gpose, gdT (12 or 4)."""
+    D, N, M = h.D, h.N, h.M
+    R = _cm(R, (D, D))
+    t = _cm(np.ravel(t))
+    E = ctypes.c_double()
+    gpose = np.zeros(6 if D == 3 else 3)
+    gX = np.zeros((N, D))
+    gXC = np.zeros((N, D, D))
+    gY = np.zeros((M, D))
+    gYC = np.zeros((M, D, D))
+    gdT = np.zeros(12 if D == 3 else 4)
+    _lib.check(_lib.load().dugma_energy_grad_points(h.ptr, _dp(R), _dp(t), _cov_mode(cov),
+                                                    ctypes.byref(E), _dp(gpose), _dp(gX), _dp(gXC),
+                                                    _dp(gY), _dp(gYC), _dp(gdT)))
+    # covariance gradients are column-major per point
+    return dict(E=E.value, gpose=gpose, gX=gX, gXC=np.transpose(gXC, (0, 2, 1)), gY=gY,
+                gYC=np.transpose(gYC, (0, 2, 1)), gdT=gdT)
+
+
+def knn(h, which, k, R=None, t=None):
+    """(idx, d2) of the k nearest neighbours; see include/dugma.h for ``which``."""
+    nq = h.M if which == KNN_SOURCE_SELF else h.N
+    idx = np.empty((nq, k), np.int32)
+    d2 = np.empty((nq, k), np.float32)
+    Rp = tp = None
+    if R is not None:
+        R = _cm(R, (h.D, h.D))
+        t = _cm(np.ravel(t))
+        Rp, tp = _dp(R), _dp(t)
+    _lib.check(_lib.load().dugma_knn(h.ptr, int(which), int(k), Rp, tp,
+                                     idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     d2.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return idx, d2
+
+
+def version():
+    return _lib.load().dugma_version().decode()

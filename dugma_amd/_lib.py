@@ -1,0 +1,71 @@
+"""ctypes binding of libdugma_hip.so (the C ABI declared in include/dugma.h).
+
+The library is loaded from this package directory; there is no fallback: if
+it is missing or does not load, every call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdugma_hip.so")
+
+_d = ctypes.POINTER(ctypes.c_double)
+_f = ctypes.POINTER(ctypes.c_float)
+_i = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_ci = ctypes.c_int
+_cd = ctypes.c_double
+
+# name: (restype, argtypes) -- one entry per declaration of include/dugma.h
+SIGNATURES = {
+    "dugma_init": (_ci, [_d, _d, _d, _d, _ci, _ci, _ci, ctypes.POINTER(_vp), _d]),
+    "dugma_update_sig": (_ci, [_vp, _d, _d, _cd, _cd, _cd, _d]),
+    "dugma_compi": (_ci, [_vp, _d, _d, _cd, _cd, _cd, _cd, _d]),
+    "dugma_fr_m": (_ci, [_vp, _d]),
+    "dugma_energy": (_ci, [_vp, _d, _d, _ci, _d]),
+    "dugma_energy_grad": (_ci, [_vp, _d, _d, _ci, _d, _d]),
+    "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),
+    "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),
+    "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),
+    "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
+    "dugma_read_handle": (_ci, [_vp, _ci, _f]),
+    "dugma_ctx_dims": (_ci, [_vp, _i, _i, _i]),
+    "dugma_ctx_energy_info": (_ci, [_vp, _i, _i, _i]),
+    "dugma_ctx_stream": (_vp, [_vp]),
+    "dugma_ctx_set_timing": (_ci, [_vp, _ci]),
+    "dugma_ctx_pair_time": (_ci, [_vp, _d, ctypes.POINTER(ctypes.c_longlong)]),
+    "dugma_last_error": (ctypes.c_char_p, []),
+    "dugma_version": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+class DugmaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dugma error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libdugma_hip.so once and bind every C-ABI symbol."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libdugma_hip.so is not built (run __graft_entry__.build() "
+                           "or python dugma_amd/build.py)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = load().dugma_last_error()
+        raise DugmaError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,701 @@
+// dugma_energy.hip -- convolved-uncertainty energy, fused pose adjoint,
+// per-point adjoint and the finite-difference ("plain") gradient.
+//
+// Kernel shape: a workgroup is 4 waves; each lane owns T stationary points
+// (targets, or sources in the per-point source pass) in VGPRs and walks a
+// chunk of the other cloud.  The streamed records are read with wave-uniform
+// addresses, so they come through the scalar cache into SGPRs and feed the
+// VALU directly: no LDS, no barriers in the pair loop.  Per-pair terms are
+// FP32, summed in FP32 over sub-tiles of kSub streamed points and then in
+// FP64 per lane; workgroup partials are reduced in a fixed order, so results
+// are deterministic run to run.  Nothing of size N x M is stored.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dugma_internal.h"
+#include "dugma_pairs.h"
+
+namespace dugma {
+
+constexpr int kBlock = 256;  // 4 waves of 64
+constexpr int kSub = 64;     // FP32 sub-tile length before the FP64 flush
+
+enum { kModeEnergy = 0, kModePose = 1, kModePoint = 2 };
+
+struct PoseArg {
+  double R[9];  // column-major
+  double t[3];
+};
+
+// ---------------------------------------------------------------------------
+// Pair kernel
+// ---------------------------------------------------------------------------
+template <class Rec, int MODE, int T, bool COMP>
+__global__ __launch_bounds__(kBlock) void k_pairs(
+    const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
+    int qchunk, double *__restrict__ out, double t0, double t1, double t2) {
+  using Tr = RecTraits<Rec>;
+  constexpr bool GRAD = MODE != kModeEnergy;
+  constexpr int NA = GRAD ? Tr::NACC : 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int pbase = blockIdx.x * (kBlock * T) + wave * (64 * T) + lane;
+  const int q0 = blockIdx.y * qchunk;
+  const int q1 = min(q0 + qchunk, nQ);
+
+  Rec p[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) p[k] = P[min(pbase + 64 * k, nP - 1)];
+
+  double dacc[T][NA];
+#pragma unroll
+  for (int k = 0; k < T; ++k)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) dacc[k][a] = 0.0;
+
+  for (int j0 = q0; j0 < q1; j0 += kSub) {
+    float acc[T][NA];
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+#pragma unroll
+      for (int a = 0; a < NA; ++a) acc[k][a] = 0.0f;
+    const int j1 = min(j0 + kSub, q1);
+#pragma unroll 2
+    for (int j = j0; j < j1; ++j) {
+      const Rec q = Q[j];
+#pragma unroll
+      for (int k = 0; k < T; ++k) pair_eval<GRAD, COMP>(p[k], q, acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+#pragma unroll
+      for (int a = 0; a < NA; ++a) dacc[k][a] += (double)acc[k][a];
+  }
+
+  if (MODE == kModePoint) {
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      const int pi = pbase + 64 * k;
+      if (pi < nP) {
+        double *o = out + ((size_t)blockIdx.y * nP + pi) * NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) o[a] = dacc[k][a];
+      }
+    }
+    return;
+  }
+
+  constexpr int NR = GRAD ? kPosePart : 1;
+  double r[kPosePart];
+#pragma unroll
+  for (int a = 0; a < kPosePart; ++a) r[a] = 0.0;
+  const double tt[3] = {t0, t1, t2};
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    if (pbase + 64 * k < nP) {
+      if (GRAD)
+        pose_contrib(p[k], dacc[k], tt, r);
+      else
+        r[0] += dacc[k][0];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < NR; ++a) r[a] += __shfl_xor(r[a], off);
+  __shared__ double red[kBlock / 64][kPosePart];
+  if (lane == 0)
+#pragma unroll
+    for (int a = 0; a < NR; ++a) red[wave][a] = r[a];
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double s = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart + threadIdx.x] = s;
+  }
+}
+
+// Fixed-order sum of per-workgroup pose partials (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_reduce_parts(
+    const double *__restrict__ parts, int nparts, int width,
+    double *__restrict__ res) {
+  __shared__ double sh[kBlock][kPosePart];
+  double s[kPosePart];
+#pragma unroll
+  for (int a = 0; a < kPosePart; ++a) s[a] = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += kBlock)
+    for (int a = 0; a < width; ++a) s[a] += parts[(size_t)k * kPosePart + a];
+#pragma unroll
+  for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] = s[a];
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+      for (int a = 0; a < width; ++a) sh[threadIdx.x][a] += sh[threadIdx.x + stride][a];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < width) res[threadIdx.x] = sh[0][threadIdx.x];
+}
+
+// Per-point sums over streamed chunks, in chunk order.
+__global__ __launch_bounds__(kBlock) void k_reduce_points(
+    const double *__restrict__ parts, int nP, int nchunks, int nacc,
+    double *__restrict__ res) {
+  const size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const size_t total = (size_t)nP * nacc;
+  if (idx >= total) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += parts[(size_t)c * total + idx];
+  res[idx] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Per-call source records: x' = R x + t and S' = R S R^T in FP64, then
+// split to FP32 hi/lo (positions) and rounded (covariance).
+// ---------------------------------------------------------------------------
+template <int D, bool ISO>
+__global__ __launch_bounds__(kBlock) void k_transform_src(
+    const double *__restrict__ src, int n, PoseArg ps, void *__restrict__ outv) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const double *s = src + (size_t)k * (D + D * D);
+  double xp[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double v = ps.t[a];
+#pragma unroll
+    for (int b = 0; b < D; ++b) v = fma(ps.R[a + D * b], s[b], v);
+    xp[a] = v;
+  }
+  float hi[3], lo[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    hi[a] = (float)xp[a];
+    lo[a] = (float)(xp[a] - (double)hi[a]);
+  }
+  const double *S = s + D;
+  if (ISO) {
+    if (D == 3) {
+      Rec3I r;
+      r.hx = hi[0]; r.hy = hi[1]; r.hz = hi[2];
+      r.lx = lo[0]; r.ly = lo[1]; r.lz = lo[2];
+      r.s = (float)S[0];
+      r.pad0 = 0.0f;
+      reinterpret_cast<Rec3I *>(outv)[k] = r;
+    } else {
+      Rec2I r;
+      r.hx = hi[0]; r.hy = hi[1]; r.lx = lo[0]; r.ly = lo[1];
+      r.s = (float)S[0];
+      r.pad0 = r.pad1 = r.pad2 = 0.0f;
+      reinterpret_cast<Rec2I *>(outv)[k] = r;
+    }
+    return;
+  }
+  // S' = R S R^T
+  double RS[9], Sp[9];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < D; ++b) v = fma(ps.R[a + D * b], S[b + D * c], v);
+      RS[a + D * c] = v;
+    }
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < D; ++b) v = fma(RS[a + D * b], ps.R[c + D * b], v);
+      Sp[a + D * c] = v;
+    }
+  if (D == 3) {
+    Rec3F r;
+    r.hx = hi[0]; r.hy = hi[1]; r.hz = hi[2];
+    r.lx = lo[0]; r.ly = lo[1]; r.lz = lo[2];
+    r.s11 = (float)Sp[0];
+    r.s22 = (float)Sp[4];
+    r.s33 = (float)Sp[8];
+    r.s12 = (float)(0.5 * (Sp[3] + Sp[1]));
+    r.s13 = (float)(0.5 * (Sp[6] + Sp[2]));
+    r.s23 = (float)(0.5 * (Sp[7] + Sp[5]));
+    r.pad0 = r.pad1 = r.pad2 = r.pad3 = 0.0f;
+    reinterpret_cast<Rec3F *>(outv)[k] = r;
+  } else {
+    Rec2F r;
+    r.hx = hi[0]; r.hy = hi[1]; r.lx = lo[0]; r.ly = lo[1];
+    r.s11 = (float)Sp[0];
+    r.s22 = (float)Sp[3];
+    r.s12 = (float)(0.5 * (Sp[2] + Sp[1]));
+    r.pad0 = 0.0f;
+    reinterpret_cast<Rec2F *>(outv)[k] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch planning
+// ---------------------------------------------------------------------------
+struct Plan {
+  int T = 1;
+  int pblocks = 1, qchunks = 1, qchunk = 1;
+};
+
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+
+// Enough workgroups to keep every SIMD of the 256 CUs several waves deep,
+// with streamed chunks of at least kSub points.
+static Plan make_plan(int nP, int nQ, int mode, int T) {
+  Plan pl;
+  pl.T = T;
+  pl.pblocks = (nP + kBlock * T - 1) / (kBlock * T);
+  const int target_waves = env_int("DUGMA_TARGET_WAVES", 8192);
+  const int waves_p = pl.pblocks * (kBlock / 64);
+  int qc = (target_waves + waves_p - 1) / waves_p;
+  int maxc = (nQ + kSub - 1) / kSub;
+  if (mode == kModePoint) maxc = std::min(maxc, 64);
+  qc = std::max(1, std::min(qc, maxc));
+  pl.qchunk = (nQ + qc - 1) / qc;
+  pl.qchunks = (nQ + pl.qchunk - 1) / pl.qchunk;
+  return pl;
+}
+
+template <class Rec, int MODE>
+static void launch_pairs_t(const Plan &pl, bool comp, const Rec *P, int nP,
+                           const Rec *Q, int nQ, double *out, const double *t,
+                           hipStream_t s) {
+  dim3 grid(pl.pblocks, pl.qchunks), block(kBlock);
+  const double t0 = t[0], t1 = t[1], t2 = (RecTraits<Rec>::D == 3) ? t[2] : 0.0;
+  if (pl.T == 2) {
+    if (comp)
+      hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, true>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
+    else
+      hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, false>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
+  } else {
+    if (comp)
+      hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, true>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
+    else
+      hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, false>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
+  }
+}
+
+template <class Rec>
+static void launch_pairs(int mode, const Plan &pl, bool comp, const Rec *P,
+                         int nP, const Rec *Q, int nQ, double *out,
+                         const double *t, hipStream_t s) {
+  if (mode == kModeEnergy)
+    launch_pairs_t<Rec, kModeEnergy>(pl, comp, P, nP, Q, nQ, out, t, s);
+  else if (mode == kModePose)
+    launch_pairs_t<Rec, kModePose>(pl, comp, P, nP, Q, nQ, out, t, s);
+  else
+    launch_pairs_t<Rec, kModePoint>(pl, comp, P, nP, Q, nQ, out, t, s);
+}
+
+static PoseArg make_pose(int D, const double *R, const double *t) {
+  PoseArg ps;
+  memset(&ps, 0, sizeof(ps));
+  for (int a = 0; a < D * D; ++a) ps.R[a] = R[a];
+  for (int a = 0; a < D; ++a) ps.t[a] = t[a];
+  return ps;
+}
+
+static int transform_sources(dugma_ctx *c, const PoseArg &ps, bool iso) {
+  if (c->nS == 0) return DUGMA_OK;
+  dim3 grid((c->nS + kBlock - 1) / kBlock), block(kBlock);
+  void *out = iso ? c->dSrecI : c->dSrecF;
+  if (c->D == 3) {
+    if (iso)
+      hipLaunchKernelGGL((k_transform_src<3, true>), grid, block, 0, c->stream, c->dSsrc, c->nS, ps, out);
+    else
+      hipLaunchKernelGGL((k_transform_src<3, false>), grid, block, 0, c->stream, c->dSsrc, c->nS, ps, out);
+  } else {
+    if (iso)
+      hipLaunchKernelGGL((k_transform_src<2, true>), grid, block, 0, c->stream, c->dSsrc, c->nS, ps, out);
+    else
+      hipLaunchKernelGGL((k_transform_src<2, false>), grid, block, 0, c->stream, c->dSsrc, c->nS, ps, out);
+  }
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+static int resolve_iso(dugma_ctx *c, int cov_mode, bool *iso) {
+  if (cov_mode == DUGMA_COV_FULL) {
+    *iso = false;
+  } else if (cov_mode == DUGMA_COV_ISO) {
+    if (!c->iso) return fail(DUGMA_ERR_ARG, "DUGMA_COV_ISO: covariances are not all isotropic");
+    *iso = true;
+  } else if (cov_mode == DUGMA_COV_AUTO) {
+    *iso = c->iso;
+  } else {
+    return fail(DUGMA_ERR_ARG, "unknown cov_mode");
+  }
+  return DUGMA_OK;
+}
+
+static double gauss_norm(int D) {
+  const double two_pi = 6.283185307179586476925;
+  return D == 3 ? 1.0 / (two_pi * std::sqrt(two_pi)) : 1.0 / two_pi;
+}
+
+static bool comp_enabled() { return env_int("DUGMA_COMPENSATED", 1) != 0; }
+static int lane_points() { return env_int("DUGMA_LANE_POINTS", 1) == 2 ? 2 : 1; }
+
+// Energy-only or fused pose pass; res receives [E, gpose...] scaled.
+static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
+                     int mode, double *res) {
+  const int D = c->D;
+  const int width = mode == kModeEnergy ? 1 : (D == 3 ? 7 : 4);
+  for (int a = 0; a < width; ++a) res[a] = 0.0;
+  if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
+  PoseArg ps = make_pose(D, R, t);
+  int rc = transform_sources(c, ps, iso);
+  if (rc) return rc;
+  Plan pl = make_plan(c->nT, c->nS, mode, lane_points());
+  const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
+  rc = c->part.ensure((nparts + 1) * kPosePart * sizeof(double));
+  if (rc) return rc;
+  double *parts = (double *)c->part.ptr;
+  double *dres = parts + nparts * kPosePart;
+  const bool comp = comp_enabled();
+  PairTimer timer(c);
+  if (D == 3) {
+    if (iso)
+      launch_pairs<Rec3I>(mode, pl, comp, (const Rec3I *)c->dTrecI, c->nT, (const Rec3I *)c->dSrecI, c->nS, parts, t, c->stream);
+    else
+      launch_pairs<Rec3F>(mode, pl, comp, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS, parts, t, c->stream);
+  } else {
+    if (iso)
+      launch_pairs<Rec2I>(mode, pl, comp, (const Rec2I *)c->dTrecI, c->nT, (const Rec2I *)c->dSrecI, c->nS, parts, t, c->stream);
+    else
+      launch_pairs<Rec2F>(mode, pl, comp, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS, parts, t, c->stream);
+  }
+  timer.stop();
+  DUGMA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, c->stream, parts, (int)nparts, width, dres);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipMemcpyAsync(c->hpinned, dres, sizeof(double) * width, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  timer.finish();
+  const double cn = gauss_norm(D);
+  for (int a = 0; a < width; ++a) res[a] = cn * c->hpinned[a];
+  return DUGMA_OK;
+}
+
+static int check_pose_args(dugma_ctx *c, const double *R, const double *t) {
+  if (!c) return fail(DUGMA_ERR_ARG, "null context");
+  if (!R || !t) return fail(DUGMA_ERR_ARG, "null R or t");
+  for (int a = 0; a < c->D * c->D; ++a)
+    if (!std::isfinite(R[a])) return fail(DUGMA_ERR_ARG, "R is not finite");
+  for (int a = 0; a < c->D; ++a)
+    if (!std::isfinite(t[a])) return fail(DUGMA_ERR_ARG, "t is not finite");
+  return DUGMA_OK;
+}
+
+// Left perturbation of the pose: 3-D p = [w, dt], R <- exp([w]x) R;
+// 2-D p = [theta, dt], R <- Rot(theta) R.
+static void perturb(int D, const double *R, const double *t, const double *p,
+                    double *Rn, double *tn) {
+  if (D == 3) {
+    const double *w = p;
+    double th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double A = th < 1e-12 ? 1.0 : std::sin(th) / th;
+    double B = th < 1e-12 ? 0.5 : (1.0 - std::cos(th)) / (th * th);
+    double K[9] = {0, w[2], -w[1], -w[2], 0, w[0], w[1], -w[0], 0};
+    double E[9];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double k2 = 0;
+        for (int b = 0; b < 3; ++b) k2 += K[a + 3 * b] * K[b + 3 * c];
+        E[a + 3 * c] = (a == c ? 1.0 : 0.0) + A * K[a + 3 * c] + B * k2;
+      }
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double s = 0;
+        for (int b = 0; b < 3; ++b) s += E[a + 3 * b] * R[b + 3 * c];
+        Rn[a + 3 * c] = s;
+      }
+    for (int a = 0; a < 3; ++a) tn[a] = t[a] + p[3 + a];
+  } else {
+    double cs = std::cos(p[0]), sn = std::sin(p[0]);
+    double E[4] = {cs, sn, -sn, cs};
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c)
+        Rn[a + 2 * c] = E[a] * R[2 * c] + E[a + 2] * R[1 + 2 * c];
+    tn[0] = t[0] + p[1];
+    tn[1] = t[1] + p[2];
+  }
+}
+
+// One per-point pass (stationary set P, streamed set Q), full covariance
+// records; writes per-point FP64 sums [nP][NACC] to out (device).
+static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
+                      int nQ, double *out) {
+  const int D = c->D;
+  const int nacc = D == 3 ? kAcc3F : kAcc2F;
+  Plan pl = make_plan(nP, nQ, kModePoint, 1);
+  const size_t nparts = (size_t)pl.qchunks * nP * nacc;
+  int rc = c->part.ensure(nparts * sizeof(double));
+  if (rc) return rc;
+  double *parts = (double *)c->part.ptr;
+  const double t0[3] = {0, 0, 0};
+  const bool comp = comp_enabled();
+  if (D == 3)
+    launch_pairs<Rec3F>(kModePoint, pl, comp, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, t0, c->stream);
+  else
+    launch_pairs<Rec2F>(kModePoint, pl, comp, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, t0, c->stream);
+  DUGMA_HIP(hipGetLastError());
+  const size_t total = (size_t)nP * nacc;
+  hipLaunchKernelGGL(k_reduce_points, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                     parts, nP, pl.qchunks, nacc, out);
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+}  // namespace dugma
+
+using namespace dugma;
+
+extern "C" int dugma_energy(dugma_ctx *ctx, const double *R, const double *t,
+                            int cov_mode, double *E) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  if (!E) return fail(DUGMA_ERR_ARG, "null E");
+  bool iso;
+  rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  double res[kPosePart];
+  rc = pose_pass(ctx, R, t, iso, kModeEnergy, res);
+  if (rc) return rc;
+  *E = res[0];
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_energy_grad(dugma_ctx *ctx, const double *R,
+                                 const double *t, int cov_mode, double *E,
+                                 double *gpose) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  bool iso;
+  rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  double res[kPosePart];
+  rc = pose_pass(ctx, R, t, iso, kModePose, res);
+  if (rc) return rc;
+  if (E) *E = res[0];
+  const int nd = ctx->D == 3 ? 6 : 3;
+  if (gpose)
+    for (int a = 0; a < nd; ++a) gpose[a] = res[1 + a];
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_energy_grad_fd(dugma_ctx *ctx, const double *R,
+                                    const double *t, int cov_mode, double h,
+                                    double *E, double *gpose) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  if (!(h > 0) || !std::isfinite(h)) return fail(DUGMA_ERR_ARG, "h must be positive");
+  bool iso;
+  rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  const int D = ctx->D, nd = D == 3 ? 6 : 3;
+  double res[kPosePart];
+  rc = pose_pass(ctx, R, t, iso, kModeEnergy, res);
+  if (rc) return rc;
+  if (E) *E = res[0];
+  for (int k = 0; k < nd; ++k) {
+    double p[6] = {0, 0, 0, 0, 0, 0}, Rn[9], tn[3], ep, em;
+    p[k] = h;
+    perturb(D, R, t, p, Rn, tn);
+    rc = pose_pass(ctx, Rn, tn, iso, kModeEnergy, res);
+    if (rc) return rc;
+    ep = res[0];
+    p[k] = -h;
+    perturb(D, R, t, p, Rn, tn);
+    rc = pose_pass(ctx, Rn, tn, iso, kModeEnergy, res);
+    if (rc) return rc;
+    em = res[0];
+    if (gpose) gpose[k] = (ep - em) / (2.0 * h);
+  }
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
+                                        const double *t, int cov_mode,
+                                        double *E, double *gpose, double *gX,
+                                        double *gXC, double *gY, double *gYC,
+                                        double *gdT) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  if (cov_mode != DUGMA_COV_AUTO && cov_mode != DUGMA_COV_FULL && cov_mode != DUGMA_COV_ISO)
+    return fail(DUGMA_ERR_ARG, "unknown cov_mode");
+  dugma_ctx *c = ctx;
+  const int D = c->D, DD = D * D, N = c->N, M = c->M;
+  const int nacc = D == 3 ? kAcc3F : kAcc2F;
+  const int nd = D == 3 ? 6 : 3;
+  if (gX) std::fill(gX, gX + (size_t)N * D, 0.0);
+  if (gXC) std::fill(gXC, gXC + (size_t)N * DD, 0.0);
+  if (gY) std::fill(gY, gY + (size_t)M * D, 0.0);
+  if (gYC) std::fill(gYC, gYC + (size_t)M * DD, 0.0);
+  if (gpose) std::fill(gpose, gpose + nd, 0.0);
+  if (gdT) std::fill(gdT, gdT + (D == 3 ? 12 : 4), 0.0);
+  if (E) *E = 0.0;
+  if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
+
+  // per-point passes use full-covariance records (isotropic points included)
+  PoseArg ps = make_pose(D, R, t);
+  rc = transform_sources(c, ps, false);
+  if (rc) return rc;
+  double *acc = nullptr;
+  const size_t nT = c->nT, nS = c->nS;
+  DUGMA_HIP(hipMalloc(&acc, sizeof(double) * (nT + nS) * nacc));
+  std::vector<double> hT(nT * nacc), hS(nS * nacc);
+  rc = point_pass(c, c->dTrecF, c->nT, c->dSrecF, c->nS, acc);
+  if (!rc) rc = point_pass(c, c->dSrecF, c->nS, c->dTrecF, c->nT, acc + nT * nacc);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(hT.data(), acc, sizeof(double) * nT * nacc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hS.data(), acc + nT * nacc, sizeof(double) * nS * nacc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "dugma_energy_grad_points: download");
+  }
+  (void)hipFree(acc);
+  if (rc) return rc;
+
+  const double cn = gauss_norm(D);
+  // unpack symmetric (WW - C)/2 into a full column-major matrix
+  auto cov_grad = [&](const double *a, double *G) {
+    if (D == 3) {
+      static const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+      for (int k = 0; k < 6; ++k) {
+        double v = 0.5 * cn * (a[4 + k] - a[10 + k]);
+        G[ij[k][0] + 3 * ij[k][1]] = v;
+        G[ij[k][1] + 3 * ij[k][0]] = v;
+      }
+    } else {
+      static const int ij[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+      for (int k = 0; k < 3; ++k) {
+        double v = 0.5 * cn * (a[3 + k] - a[6 + k]);
+        G[ij[k][0] + 2 * ij[k][1]] = v;
+        G[ij[k][1] + 2 * ij[k][0]] = v;
+      }
+    }
+  };
+  double Etot = 0.0;
+  for (size_t k = 0; k < nT; ++k) {
+    const double *a = hT.data() + k * nacc;
+    Etot += a[0];
+    const int i = c->hTidx[k];
+    if (gX)
+      for (int d = 0; d < D; ++d) gX[(size_t)i * D + d] = -cn * a[1 + d];
+    if (gXC) cov_grad(a, gXC + (size_t)i * DD);
+  }
+  if (E) *E = cn * Etot;
+
+  double gw[3] = {0, 0, 0}, gt[3] = {0, 0, 0}, gth = 0.0, G[9] = {0};
+  for (size_t k = 0; k < nS; ++k) {
+    const double *a = hS.data() + k * nacc;
+    const double *s = c->hSsrc.data() + k * (D + DD);
+    double g[3], H[9], xp[3], p[3], Sp[9];
+    for (int d = 0; d < D; ++d) g[d] = -cn * a[1 + d];  // dE/dx'_j
+    cov_grad(a, H);                                     // dE/dS'_j
+    for (int d = 0; d < D; ++d) {
+      double v = 0;
+      for (int b = 0; b < D; ++b) v += R[d + D * b] * s[b];
+      p[d] = v;  // R x_j
+      xp[d] = v + t[d];
+    }
+    double RS[9];
+    for (int a1 = 0; a1 < D; ++a1)
+      for (int c1 = 0; c1 < D; ++c1) {
+        double v = 0;
+        for (int b = 0; b < D; ++b) v += R[a1 + D * b] * s[D + b + D * c1];
+        RS[a1 + D * c1] = v;
+      }
+    for (int a1 = 0; a1 < D; ++a1)
+      for (int c1 = 0; c1 < D; ++c1) {
+        double v = 0;
+        for (int b = 0; b < D; ++b) v += RS[a1 + D * b] * R[c1 + D * b];
+        Sp[a1 + D * c1] = v;
+      }
+    (void)xp;
+    for (int d = 0; d < D; ++d) gt[d] += g[d];
+    // H S' (used by the rotation and dR gradients)
+    double HS[9], SH[9];
+    for (int a1 = 0; a1 < D; ++a1)
+      for (int c1 = 0; c1 < D; ++c1) {
+        double v = 0, w = 0;
+        for (int b = 0; b < D; ++b) {
+          v += H[a1 + D * b] * Sp[b + D * c1];
+          w += Sp[a1 + D * b] * H[b + D * c1];
+        }
+        HS[a1 + D * c1] = v;
+        SH[a1 + D * c1] = w;
+      }
+    // dE/d dR at identity: g p^T + 2 H S'
+    for (int a1 = 0; a1 < D; ++a1)
+      for (int c1 = 0; c1 < D; ++c1) G[a1 + D * c1] += g[a1] * p[c1] + 2.0 * HS[a1 + D * c1];
+    if (D == 3) {
+      gw[0] += p[1] * g[2] - p[2] * g[1];
+      gw[1] += p[2] * g[0] - p[0] * g[2];
+      gw[2] += p[0] * g[1] - p[1] * g[0];
+      // -2 vee(S'H - HS')
+      gw[0] -= 2.0 * (SH[2 + 3 * 1] - HS[2 + 3 * 1]);
+      gw[1] -= 2.0 * (SH[0 + 3 * 2] - HS[0 + 3 * 2]);
+      gw[2] -= 2.0 * (SH[1 + 3 * 0] - HS[1 + 3 * 0]);
+    } else {
+      gth += g[1] * p[0] - g[0] * p[1];
+      // tr(H (J S' - S' J)) = tr((S'H - HS') J) = 2 (S'H - HS')_12
+      gth += 2.0 * (SH[0 + 2 * 1] - HS[0 + 2 * 1]);
+    }
+    const int j = c->hSidx[k];
+    if (gY)
+      for (int b = 0; b < D; ++b) {
+        double v = 0;
+        for (int a1 = 0; a1 < D; ++a1) v += R[a1 + D * b] * g[a1];
+        gY[(size_t)j * D + b] = v;
+      }
+    if (gYC) {
+      double *o = gYC + (size_t)j * DD;
+      for (int b = 0; b < D; ++b)
+        for (int c1 = 0; c1 < D; ++c1) {
+          double v = 0;
+          for (int a1 = 0; a1 < D; ++a1)
+            for (int e = 0; e < D; ++e) v += R[a1 + D * b] * H[a1 + D * e] * R[e + D * c1];
+          o[b + D * c1] = v;
+        }
+    }
+  }
+  if (gpose) {
+    if (D == 3) {
+      for (int a = 0; a < 3; ++a) {
+        gpose[a] = gw[a];
+        gpose[3 + a] = gt[a];
+      }
+    } else {
+      gpose[0] = gth;
+      gpose[1] = gt[0];
+      gpose[2] = gt[1];
+    }
+  }
+  if (gdT) {
+    if (D == 3) {
+      // energy3D.m variable order: t1 t2 t3 r11 r12 r13 r21 ... r33 (dR row-major)
+      for (int a = 0; a < 3; ++a) gdT[a] = gt[a];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) gdT[3 + 3 * a + b] = G[a + 3 * b];
+    } else {
+      // energy2D.m: t1 t2 r11 r21 with dR = [[r11, -r21], [r21, r11]]
+      gdT[0] = gt[0];
+      gdT[1] = gt[1];
+      gdT[2] = G[0] + G[3];
+      gdT[3] = G[1] - G[2];
+    }
+  }
+  return DUGMA_OK;
+}

@@ -1,0 +1,125 @@
+// dugma_internal.h -- shared definitions of the gfx950 DUGMA library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dugma.h"
+
+namespace dugma {
+
+// ---------------------------------------------------------------------------
+// Point records.  A point's position is kept as an FP32 pair p = hi + lo
+// (hi = fl32(p), lo = fl32(p - hi)), so pair differences of nearby points
+// keep ~48 bits although every per-pair operation is FP32.  The records are
+// read wave-uniformly by the pair kernels (scalar loads into SGPRs), so each
+// is a power-of-two size and 16-byte aligned.
+// ---------------------------------------------------------------------------
+struct alignas(16) Rec3F {  // 3-D, full symmetric covariance, 64 B
+  float hx, hy, hz, s11;
+  float lx, ly, lz, s22;
+  float s12, s13, s23, s33;
+  float pad0, pad1, pad2, pad3;
+};
+struct alignas(16) Rec3I {  // 3-D, isotropic covariance s*I, 32 B
+  float hx, hy, hz, s;
+  float lx, ly, lz, pad0;
+};
+struct alignas(16) Rec2F {  // 2-D, full symmetric covariance, 32 B
+  float hx, hy, lx, ly;
+  float s11, s12, s22, pad0;
+};
+struct alignas(16) Rec2I {  // 2-D, isotropic covariance s*I, 32 B
+  float hx, hy, lx, ly;
+  float s, pad0, pad1, pad2;
+};
+
+// Per-stationary-point accumulator layout of the pair kernels (FP64):
+//  [0]      energy sum_j e_ij                    (without the (2pi)^-D/2 factor)
+//  [1..D]   EW = sum_j e_ij w_ij,  w = S^-1 d,  d = p - q
+//  full covariance:  WW = sum_j e w w^T (D(D+1)/2), C = sum_j e adj(S)/det
+//  isotropic:        G = sum_j (e/s)(|d|^2/s - D)
+enum { kAcc3F = 16, kAcc3I = 5, kAcc2F = 9, kAcc2I = 4 };
+
+// Pose-gradient partial per workgroup: [E, dofs...], padded to 8 doubles.
+enum { kPosePart = 8 };
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define DUGMA_HIP(call)                                   \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return ::dugma::hip_fail(e_, #call); \
+  } while (0)
+
+// Timing bracket around a pair-kernel launch (no-op unless timing is on);
+// finish() must run after the stream has been synchronised.
+struct PairTimer {
+  dugma_ctx *c;
+  explicit PairTimer(dugma_ctx *ctx);
+  void stop();
+  void finish();
+};
+
+// Grow-only device scratch buffer owned by a context.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+};
+
+}  // namespace dugma
+
+// ---------------------------------------------------------------------------
+// Context (opaque to callers).  The first six device arrays are the state the
+// reference keeps behind its six MEX handles (init.cu:326-331).
+// ---------------------------------------------------------------------------
+struct dugma_ctx {
+  int N = 0, M = 0, D = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // reference state (FP32, as after init.cu)
+  float *dX = nullptr;      // N*D
+  float *dXCinv = nullptr;  // N*D*D  inverse covariances, column-major
+  float *dY = nullptr;      // M*D
+  float *dYCinv = nullptr;  // M*D*D
+  float *dgX = nullptr;     // N      sqrtf(1/det)
+  float *dgY = nullptr;     // M
+
+  // convolved-energy state: valid points only
+  int nT = 0, nS = 0;        // kept targets / sources
+  bool iso = false;          // every kept covariance is s^2 I
+  int *dTidx = nullptr;      // nT: original target index of each kept point
+  int *dSidx = nullptr;      // nS
+  void *dTrecF = nullptr;    // nT full-covariance target records
+  void *dTrecI = nullptr;    // nT isotropic target records (iso only)
+  double *dSsrc = nullptr;   // nS * (D + D*D) FP64 source points + covariances
+  void *dSrecF = nullptr;    // nS per-call transformed source records
+  void *dSrecI = nullptr;
+
+  std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
+  std::vector<double> hSsrc;      // host copy of dSsrc
+  std::vector<double> hTpts;      // kept targets: point (D) + covariance (D*D)
+
+  // optional kernel timing (HIP events on `stream`)
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pair_ms = 0.0;
+  long long pair_launches = 0;
+
+  dugma::DevBuf part;        // pair-kernel partials
+  dugma::DevBuf scratch2;    // reductions / k-NN merge
+  double *hpinned = nullptr; // pinned host staging, kHostStage doubles
+  static constexpr int kHostStage = 4096;
+};

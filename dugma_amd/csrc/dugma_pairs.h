@@ -1,0 +1,226 @@
+// dugma_pairs.h -- per-pair arithmetic of the convolved-uncertainty energy
+// and its adjoint, for the four point-record kinds.
+//
+// For a stationary point p (in VGPRs) and a streamed point q (wave-uniform,
+// in SGPRs): d = p - q, S = Sp + Sq,
+//   e = det(S)^-1/2 exp(-d^T S^-1 d / 2)          (times (2 pi)^-D/2 later)
+// and, with w = S^-1 d, the adjoint accumulators
+//   EW += e w,  WW += e w w^T,  C += e S^-1        (full covariance)
+//   EW += e w,  G  += (e/s)(|d|^2/s - D)          (isotropic, S = s I)
+// from which dE/dp = -c EW and dE/dSp = c (WW - C) / 2, dE/ds_p = c G / 2.
+// FP32 throughout; one v_rsq (or v_rcp) and one v_exp per pair.
+#pragma once
+
+#include "dugma_internal.h"
+
+namespace dugma {
+
+constexpr float kNegHalfLog2e = -0.72134752044448170368f;  // -log2(e)/2
+
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <class Rec> struct RecTraits;
+template <> struct RecTraits<Rec3F> { static constexpr int D = 3, NACC = kAcc3F; static constexpr bool FULL = true; };
+template <> struct RecTraits<Rec3I> { static constexpr int D = 3, NACC = kAcc3I; static constexpr bool FULL = false; };
+template <> struct RecTraits<Rec2F> { static constexpr int D = 2, NACC = kAcc2F; static constexpr bool FULL = true; };
+template <> struct RecTraits<Rec2I> { static constexpr int D = 2, NACC = kAcc2I; static constexpr bool FULL = false; };
+
+// ---- 3-D, full covariance -------------------------------------------------
+template <bool GRAD, bool COMP>
+__device__ __forceinline__ void pair_eval(const Rec3F &p, const Rec3F &q, float *acc) {
+  float dx = p.hx - q.hx, dy = p.hy - q.hy, dz = p.hz - q.hz;
+  if (COMP) {
+    dx += p.lx - q.lx;
+    dy += p.ly - q.ly;
+    dz += p.lz - q.lz;
+  }
+  const float a11 = p.s11 + q.s11, a12 = p.s12 + q.s12, a13 = p.s13 + q.s13;
+  const float a22 = p.s22 + q.s22, a23 = p.s23 + q.s23, a33 = p.s33 + q.s33;
+  // adjugate (symmetric) and determinant
+  const float c11 = fmaf(a22, a33, -(a23 * a23));
+  const float c12 = fmaf(a13, a23, -(a12 * a33));
+  const float c13 = fmaf(a12, a23, -(a13 * a22));
+  const float c22 = fmaf(a11, a33, -(a13 * a13));
+  const float c23 = fmaf(a12, a13, -(a11 * a23));
+  const float c33 = fmaf(a11, a22, -(a12 * a12));
+  const float det = fmaf(a11, c11, fmaf(a12, c12, a13 * c13));
+  const float u1 = fmaf(c11, dx, fmaf(c12, dy, c13 * dz));
+  const float u2 = fmaf(c12, dx, fmaf(c22, dy, c23 * dz));
+  const float u3 = fmaf(c13, dx, fmaf(c23, dy, c33 * dz));
+  const float qa = fmaf(dx, u1, fmaf(dy, u2, dz * u3));  // d^T adj(S) d
+  const float r = frsq(det);
+  const float r2 = r * r;  // 1/det
+  const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
+  acc[0] += e;
+  if (GRAD) {
+    const float w1 = u1 * r2, w2 = u2 * r2, w3 = u3 * r2;
+    const float ew1 = e * w1, ew2 = e * w2, ew3 = e * w3;
+    acc[1] += ew1;
+    acc[2] += ew2;
+    acc[3] += ew3;
+    acc[4] = fmaf(ew1, w1, acc[4]);
+    acc[5] = fmaf(ew1, w2, acc[5]);
+    acc[6] = fmaf(ew1, w3, acc[6]);
+    acc[7] = fmaf(ew2, w2, acc[7]);
+    acc[8] = fmaf(ew2, w3, acc[8]);
+    acc[9] = fmaf(ew3, w3, acc[9]);
+    const float er2 = e * r2;
+    acc[10] = fmaf(er2, c11, acc[10]);
+    acc[11] = fmaf(er2, c12, acc[11]);
+    acc[12] = fmaf(er2, c13, acc[12]);
+    acc[13] = fmaf(er2, c22, acc[13]);
+    acc[14] = fmaf(er2, c23, acc[14]);
+    acc[15] = fmaf(er2, c33, acc[15]);
+  }
+}
+
+// ---- 3-D, isotropic covariance --------------------------------------------
+template <bool GRAD, bool COMP>
+__device__ __forceinline__ void pair_eval(const Rec3I &p, const Rec3I &q, float *acc) {
+  float dx = p.hx - q.hx, dy = p.hy - q.hy, dz = p.hz - q.hz;
+  if (COMP) {
+    dx += p.lx - q.lx;
+    dy += p.ly - q.ly;
+    dz += p.lz - q.lz;
+  }
+  const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float s = p.s + q.s;
+  const float rs = frsq(s);
+  const float inv = rs * rs;  // 1/s
+  const float e = fexp2((d2 * inv) * kNegHalfLog2e) * (rs * inv);  // s^-3/2
+  acc[0] += e;
+  if (GRAD) {
+    const float einv = e * inv;
+    acc[1] = fmaf(einv, dx, acc[1]);
+    acc[2] = fmaf(einv, dy, acc[2]);
+    acc[3] = fmaf(einv, dz, acc[3]);
+    acc[4] = fmaf(einv, fmaf(d2, inv, -3.0f), acc[4]);
+  }
+}
+
+// ---- 2-D, full covariance -------------------------------------------------
+template <bool GRAD, bool COMP>
+__device__ __forceinline__ void pair_eval(const Rec2F &p, const Rec2F &q, float *acc) {
+  float dx = p.hx - q.hx, dy = p.hy - q.hy;
+  if (COMP) {
+    dx += p.lx - q.lx;
+    dy += p.ly - q.ly;
+  }
+  const float a11 = p.s11 + q.s11, a12 = p.s12 + q.s12, a22 = p.s22 + q.s22;
+  const float det = fmaf(a11, a22, -(a12 * a12));
+  const float u1 = fmaf(a22, dx, -(a12 * dy));
+  const float u2 = fmaf(a11, dy, -(a12 * dx));
+  const float qa = fmaf(dx, u1, dy * u2);
+  const float r = frsq(det);
+  const float r2 = r * r;
+  const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
+  acc[0] += e;
+  if (GRAD) {
+    const float w1 = u1 * r2, w2 = u2 * r2;
+    const float ew1 = e * w1, ew2 = e * w2;
+    acc[1] += ew1;
+    acc[2] += ew2;
+    acc[3] = fmaf(ew1, w1, acc[3]);
+    acc[4] = fmaf(ew1, w2, acc[4]);
+    acc[5] = fmaf(ew2, w2, acc[5]);
+    const float er2 = e * r2;
+    acc[6] = fmaf(er2, a22, acc[6]);
+    acc[7] = fmaf(-er2, a12, acc[7]);
+    acc[8] = fmaf(er2, a11, acc[8]);
+  }
+}
+
+// ---- 2-D, isotropic covariance --------------------------------------------
+template <bool GRAD, bool COMP>
+__device__ __forceinline__ void pair_eval(const Rec2I &p, const Rec2I &q, float *acc) {
+  float dx = p.hx - q.hx, dy = p.hy - q.hy;
+  if (COMP) {
+    dx += p.lx - q.lx;
+    dy += p.ly - q.ly;
+  }
+  const float d2 = fmaf(dx, dx, dy * dy);
+  const float inv = frcp(p.s + q.s);
+  const float e = fexp2((d2 * inv) * kNegHalfLog2e) * inv;  // s^-1
+  acc[0] += e;
+  if (GRAD) {
+    const float einv = e * inv;
+    acc[1] = fmaf(einv, dx, acc[1]);
+    acc[2] = fmaf(einv, dy, acc[2]);
+    acc[3] = fmaf(einv, fmaf(d2, inv, -2.0f), acc[3]);
+  }
+}
+
+// ---- pose-gradient contribution of one stationary target ------------------
+// The energy is invariant under one rigid motion of both clouds, so the
+// gradient for moving the sources (rotation about t, then translation) is
+// minus the gradient for moving the targets the same way:
+//   dE/dt = c sum_i EW_i
+//   dE/dw = c sum_i [ (y_i - t) x EW_i + vee(Sy_i B_i - B_i Sy_i) ],  B = WW - C
+// (2-D: dE/dtheta = c sum_i [ EW^T J (y - t) - ((Sy B)_12 - (B Sy)_12) ]).
+// r: [E, rotation dofs, translation dofs].
+__device__ __forceinline__ void pose_contrib(const Rec3F &p, const double *a,
+                                             const double *t, double *r) {
+  const double px = (double)p.hx + (double)p.lx - t[0];
+  const double py = (double)p.hy + (double)p.ly - t[1];
+  const double pz = (double)p.hz + (double)p.lz - t[2];
+  const double ex = a[1], ey = a[2], ez = a[3];
+  const double b11 = a[4] - a[10], b12 = a[5] - a[11], b13 = a[6] - a[12];
+  const double b22 = a[7] - a[13], b23 = a[8] - a[14], b33 = a[9] - a[15];
+  const double s11 = p.s11, s12 = p.s12, s13 = p.s13, s22 = p.s22, s23 = p.s23, s33 = p.s33;
+  // K = S B - B S (antisymmetric); vee(K) = (K32, K13, K21)
+  const double sb32 = s13 * b12 + s23 * b22 + s33 * b23;  // (S B)_32
+  const double bs32 = b13 * s12 + b23 * s22 + b33 * s23;  // (B S)_32
+  const double sb13 = s11 * b13 + s12 * b23 + s13 * b33;
+  const double bs13 = b11 * s13 + b12 * s23 + b13 * s33;
+  const double sb21 = s12 * b11 + s22 * b12 + s23 * b13;
+  const double bs21 = b12 * s11 + b22 * s12 + b23 * s13;
+  r[0] += a[0];
+  r[1] += py * ez - pz * ey + (sb32 - bs32);
+  r[2] += pz * ex - px * ez + (sb13 - bs13);
+  r[3] += px * ey - py * ex + (sb21 - bs21);
+  r[4] += ex;
+  r[5] += ey;
+  r[6] += ez;
+}
+
+__device__ __forceinline__ void pose_contrib(const Rec3I &p, const double *a,
+                                             const double *t, double *r) {
+  const double px = (double)p.hx + (double)p.lx - t[0];
+  const double py = (double)p.hy + (double)p.ly - t[1];
+  const double pz = (double)p.hz + (double)p.lz - t[2];
+  const double ex = a[1], ey = a[2], ez = a[3];
+  r[0] += a[0];
+  r[1] += py * ez - pz * ey;
+  r[2] += pz * ex - px * ez;
+  r[3] += px * ey - py * ex;
+  r[4] += ex;
+  r[5] += ey;
+  r[6] += ez;
+}
+
+__device__ __forceinline__ void pose_contrib(const Rec2F &p, const double *a,
+                                             const double *t, double *r) {
+  const double px = (double)p.hx + (double)p.lx - t[0];
+  const double py = (double)p.hy + (double)p.ly - t[1];
+  const double ex = a[1], ey = a[2];
+  const double b11 = a[3] - a[6], b12 = a[4] - a[7], b22 = a[5] - a[8];
+  const double s11 = p.s11, s12 = p.s12, s22 = p.s22;
+  r[0] += a[0];
+  r[1] += ey * px - ex * py - ((s11 * b12 + s12 * b22) - (b11 * s12 + b12 * s22));
+  r[2] += ex;
+  r[3] += ey;
+}
+
+__device__ __forceinline__ void pose_contrib(const Rec2I &p, const double *a,
+                                             const double *t, double *r) {
+  const double px = (double)p.hx + (double)p.lx - t[0];
+  const double py = (double)p.hy + (double)p.ly - t[1];
+  r[0] += a[0];
+  r[1] += a[2] * px - a[1] * py;
+  r[2] += a[1];
+  r[3] += a[2];
+}
+
+}  // namespace dugma

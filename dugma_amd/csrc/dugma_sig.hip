@@ -1,0 +1,341 @@
+// dugma_sig.hip -- update_sig (update_sig.cpp / update_sig.cu) and exact
+// k-nearest neighbours with lowest-index tie-break.
+//
+// Both are integer/ordering results, so they are bit-exact: this file is
+// compiled without FP contraction, every FP32 operation is written in a fixed
+// order (explicit fmaf where a fused operation is meant), and the minimum of
+// the quantised distances is order-independent.  Queries sit one per lane;
+// candidates are streamed with wave-uniform addresses (scalar loads).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "dugma_internal.h"
+
+namespace dugma {
+
+constexpr int kSBlock = 256;
+
+struct PoseArgC {
+  double R[9];  // column-major
+  double t[3];
+};
+
+// update_sig.cu:247-252: m_j = fl32(r_a1*y1 + r_a2*y2 (+ r_a3*y3) + t_a), FP64.
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_transform_compat(
+    const float *__restrict__ Y, int M, PoseArgC ps, float *__restrict__ out) {
+  const int j = blockIdx.x * kSBlock + threadIdx.x;
+  if (j >= M) return;
+  const float *y = Y + (size_t)j * D;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double s = ps.R[a] * (double)y[0];
+#pragma unroll
+    for (int b = 1; b < D; ++b) s = s + ps.R[a + D * b] * (double)y[b];
+    s = s + ps.t[a];
+    out[(size_t)j * D + a] = (float)s;
+  }
+}
+
+__global__ __launch_bounds__(kSBlock) void k_fill_u32(unsigned *__restrict__ p,
+                                                      int n, unsigned v) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// __float2uint_rn: round half to even, saturating; NaN and negatives -> 0.
+__device__ __forceinline__ unsigned f2u_rn(float s) {
+  if (!(s > 0.0f)) return 0u;
+  if (s >= 4294967296.0f) return 4294967295u;
+  return (unsigned)rintf(s);
+}
+
+// update_sig.cu:238-265: per target the minimum over a chunk of sources of
+// rn_u32(fl32(fl32(|n - m|^2) / D) * 1e6), merged with atomicMin (the
+// minimum does not depend on the order).
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_nn1_quant(
+    const float *__restrict__ X, int N, const float *__restrict__ Mp, int M,
+    int chunk, unsigned *__restrict__ qmin) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  const int j0 = blockIdx.y * chunk;
+  const int j1 = min(j0 + chunk, M);
+  const int ii = min(i, N - 1);
+  float n[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) n[a] = X[(size_t)ii * D + a];
+  const float DD = (float)D;
+  unsigned best = 4000000000u;
+  for (int j = j0; j < j1; ++j) {
+    const float *m = Mp + (size_t)j * D;
+    const float e0 = n[0] - m[0], e1 = n[1] - m[1];
+    float s2 = e0 * e0 + e1 * e1;
+    if (D == 3) {
+      const float e2 = n[2] - m[2];
+      s2 = s2 + e2 * e2;
+    }
+    float s = s2 / DD;
+    s = s * 1000000.0f;
+    best = min(best, f2u_rn(s));
+  }
+  if (i < N) atomicMin(qmin + i, best);
+}
+
+// update_sig.cu:79-91: fl32(q) / 1e6 (IEEE division here).
+__global__ __launch_bounds__(kSBlock) void k_convert_back(
+    const unsigned *__restrict__ q, int n, float *__restrict__ f) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i < n) f[i] = ((float)q[i]) / 1000000.0f;
+}
+
+// update_sig.cu:291-307 with n_c = 1: pairwise tree, level by level; at each
+// level element 2k*stride absorbs element (2k+1)*stride, an odd last element
+// is carried.  One workgroup; returns sum / N (update_sig.cu:385).
+__global__ __launch_bounds__(1024) void k_tree_sum_div(float *__restrict__ a,
+                                                       int n,
+                                                       float *__restrict__ out) {
+  long L = n, stride = 1;
+  while (L > 1) {
+    const long half = (L + 1) / 2;
+    const long npairs = (L & 1) ? half - 1 : half;
+    for (long k = threadIdx.x; k < npairs; k += blockDim.x)
+      a[k * 2 * stride] = a[k * 2 * stride] + a[k * 2 * stride + stride];
+    __syncthreads();
+    L = half;
+    stride *= 2;
+  }
+  if (threadIdx.x == 0) out[0] = a[0] / (float)n;
+}
+
+// ---------------------------------------------------------------------------
+// k-NN: running sorted top-K per lane; candidates of a chunk arrive in
+// increasing index, so a candidate enters only when strictly closer, which
+// ranks equal distances by lowest index.
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void topk_insert(float *bd, int *bi, float dd, int j) {
+  if (!(dd < bd[K - 1])) return;
+#pragma unroll
+  for (int s = K - 1; s >= 0; --s) {
+    if (bd[s] > dd) {
+      if (s > 0 && bd[s - 1] > dd) {
+        bd[s] = bd[s - 1];
+        bi[s] = bi[s - 1];
+      } else {
+        bd[s] = dd;
+        bi[s] = j;
+      }
+    }
+  }
+}
+
+template <int D, int K>
+__global__ __launch_bounds__(kSBlock) void k_knn_chunk(
+    const float *__restrict__ Qp, int nq, const float *__restrict__ Pp, int np,
+    int chunk, int self_exclude, float *__restrict__ pd, int *__restrict__ pi) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  const int j0 = blockIdx.y * chunk;
+  const int j1 = min(j0 + chunk, np);
+  const int ii = min(i, nq - 1);
+  float q[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) q[a] = Qp[(size_t)ii * D + a];
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    bd[s] = INFINITY;
+    bi[s] = -1;
+  }
+  for (int j = j0; j < j1; ++j) {
+    const float *p = Pp + (size_t)j * D;
+    const float dx = q[0] - p[0], dy = q[1] - p[1];
+    float dd;
+    if (D == 3) {
+      const float dz = q[2] - p[2];
+      dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    } else {
+      dd = fmaf(dy, dy, dx * dx);
+    }
+    if (self_exclude && j == i) dd = INFINITY;
+    topk_insert<K>(bd, bi, dd, j);
+  }
+  if (i < nq) {
+    const size_t base = ((size_t)blockIdx.y * nq + i) * K;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      pd[base + s] = bd[s];
+      pi[base + s] = bi[s];
+    }
+  }
+}
+
+// Merge the per-chunk lists in chunk order (chunk c holds larger indices
+// than chunk c-1, so strict-less insertion keeps the lowest-index rule).
+template <int K>
+__global__ __launch_bounds__(kSBlock) void k_knn_merge(
+    const float *__restrict__ pd, const int *__restrict__ pi, int nq,
+    int nchunks, int k, float *__restrict__ od, int *__restrict__ oi) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i >= nq) return;
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    bd[s] = INFINITY;
+    bi[s] = -1;
+  }
+  for (int c = 0; c < nchunks; ++c) {
+    const size_t base = ((size_t)c * nq + i) * K;
+    for (int s = 0; s < K; ++s) {
+      const int j = pi[base + s];
+      if (j < 0) break;
+      topk_insert<K>(bd, bi, pd[base + s], j);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (s < k) {
+      od[(size_t)i * k + s] = bd[s];
+      oi[(size_t)i * k + s] = bi[s];
+    }
+}
+
+static int chunks_for(int nq, int np) {
+  const int qblocks = (nq + kSBlock - 1) / kSBlock;
+  int c = (8192 + qblocks * 4 - 1) / (qblocks * 4);
+  c = std::max(1, std::min(c, (np + 255) / 256));
+  return c;
+}
+
+static int transform_compat(dugma_ctx *c, const double *R, const double *t,
+                            float *out) {
+  PoseArgC ps;
+  memset(&ps, 0, sizeof(ps));
+  for (int a = 0; a < c->D * c->D; ++a) ps.R[a] = R[a];
+  for (int a = 0; a < c->D; ++a) ps.t[a] = t[a];
+  dim3 grid((c->M + kSBlock - 1) / kSBlock);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_transform_compat<3>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out);
+  else
+    hipLaunchKernelGGL(k_transform_compat<2>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out);
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+static int check_dims(dugma_ctx *c, double N, double M, double D) {
+  if ((int)N != c->N || (int)M != c->M || (int)D != c->D)
+    return fail(DUGMA_ERR_STATE, "N, M, D do not match the context");
+  return DUGMA_OK;
+}
+
+}  // namespace dugma
+
+using namespace dugma;
+
+extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t,
+                                double N, double M, double D, double *sig) {
+  if (!ctx || !R || !t || !sig) return fail(DUGMA_ERR_ARG, "dugma_update_sig: null argument");
+  int rc = check_dims(ctx, N, M, D);
+  if (rc) return rc;
+  dugma_ctx *c = ctx;
+  // scratch: m (M*D floats) | q (N u32) | f (N floats) | result
+  const size_t off_q = ((size_t)c->M * c->D * 4 + 255) & ~(size_t)255;
+  const size_t off_f = off_q + (((size_t)c->N * 4 + 255) & ~(size_t)255);
+  const size_t off_r = off_f + (((size_t)c->N * 4 + 255) & ~(size_t)255);
+  rc = c->scratch2.ensure(off_r + 256);
+  if (rc) return rc;
+  char *base = (char *)c->scratch2.ptr;
+  float *m = (float *)base;
+  unsigned *q = (unsigned *)(base + off_q);
+  float *f = (float *)(base + off_f);
+  float *res = (float *)(base + off_r);
+  rc = transform_compat(c, R, t, m);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_u32, dim3((c->N + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, c->stream, q, c->N, 4000000000u);
+  const int nch = chunks_for(c->N, c->M);
+  const int chunk = (c->M + nch - 1) / nch;
+  const int nchunks = (c->M + chunk - 1) / chunk;
+  dim3 grid((c->N + kSBlock - 1) / kSBlock, nchunks);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_nn1_quant<3>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
+  else
+    hipLaunchKernelGGL(k_nn1_quant<2>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
+  hipLaunchKernelGGL(k_convert_back, dim3((c->N + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, c->stream, q, c->N, f);
+  hipLaunchKernelGGL(k_tree_sum_div, dim3(1), dim3(1024), 0, c->stream, f, c->N, res);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipMemcpyAsync(c->hpinned, res, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  float s;
+  memcpy(&s, c->hpinned, sizeof(float));
+  *sig = (double)s;
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
+                         const double *t, int *idx, float *d2) {
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_knn: null context");
+  if (k < 1 || k > 32) return fail(DUGMA_ERR_ARG, "dugma_knn: k must be in 1..32");
+  dugma_ctx *c = ctx;
+  const int D = c->D;
+  const float *Qp;
+  const float *Pp;
+  int nq, np, self = 0;
+  size_t off = 0;
+  // scratch layout: [transformed candidates] | partial d | partial idx | out d | out idx
+  size_t need_m = 0;
+  if (which == DUGMA_KNN_TARGET_SELF) {
+    Qp = c->dX; nq = c->N; Pp = c->dX; np = c->N; self = 1;
+  } else if (which == DUGMA_KNN_SOURCE_SELF) {
+    Qp = c->dY; nq = c->M; Pp = c->dY; np = c->M; self = 1;
+  } else if (which == DUGMA_KNN_TARGET_TO_SOURCE) {
+    if (!R || !t) return fail(DUGMA_ERR_ARG, "dugma_knn: R and t are required");
+    Qp = c->dX; nq = c->N; Pp = nullptr; np = c->M;
+    need_m = ((size_t)c->M * D * 4 + 255) & ~(size_t)255;
+  } else {
+    return fail(DUGMA_ERR_ARG, "dugma_knn: unknown query set");
+  }
+  const int K = k <= 8 ? 8 : (k <= 16 ? 16 : 32);
+  const int nch = chunks_for(nq, np);
+  const int chunk = (np + nch - 1) / nch;
+  const int nchunks = (np + chunk - 1) / chunk;
+  const size_t part = (size_t)nchunks * nq * K * 4;
+  const size_t part_al = (part + 255) & ~(size_t)255;
+  const size_t outsz = (((size_t)nq * k * 4) + 255) & ~(size_t)255;
+  int rc = c->scratch2.ensure(need_m + 2 * part_al + 2 * outsz);
+  if (rc) return rc;
+  char *base = (char *)c->scratch2.ptr;
+  if (need_m) {
+    rc = transform_compat(c, R, t, (float *)base);
+    if (rc) return rc;
+    Pp = (const float *)base;
+  }
+  off = need_m;
+  float *pd = (float *)(base + off);
+  int *pi = (int *)(base + off + part_al);
+  float *od = (float *)(base + off + 2 * part_al);
+  int *oi = (int *)(base + off + 2 * part_al + outsz);
+  dim3 grid((nq + kSBlock - 1) / kSBlock, nchunks), block(kSBlock);
+#define KNN_LAUNCH(DD, KK)                                                              \
+  hipLaunchKernelGGL((k_knn_chunk<DD, KK>), grid, block, 0, c->stream, Qp, nq, Pp, np, \
+                     chunk, self, pd, pi)
+  if (D == 3) {
+    if (K == 8) KNN_LAUNCH(3, 8); else if (K == 16) KNN_LAUNCH(3, 16); else KNN_LAUNCH(3, 32);
+  } else {
+    if (K == 8) KNN_LAUNCH(2, 8); else if (K == 16) KNN_LAUNCH(2, 16); else KNN_LAUNCH(2, 32);
+  }
+#undef KNN_LAUNCH
+  dim3 mgrid((nq + kSBlock - 1) / kSBlock);
+  if (K == 8)
+    hipLaunchKernelGGL(k_knn_merge<8>, mgrid, block, 0, c->stream, pd, pi, nq, nchunks, k, od, oi);
+  else if (K == 16)
+    hipLaunchKernelGGL(k_knn_merge<16>, mgrid, block, 0, c->stream, pd, pi, nq, nchunks, k, od, oi);
+  else
+    hipLaunchKernelGGL(k_knn_merge<32>, mgrid, block, 0, c->stream, pd, pi, nq, nchunks, k, od, oi);
+  DUGMA_HIP(hipGetLastError());
+  if (d2) DUGMA_HIP(hipMemcpyAsync(d2, od, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
+  if (idx) DUGMA_HIP(hipMemcpyAsync(idx, oi, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  return DUGMA_OK;
+}

@@ -1,0 +1,157 @@
+/*
+ * dugma.h -- C ABI of the MI355X (gfx950) DUGMA registration hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns a status code
+ * (DUGMA_OK = 0, negative on error) and never terminates the process;
+ * dugma_last_error() returns the message of the calling thread's last error.
+ *
+ * Layout conventions (those of the reference's MEX gateways):
+ *  - a point cloud is D-by-N column-major double, i.e. x,y[,z] of point 0,
+ *    then point 1, ... (init.cpp:43-60, mxGetPr of X);
+ *  - covariances are D-by-(D*N) column-major double: the D-by-D block of
+ *    point k is contiguous, column-major (dataset_all/Read Me:7-8);
+ *  - R is D-by-D column-major double, t has D doubles;
+ *  - "target" is the reference's fixed cloud X (N points), "source" is its
+ *    moving cloud Y (M points); the model is X ~ R*Y + t (DUGMA.m:4-7).
+ *
+ * Threading: a context may be used by one host thread at a time; calls on
+ * one context are serialised on the context's own HIP stream.
+ */
+#ifndef DUGMA_H
+#define DUGMA_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dugma_ctx dugma_ctx;
+
+enum {
+  DUGMA_OK = 0,
+  DUGMA_ERR_ARG = -1,     /* bad argument (null pointer, D not 2/3, sizes) */
+  DUGMA_ERR_DEVICE = -2,  /* HIP runtime error or no device */
+  DUGMA_ERR_ALLOC = -3,   /* device or pinned allocation failed */
+  DUGMA_ERR_SIZE = -4,    /* point count beyond the supported maximum */
+  DUGMA_ERR_STATE = -5    /* mismatch with the context (N, M or D) */
+};
+
+/* Largest point count per cloud accepted by dugma_init (the reference caps
+ * at 3,000,000 and exits: compi.cu:535-539, update_sig.cu:341-345). */
+#define DUGMA_MAX_POINTS 16777216
+
+/* ---- reference MEX entry points ---------------------------------------- */
+
+/* [ok, h0..h5] = init(X, X_C, Y, Y_C)   (init.cpp:36-83, init.cu:249-348).
+ * Uploads both clouds, keeps the FP32 points, FP32 inverse covariances and
+ * g = sqrtf(1/det C) of the reference (det <= 0 or NaN: inverse 100000 in
+ * every entry, g = 0), plus the data the convolved energy needs.  Writes
+ * the new context to *out; *ok (may be NULL) receives 1.0 like init's first
+ * output. */
+int dugma_init(const double *X, const double *X_C, const double *Y,
+               const double *Y_C, int N, int M, int D, dugma_ctx **out,
+               double *ok);
+
+/* [sig, h0..h5] = update_sig(h0..h5, R, t, N, M, D)
+ * (update_sig.cpp:8-66, update_sig.cu:319-394).  N, M, D are passed as
+ * doubles like the MEX arguments and must match the context.
+ * sig = (1/N) sum_i min_j |X_i - fl32(R Y_j + t)|^2 / D, each minimum
+ * quantised to 1e-6 (round to nearest, capped at 4000), summed in FP32 in
+ * the reference's pairwise order. */
+int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t,
+                     double N, double M, double D, double *sig);
+
+/* [coef, h0..h5] = compi(h0..h5, R, t, N, M, D, sig)
+ * (compi.cpp:8-74, compi.cu:511-628).  Writes the 637 (3-D) or 45 (2-D)
+ * coefficients of the surrogate polynomial in energy3D.m:15 / energy2D.m:7
+ * monomial order. */
+int dugma_compi(dugma_ctx *ctx, const double *R, const double *t, double N,
+                double M, double D, double sig, double *coef);
+
+/* ok = fr_m(h0..h5)   (fr_m.cpp:8-35, fr_m.cu:14-31).  Frees the context's
+ * device memory.  Unlike the reference it does not reset the device. */
+int dugma_fr_m(dugma_ctx *ctx, double *ok);
+
+/* ---- convolved-uncertainty energy (no reference counterpart) ----------- */
+
+/* Evaluation modes for the energy entry points. */
+enum {
+  DUGMA_COV_AUTO = 0, /* isotropic kernels when every covariance is s^2 I */
+  DUGMA_COV_FULL = 1, /* full-covariance kernels always */
+  DUGMA_COV_ISO = 2   /* isotropic kernels; error unless all are s^2 I */
+};
+
+/* E(R,t) = sum_i sum_j N(X_i - (R Y_j + t); CX_i + R CY_j R^T), per-pair
+ * FP32 with FP64 accumulation.  Points whose coordinates or covariance are
+ * not finite, or whose covariance is not positive definite, are left out. */
+int dugma_energy(dugma_ctx *ctx, const double *R, const double *t, int cov_mode,
+                 double *E);
+
+/* Energy and its gradient with respect to the pose from one fused
+ * forward+adjoint pass (the optimiser's gradient callback).
+ * gpose (3-D): [dE/dw1 dE/dw2 dE/dw3 dE/dt1 dE/dt2 dE/dt3] for the left
+ * perturbation R <- exp([w]x) R, t <- t + dt; (2-D): [dE/dtheta dE/dt1 dE/dt2]. */
+int dugma_energy_grad(dugma_ctx *ctx, const double *R, const double *t,
+                      int cov_mode, double *E, double *gpose);
+
+/* Energy with per-point gradients (no N x M storage: one target-major and
+ * one source-major pass).  Any output may be NULL.
+ *  gX  : N x D      dE/dX_i            gY  : M x D      dE/dY_j
+ *  gXC : N x D x D  dE/dCX_i           gYC : M x D x D  dE/dCY_j
+ * (covariance gradients column-major per point, entries treated as
+ * independent variables; zero for left-out points).
+ *  gdT : 12 (3-D) or 4 (2-D) doubles, dE with respect to the fmincon
+ *        variables dT of energy3D.m / energy2D.m at dT = identity, where the
+ *        pose is updated as R <- dR R, t <- t + dt (DUGMA.m:67-78). */
+int dugma_energy_grad_points(dugma_ctx *ctx, const double *R, const double *t,
+                             int cov_mode, double *E, double *gpose,
+                             double *gX, double *gXC, double *gY, double *gYC,
+                             double *gdT);
+
+/* "Plain" gradient: central finite differences of dugma_energy over the pose
+ * (2*dofs + 1 forward launches), step h.  Same gpose layout. */
+int dugma_energy_grad_fd(dugma_ctx *ctx, const double *R, const double *t,
+                         int cov_mode, double h, double *E, double *gpose);
+
+/* ---- k nearest neighbours ---------------------------------------------- */
+
+enum {
+  DUGMA_KNN_TARGET_SELF = 0, /* queries X, candidates X, self excluded */
+  DUGMA_KNN_SOURCE_SELF = 1, /* queries Y, candidates Y, self excluded */
+  DUGMA_KNN_TARGET_TO_SOURCE = 2 /* queries X, candidates fl32(R Y_j + t) */
+};
+
+/* k <= 32 nearest neighbours with FP32 squared distances computed as
+ * fmaf(dz,dz,fmaf(dy,dy,dx*dx)) on the FP32 points of init, ranked by
+ * (distance, index) so that ties go to the lowest index.  idx: nq x k int,
+ * d2: nq x k float (either may be NULL); missing neighbours are -1 / +inf.
+ * R, t are used by DUGMA_KNN_TARGET_TO_SOURCE only. */
+int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
+              const double *t, int *idx, float *d2);
+
+/* ---- the six reference handles ------------------------------------------ */
+/* Device pointers of the state the reference keeps behind its six MEX handles
+ * (init.cu:326-331, in that order): X, inverse X_C, Y, inverse Y_C, g_X, g_Y
+ * (FP32; sizes N*D, N*D*D, M*D, M*D*D, N, M).  ptrs must hold 6 entries. */
+int dugma_ctx_handles(const dugma_ctx *ctx, void **ptrs);
+/* Copy handle `which` (0..5, order as above) to host memory `out`. */
+int dugma_read_handle(dugma_ctx *ctx, int which, float *out);
+
+/* ---- context queries ---------------------------------------------------- */
+int dugma_ctx_dims(const dugma_ctx *ctx, int *N, int *M, int *D);
+/* Numbers of points kept for the energy (after leaving out invalid ones) and
+ * whether every kept covariance is isotropic. */
+int dugma_ctx_energy_info(const dugma_ctx *ctx, int *n_target, int *n_source,
+                          int *isotropic);
+/* The HIP stream (hipStream_t) the context launches on, for event timing. */
+void *dugma_ctx_stream(const dugma_ctx *ctx);
+/* Kernel timing: when on, HIP events are recorded on the context's stream
+ * around every launch of the all-pairs kernels; dugma_ctx_pair_time returns
+ * the summed milliseconds and launch count since timing was switched on. */
+int dugma_ctx_set_timing(dugma_ctx *ctx, int on);
+int dugma_ctx_pair_time(const dugma_ctx *ctx, double *ms, long long *launches);
+const char *dugma_last_error(void);
+const char *dugma_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""GPU parity of the convolved energy and its adjoints against the FP64 oracle.
+
+Bars: energy relative error <= 1e-6; gradients (pose, per point, per
+covariance) relative error <= 1e-5 in the norm of each output array.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import dugma_amd as dg
+from dugma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-6
+G_TOL = 1e-5
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def as_oracle(d):
+    """MATLAB-layout workload -> oracle arrays (point-major, (n,D,D))."""
+    return d["X"].T, W.unblocks(d["X_C"]), d["Y"].T, W.unblocks(d["Y_C"])
+
+
+def perturbed_pose(d, deg=3.0, dt=0.01, seed=0):
+    rng = np.random.default_rng(seed)
+    D = d["X"].shape[0]
+    if D == 3:
+        R = W.random_rotation(rng, deg) @ d["R_G"]
+    else:
+        R = W.rot2(math.radians(deg)) @ d["R_G"]
+    return R, np.ravel(d["t_G"]) + rng.uniform(-dt, dt, size=D)
+
+
+def check_energy(h, d, R, t, oracle, cov="auto"):
+    Y, SY, X, SX = as_oracle(d)
+    ref = oracle.energy(Y, SY, X, SX, R, t, grads=True)
+    E = dg.energy(h, R, t, cov)
+    assert abs(E / ref["E"] - 1) <= E_TOL, (E, ref["E"])
+    E2, g = dg.energy_grad(h, R, t, cov)
+    assert abs(E2 / ref["E"] - 1) <= E_TOL
+    assert rel(g, ref["gpose"]) <= G_TOL, (g, ref["gpose"])
+    return ref
+
+
+def test_energy_config2_full_covariance(oracle):
+    d = W.config2(n=900, m=700)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    assert ok == 1.0
+    try:
+        nT, nS, iso = h.energy_info()
+        assert (nT, nS, iso) == (900, 700, False)
+        for s in range(3):
+            R, t = perturbed_pose(d, deg=1.0 + s, dt=0.003, seed=s)
+            check_energy(h, d, R, t, oracle)
+    finally:
+        dg.fr_m(h)
+
+
+def test_energy_grad_points_config2(oracle):
+    d = W.config2(n=500, m=400)
+    R, t = perturbed_pose(d, deg=2.0, dt=0.002, seed=7)
+    Y, SY, X, SX = as_oracle(d)
+    ref = oracle.energy(Y, SY, X, SX, R, t, grads=True)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        out = dg.energy_grad_points(h, R, t)
+    finally:
+        dg.fr_m(h)
+    assert abs(out["E"] / ref["E"] - 1) <= E_TOL
+    assert rel(out["gpose"], ref["gpose"]) <= G_TOL
+    assert rel(out["gX"], ref["gY"]) <= G_TOL  # library X = target = oracle Y
+    assert rel(out["gXC"], ref["gSY"]) <= G_TOL
+    assert rel(out["gY"], ref["gX"]) <= G_TOL
+    assert rel(out["gYC"], ref["gSX"]) <= G_TOL
+    # dE/d(dT) of energy3D.m's variables: central differences of the FP64
+    # oracle along R <- dR R, t <- t + dt (DUGMA.m:76-78)
+    hstep = 1e-7
+    gd = np.zeros(12)
+    base = np.concatenate([np.zeros(3), np.eye(3).ravel()])
+    for k in range(12):
+        vals = []
+        for sgn in (1, -1):
+            v = base.copy()
+            v[k] += sgn * hstep
+            dR = v[3:].reshape(3, 3)
+            vals.append(oracle.energy(Y, SY, X, SX, dR @ R, t + v[:3])["E"])
+        gd[k] = (vals[0] - vals[1]) / (2 * hstep)
+    assert rel(out["gdT"], gd) <= 1e-4  # FD truncation/rounding of the oracle itself
+
+
+def test_energy_isotropic_config3_small(oracle):
+    d = W.config3(n=3000, m=2500)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        assert h.energy_info()[2]  # isotropic path
+        R, t = perturbed_pose(d, deg=2.0, dt=0.005, seed=3)
+        check_energy(h, d, R, t, oracle, "auto")
+        check_energy(h, d, R, t, oracle, "full")  # full kernels on isotropic data
+        check_energy(h, d, R, t, oracle, "iso")
+    finally:
+        dg.fr_m(h)
+
+
+def test_energy_2d(oracle):
+    d = W.config5_2d(n=1500, m=1300)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = perturbed_pose(d, deg=4.0, dt=0.01, seed=5)
+        check_energy(h, d, R, t, oracle, "auto")
+        check_energy(h, d, R, t, oracle, "full")
+    finally:
+        dg.fr_m(h)
+    # full 2-D covariances
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(800, 2, 2)) * 0.01
+    CX = A @ np.transpose(A, (0, 2, 1)) + 1e-5 * np.eye(2)
+    B = rng.normal(size=(600, 2, 2)) * 0.01
+    CY = B @ np.transpose(B, (0, 2, 1)) + 1e-5 * np.eye(2)
+    d2 = dict(X=d["X"][:, :800], X_C=W.blocks(CX), Y=d["Y"][:, :600], Y_C=W.blocks(CY),
+              R_G=d["R_G"], t_G=d["t_G"])
+    ok, h = dg.init(d2["X"], d2["X_C"], d2["Y"], d2["Y_C"])
+    try:
+        R, t = perturbed_pose(d2, deg=2.0, dt=0.005, seed=6)
+        ref = check_energy(h, d2, R, t, oracle)
+        out = dg.energy_grad_points(h, R, t)
+        assert rel(out["gX"], ref["gY"]) <= G_TOL
+        assert rel(out["gYC"], ref["gSX"]) <= G_TOL
+        assert rel(out["gpose"], ref["gpose"]) <= G_TOL
+    finally:
+        dg.fr_m(h)
+
+
+def test_energy_invalid_points_left_out(oracle):
+    """NaN covariances (as in Kinect scenes 2, 11, 22) and non-PD ones."""
+    g = np.load(os.path.join(HERE, "golden", "kinect_fixtures.npz"))
+    for k in (2, 11, 22):
+        d = dict(X=g["s%d_X" % k], X_C=g["s%d_X_C" % k], Y=g["s%d_Y" % k], Y_C=g["s%d_Y_C" % k],
+                 R_G=g["s%d_R_G" % k], t_G=g["s%d_t_G" % k])
+        XC = W.unblocks(d["X_C"]).copy()
+        XC[1] = np.diag([1.0, -1.0, 1.0])  # not positive definite
+        d["X_C"] = W.blocks(XC)
+        ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+        try:
+            nT, nS, _ = h.energy_info()
+            assert nT < d["X"].shape[1] or nS < d["Y"].shape[1]
+            check_energy(h, d, d["R_G"], d["t_G"].ravel(), oracle)
+        finally:
+            dg.fr_m(h)
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (1, 777), (333, 1), (257, 65), (4099, 129)])
+def test_energy_ragged_sizes(oracle, n, m):
+    rng = np.random.default_rng(n * 7 + m)
+    X = rng.normal(size=(3, n)) * 0.2
+    Y = rng.normal(size=(3, m)) * 0.2
+    A = rng.normal(size=(n, 3, 3)) * 0.05
+    B = rng.normal(size=(m, 3, 3)) * 0.05
+    CX = A @ np.transpose(A, (0, 2, 1)) + 1e-3 * np.eye(3)
+    CY = B @ np.transpose(B, (0, 2, 1)) + 1e-3 * np.eye(3)
+    d = dict(X=X, X_C=W.blocks(CX), Y=Y, Y_C=W.blocks(CY), R_G=np.eye(3), t_G=np.zeros(3))
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        check_energy(h, d, W.random_rotation(rng, 10), rng.normal(size=3) * 0.01, oracle)
+    finally:
+        dg.fr_m(h)
+
+
+def test_fd_gradient_is_consistent_with_fused(oracle):
+    d = W.config3(n=2000, m=2000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = perturbed_pose(d, deg=3.0, dt=0.01, seed=2)
+        E, g = dg.energy_grad(h, R, t)
+        E2, gfd = dg.energy_grad_fd(h, R, t, step=1e-4)
+        assert abs(E2 / E - 1) < 1e-12
+        assert rel(gfd, g) < 1e-3  # FD truncation at h = 1e-4 and FP32 rounding
+    finally:
+        dg.fr_m(h)
+
+
+def test_deterministic_repeats():
+    d = W.config2(n=700, m=650)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = perturbed_pose(d, seed=1)
+        a = dg.energy_grad(h, R, t)
+        b = dg.energy_grad(h, R, t)
+        assert a[0] == b[0] and (a[1] == b[1]).all()
+    finally:
+        dg.fr_m(h)

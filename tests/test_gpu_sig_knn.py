@@ -1,0 +1,115 @@
+"""GPU parity of init (the six reference handles), update_sig and k-NN:
+bit-exact against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import dugma_amd as dg
+from dugma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def kinect(k):
+    g = np.load(os.path.join(HERE, "golden", "kinect_fixtures.npz"))
+    return dict(X=g["s%d_X" % k], X_C=g["s%d_X_C" % k], Y=g["s%d_Y" % k], Y_C=g["s%d_Y_C" % k],
+                R_G=g["s%d_R_G" % k], t_G=g["s%d_t_G" % k].ravel())
+
+
+@pytest.mark.parametrize("k", [2, 20])
+def test_init_handles_bit_exact(oracle, k):
+    d = kinect(k)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        assert ok == 1.0
+        np.testing.assert_array_equal(h.read_handle(0), d["X"].T.astype(np.float32).ravel())
+        np.testing.assert_array_equal(h.read_handle(2), d["Y"].T.astype(np.float32).ravel())
+        for which, P in ((1, d["X_C"]), (3, d["Y_C"])):
+            inv, g, _ = oracle.prep_inverse(W.unblocks(P), 3)
+            np.testing.assert_array_equal(h.read_handle(which), inv.ravel())
+            np.testing.assert_array_equal(h.read_handle(which + 3), g)
+    finally:
+        dg.fr_m(h)
+
+
+def test_init_handles_2d(oracle):
+    d = W.config5_2d(n=500, m=400)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        inv, g, _ = oracle.prep_inverse(W.unblocks(d["X_C"]), 2)
+        np.testing.assert_array_equal(h.read_handle(1), inv.ravel())
+        np.testing.assert_array_equal(h.read_handle(4), g)
+    finally:
+        dg.fr_m(h)
+
+
+@pytest.mark.parametrize("k", [2, 11, 20, 22])
+def test_update_sig_kinect_bit_exact(oracle, k):
+    d = kinect(k)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        for R, t in ((np.eye(3), np.zeros(3)), (d["R_G"], d["t_G"])):
+            s = dg.update_sig(h, R, t, h.N, h.M, 3)
+            ref = oracle.update_sig(d["X"].T, d["Y"].T, R, t)
+            assert s == ref
+    finally:
+        dg.fr_m(h)
+
+
+def test_update_sig_2d_and_mismatch(oracle):
+    d = W.config5_2d(n=2000, m=1700)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        s = dg.update_sig(h, d["R_G"], d["t_G"], h.N, h.M, 2)
+        assert s == oracle.update_sig(d["X"].T, d["Y"].T, d["R_G"], d["t_G"])
+        with pytest.raises(dg.DugmaError):
+            dg.update_sig(h, d["R_G"], d["t_G"], h.N + 1, h.M, 2)
+    finally:
+        dg.fr_m(h)
+
+
+def test_update_sig_larger(oracle):
+    d = W.config3(n=30000, m=20000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        s = dg.update_sig(h, d["R_G"], d["t_G"], h.N, h.M, 3)
+        assert s == oracle.update_sig(d["X"].T, d["Y"].T, d["R_G"], d["t_G"])
+    finally:
+        dg.fr_m(h)
+
+
+@pytest.mark.parametrize("which", [dg.KNN_TARGET_SELF, dg.KNN_SOURCE_SELF, dg.KNN_TARGET_TO_SOURCE])
+def test_knn_bit_exact(oracle, which):
+    d = W.config3(n=12000, m=9000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = d["R_G"], d["t_G"]
+        idx, d2 = dg.knn(h, which, 8, R, t)
+        Xf, Yf = d["X"].T.astype(np.float32), d["Y"].T.astype(np.float32)
+        if which == dg.KNN_TARGET_SELF:
+            ri, rd = oracle.knn(Xf, Xf, 8, True)
+        elif which == dg.KNN_SOURCE_SELF:
+            ri, rd = oracle.knn(Yf, Yf, 8, True)
+        else:
+            ri, rd = oracle.knn(Xf, oracle.transform_points(Yf, R, t), 8, False)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+    finally:
+        dg.fr_m(h)
+
+
+@pytest.mark.parametrize("k", [1, 8, 13, 32])
+def test_knn_ties_lowest_index(oracle, k):
+    rng = np.random.default_rng(k)
+    X = rng.integers(0, 5, size=(3, 3000)).astype(float)  # many equal distances
+    C = W.iso_blocks(np.full(3000, 0.01))
+    ok, h = dg.init(X, C, X[:, :1000], C[:, :3000])
+    try:
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_SELF, k)
+        ri, rd = oracle.knn(X.T.astype(np.float32), X.T.astype(np.float32), k, True)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+    finally:
+        dg.fr_m(h)
