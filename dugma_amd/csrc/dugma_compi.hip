@@ -1,12 +1,622 @@
-// dugma_compi.hip -- compi (compi.cpp / compi.cu): surrogate coefficients.
+// dugma_compi.hip -- compi (compi.cpp:8-74, compi.cu:511-628): coefficients
+// of the surrogate polynomial
+//   F(dR, dt) = sum_ij P_ij [ u^T A'_j u + d^T B_i d ],
+//   d = n_i - dR m'_j - dt,  u = dR^T d,
+// with P_ij = gX_i gY_j (exp(-d0^T C_j d0 / 2) + exp(-d0^T B_i d0 / 2)),
+// d0 = n_i - R y_j - t, C_j = R A_j^T R^T, m'_j = fl32(R y_j + t),
+// A'_j = fl32(C_j), A_j = fl32(inv(CY_j) Isig), B_i = fl32(inv(CX_i) Isig)
+// (compi.cu:642-791).  Written in energy3D.m:15 / energy2D.m:7 order.
+//
+// Instead of expanding 637 terms per pair (and 637 FP64 atomics, as
+// compi.cu:800-3403 does), every coefficient is a sum over pairs of P_ij
+// times (a feature of target i) times (a feature of source j).  So:
+//  1. k_compi_w: for every source j, W_j = sum_i P_ij G_i over the 20 (3-D)
+//     or 12 (2-D) target features G = [1, n, n n^T, n^T B n, (B+B^T) n,
+//     sym(B)]; sources sit in lanes, targets stream through SGPRs.
+//  2. k_compi_moments: 325 (85) moments sum_j W'_j,f * m'_j.. * A'_j,..
+//  3. host: the moments are expanded into the polynomial in FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "dugma_internal.h"
+#include "dugma_monomials.inc"
+
+namespace dugma {
+
+constexpr int kCBlock = 256;
+constexpr int kCSub = 64;
+constexpr int kFeat = 32;  // per-source feature stride (doubles) for the moments
+constexpr float kNegHalfLog2eC = -0.72134752044448170368f;
+
+struct CompiPose {
+  double R[9];  // column-major
+  double t[3];
+  float Isig;   // FP32 1/sig (compi.cu:646 round-to-nearest, :110 round-down in 2-D)
+  float gsc;    // sqrtf(Isig^3) in 3-D (compi.cu:662), Isig in 2-D (compi.cu:126)
+};
+
+// Streamed target record: n, sym(B), gX, then the features n n^T, n^T B n,
+// (B+B^T) n.  3-D: 3+6+1+6+1+3 = 20 floats (padded to 32).  2-D: 2+3+1+3+1+2.
+struct alignas(16) TgtRec3 {
+  float n[3], b[6], g;
+  float nn[6], s, v[3];
+  float pad[12];
+};
+struct alignas(16) TgtRec2 {
+  float n[2], b[3], g;
+  float nn[3], s, v[2];
+  float pad[4];
+};
+
+// sym index of (a,b), a <= b: 3-D 00 01 02 11 12 22 ; 2-D 00 01 11
+__host__ __device__ constexpr int sidx3(int a, int b) {
+  return a <= b ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : sidx3(b, a);
+}
+__host__ __device__ constexpr int sidx2(int a, int b) { return a <= b ? a + b : a + b; }
+template <int D>
+__host__ __device__ constexpr int sidx(int a, int b) { return D == 3 ? sidx3(a, b) : sidx2(a, b); }
+
+// Per-call target records.  B = fl32(inv(CX) * Isig) (compi.cu:655-658).
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_tgt(
+    const float *__restrict__ X, const float *__restrict__ XCinv,
+    const float *__restrict__ gX, int N, CompiPose ps, void *__restrict__ outv) {
+  const int i = blockIdx.x * kCBlock + threadIdx.x;
+  if (i >= N) return;
+  float B[9];
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) B[q] = XCinv[(size_t)i * D * D + q] * ps.Isig;
+  double n[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) n[a] = (double)X[(size_t)i * D + a];
+  double s = 0.0, v[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double va = 0.0;
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      s += n[a] * (double)B[a + D * b] * n[b];
+      va += ((double)B[a + D * b] + (double)B[b + D * a]) * n[b];
+    }
+    v[a] = va;
+  }
+  const float g = gX[i] * ps.gsc;
+  if (D == 3) {
+    TgtRec3 r;
+    for (int a = 0; a < 3; ++a) {
+      r.n[a] = (float)n[a];
+      r.v[a] = (float)v[a];
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = a; b < 3; ++b) {
+        r.b[sidx3(a, b)] = (float)(0.5 * ((double)B[a + 3 * b] + (double)B[b + 3 * a]));
+        r.nn[sidx3(a, b)] = (float)(n[a] * n[b]);
+      }
+    r.g = g;
+    r.s = (float)s;
+    for (int q = 0; q < 12; ++q) r.pad[q] = 0.0f;
+    reinterpret_cast<TgtRec3 *>(outv)[i] = r;
+  } else {
+    TgtRec2 r;
+    for (int a = 0; a < 2; ++a) {
+      r.n[a] = (float)n[a];
+      r.v[a] = (float)v[a];
+    }
+    for (int a = 0; a < 2; ++a)
+      for (int b = a; b < 2; ++b) {
+        r.b[sidx2(a, b)] = (float)(0.5 * ((double)B[a + 2 * b] + (double)B[b + 2 * a]));
+        r.nn[sidx2(a, b)] = (float)(n[a] * n[b]);
+      }
+    r.g = g;
+    r.s = (float)s;
+    for (int q = 0; q < 4; ++q) r.pad[q] = 0.0f;
+    reinterpret_cast<TgtRec2 *>(outv)[i] = r;
+  }
+}
+
+// Per-call source state: m' = R y + t (FP64; hi = fl32(m') is the reference's
+// m', lo keeps the rest for the pair difference), C = R A^T R^T (FP64, 3-D;
+// R A R^T in 2-D as compi.cu:180-183 writes it), gY.
+// src (floats per source): 3-D [mh 3, ml 3, C sym 6, gY, pad 3] = 16
+//                          2-D [mh 2, ml 2, C sym 3, gY] = 8
+// feat (doubles per source, kFeat stride): [.. W set later .., m' at FM, A' at FA]
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_src(
+    const float *__restrict__ Y, const float *__restrict__ YCinv,
+    const float *__restrict__ gY, int M, CompiPose ps, float *__restrict__ src,
+    double *__restrict__ feat) {
+  const int j = blockIdx.x * kCBlock + threadIdx.x;
+  if (j >= M) return;
+  constexpr int SS = D == 3 ? 16 : 8;
+  constexpr int FM = D == 3 ? 20 : 12;
+  constexpr int FA = FM + D;
+  float A[9];
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) A[q] = YCinv[(size_t)j * D * D + q] * ps.Isig;
+  double C[9];
+#pragma unroll
+  for (int p = 0; p < D; ++p)
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        double inner = 0.0;
+#pragma unroll
+        for (int l = 0; l < D; ++l) {
+          // 3-D: C = R A^T R^T  (a_lk r_ql); 2-D: C = R A R^T (a_kl r_ql)
+          const double a = (D == 3) ? (double)A[l + D * k] : (double)A[k + D * l];
+          inner += a * ps.R[q + D * l];
+        }
+        acc += ps.R[p + D * k] * inner;
+      }
+      C[p + D * q] = acc;
+    }
+  float *o = src + (size_t)j * SS;
+  double *f = feat + (size_t)j * kFeat;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double v = ps.R[a] * (double)Y[(size_t)j * D];
+#pragma unroll
+    for (int b = 1; b < D; ++b) v = v + ps.R[a + D * b] * (double)Y[(size_t)j * D + b];
+    v = v + ps.t[a];
+    const float hi = (float)v;
+    o[a] = hi;
+    o[D + a] = (float)(v - (double)hi);
+    f[FM + a] = (double)hi;
+  }
+  int k = 2 * D;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = a; b < D; ++b) o[k++] = (float)(0.5 * (C[a + D * b] + C[b + D * a]));
+  o[k++] = gY[j] * ps.gsc;
+  for (; k < SS; ++k) o[k] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) f[FA + q] = (double)(float)C[q];
+}
+
+// W_j = sum_i P_ij G_i.  Lane = source j (stationary), targets streamed.
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_w(
+    const float *__restrict__ src, int M, const void *__restrict__ tgtv, int N,
+    int chunk, double *__restrict__ part) {
+  constexpr int NW = D == 3 ? 20 : 12;
+  constexpr int SS = D == 3 ? 16 : 8;
+  const int j = blockIdx.x * kCBlock + threadIdx.x;
+  const int jj = min(j, M - 1);
+  const float *o = src + (size_t)jj * SS;
+  float mh[3], ml[3], C[6];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    mh[a] = o[a];
+    ml[a] = o[D + a];
+  }
+#pragma unroll
+  for (int q = 0; q < D * (D + 1) / 2; ++q) C[q] = o[2 * D + q];
+  const int i0 = blockIdx.y * chunk;
+  const int i1 = min(i0 + chunk, N);
+  double W[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) W[k] = 0.0;
+  for (int s0 = i0; s0 < i1; s0 += kCSub) {
+    float w[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = 0.0f;
+    const int s1 = min(s0 + kCSub, i1);
+    for (int i = s0; i < s1; ++i) {
+      if (D == 3) {
+        const TgtRec3 t = reinterpret_cast<const TgtRec3 *>(tgtv)[i];
+        const float dx = (t.n[0] - mh[0]) - ml[0];
+        const float dy = (t.n[1] - mh[1]) - ml[1];
+        const float dz = (t.n[2] - mh[2]) - ml[2];
+        // mal1 = d^T C d, mal2 = d^T B d
+        const float c1 = fmaf(C[0], dx, fmaf(C[1], dy, C[2] * dz));
+        const float c2 = fmaf(C[1], dx, fmaf(C[3], dy, C[4] * dz));
+        const float c3 = fmaf(C[2], dx, fmaf(C[4], dy, C[5] * dz));
+        const float mal1 = fmaf(dx, c1, fmaf(dy, c2, dz * c3));
+        const float b1 = fmaf(t.b[0], dx, fmaf(t.b[1], dy, t.b[2] * dz));
+        const float b2 = fmaf(t.b[1], dx, fmaf(t.b[3], dy, t.b[4] * dz));
+        const float b3 = fmaf(t.b[2], dx, fmaf(t.b[4], dy, t.b[5] * dz));
+        const float mal2 = fmaf(dx, b1, fmaf(dy, b2, dz * b3));
+        const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
+                               __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
+        w[0] += P;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) w[4 + a] = fmaf(P, t.nn[a], w[4 + a]);
+        w[10] = fmaf(P, t.s, w[10]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[11 + a] = fmaf(P, t.v[a], w[11 + a]);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) w[14 + a] = fmaf(P, t.b[a], w[14 + a]);
+      } else {
+        const TgtRec2 t = reinterpret_cast<const TgtRec2 *>(tgtv)[i];
+        const float dx = (t.n[0] - mh[0]) - ml[0];
+        const float dy = (t.n[1] - mh[1]) - ml[1];
+        const float mal1 = fmaf(dx, fmaf(C[0], dx, C[1] * dy), dy * fmaf(C[1], dx, C[2] * dy));
+        const float mal2 = fmaf(dx, fmaf(t.b[0], dx, t.b[1] * dy), dy * fmaf(t.b[1], dx, t.b[2] * dy));
+        const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
+                               __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
+        w[0] += P;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[3 + a] = fmaf(P, t.nn[a], w[3 + a]);
+        w[6] = fmaf(P, t.s, w[6]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) w[7 + a] = fmaf(P, t.v[a], w[7 + a]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[9 + a] = fmaf(P, t.b[a], w[9 + a]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NW; ++k) W[k] += (double)w[k];
+  }
+  if (j < M) {
+    double *out = part + ((size_t)blockIdx.y * M + j) * NW;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) out[k] = W[k];
+  }
+}
+
+// Sum the W partials over target chunks (fixed order), times gY_j, into the
+// feature rows.
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_wsum(
+    const double *__restrict__ part, int M, int nchunks,
+    const float *__restrict__ src, double *__restrict__ feat) {
+  constexpr int NW = D == 3 ? 20 : 12;
+  constexpr int SS = D == 3 ? 16 : 8;
+  constexpr int GI = D == 3 ? 12 : 7;  // gY position in the source record
+  const size_t idx = (size_t)blockIdx.x * kCBlock + threadIdx.x;
+  if (idx >= (size_t)M * NW) return;
+  const size_t j = idx / NW, k = idx % NW;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += part[((size_t)c * M + j) * NW + k];
+  feat[j * kFeat + k] = s * (double)src[j * SS + GI];
+}
+
+// Moments: mom[k] = sum_j prod of up to four feature entries of source j
+// (index -1 = 1).  One thread per moment, sources in chunks per workgroup;
+// per-workgroup partials summed later in a fixed order.
+struct MomTab {
+  short ix[400][4];
+};
+
+__global__ __launch_bounds__(kCBlock) void k_compi_moments(
+    const double *__restrict__ feat, int M, int chunk, int nmom, MomTab tab,
+    double *__restrict__ part) {
+  const int j0 = blockIdx.x * chunk;
+  const int j1 = min(j0 + chunk, M);
+  for (int k = threadIdx.x; k < nmom; k += kCBlock) {
+    const int a = tab.ix[k][0], b = tab.ix[k][1], c = tab.ix[k][2], d = tab.ix[k][3];
+    double s = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      const double *f = feat + (size_t)j * kFeat;
+      double v = f[a];
+      if (b >= 0) v *= f[b];
+      if (c >= 0) v *= f[c];
+      if (d >= 0) v *= f[d];
+      s += v;
+    }
+    part[(size_t)blockIdx.x * nmom + k] = s;
+  }
+}
+
+__global__ __launch_bounds__(kCBlock) void k_compi_msum(
+    const double *__restrict__ part, int nblocks, int nmom, double *__restrict__ out) {
+  const int k = blockIdx.x * kCBlock + threadIdx.x;
+  if (k >= nmom) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nmom + k];
+  out[k] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Host side: the moment list and its polynomial expansion.
+// ---------------------------------------------------------------------------
+enum MomKind { kM0, kM1, kM2, kM3, kM4, kM5, kK1, kK2, kK3, kK4, kK5, kK6 };
+
+struct Mom {
+  MomKind kind;
+  int i0, i1, i2, i3;  // index meaning depends on kind
+  short ix[4];         // feature indices multiplied on the GPU
+};
+
+// Feature layout per source (doubles): W at 0.., m' at FM, A' (D x D,
+// column-major) at FA.  W: [1][n D][nn sym][s][v D][S sym].
+struct FeatLayout {
+  int D, W1, Wn, Wnn, Ws, Wv, WS, FM, FA, nsym;
+  explicit FeatLayout(int d) : D(d) {
+    nsym = d * (d + 1) / 2;
+    W1 = 0;
+    Wn = 1;
+    Wnn = Wn + d;
+    Ws = Wnn + nsym;
+    Wv = Ws + 1;
+    WS = Wv + d;
+    FM = WS + nsym;
+    FA = FM + d;
+  }
+  int sym(int a, int b) const { return D == 3 ? sidx3(a, b) : sidx2(a, b); }
+};
+
+static std::vector<Mom> moment_list(int D) {
+  FeatLayout L(D);
+  std::vector<Mom> v;
+  auto add = [&](MomKind k, int i0, int i1, int i2, int i3, int a, int b, int c, int d) {
+    Mom m;
+    m.kind = k;
+    m.i0 = i0; m.i1 = i1; m.i2 = i2; m.i3 = i3;
+    m.ix[0] = (short)a; m.ix[1] = (short)b; m.ix[2] = (short)c; m.ix[3] = (short)d;
+    v.push_back(m);
+  };
+  auto A = [&](int b, int c) { return L.FA + b + D * c; };
+  auto m_ = [&](int f) { return L.FM + f; };
+  // d^T B d part
+  add(kM0, 0, 0, 0, 0, L.Ws, -1, -1, -1);
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) add(kM1, a, b, 0, 0, L.Wv + a, m_(b), -1, -1);
+  for (int a = 0; a < D; ++a) add(kM2, a, 0, 0, 0, L.Wv + a, -1, -1, -1);
+  for (int a = 0; a < D; ++a)
+    for (int c = a; c < D; ++c)
+      for (int b = 0; b < D; ++b)
+        for (int d = b; d < D; ++d) add(kM3, a, c, b, d, L.WS + L.sym(a, c), m_(b), m_(d), -1);
+  for (int a = 0; a < D; ++a)
+    for (int c = a; c < D; ++c)
+      for (int d = 0; d < D; ++d) add(kM4, a, c, d, 0, L.WS + L.sym(a, c), m_(d), -1, -1);
+  for (int a = 0; a < D; ++a)
+    for (int c = a; c < D; ++c) add(kM5, a, c, 0, 0, L.WS + L.sym(a, c), -1, -1, -1);
+  // u^T A' u part
+  for (int a = 0; a < D; ++a)
+    for (int e = a; e < D; ++e)
+      for (int b = 0; b < D; ++b)
+        for (int c = 0; c < D; ++c) add(kK1, a, e, b, c, L.Wnn + L.sym(a, e), A(b, c), -1, -1);
+  for (int x = 0; x < D; ++x)
+    for (int b = 0; b < D; ++b)
+      for (int c = 0; c < D; ++c) add(kK2, x, b, c, 0, L.Wn + x, A(b, c), -1, -1);
+  for (int x = 0; x < D; ++x)
+    for (int f = 0; f < D; ++f)
+      for (int b = 0; b < D; ++b)
+        for (int c = 0; c < D; ++c) add(kK3, x, f, b, c, L.Wn + x, m_(f), A(b, c), -1);
+  for (int b = 0; b < D; ++b)
+    for (int c = 0; c < D; ++c) add(kK4, b, c, 0, 0, L.W1, A(b, c), -1, -1);
+  for (int f = 0; f < D; ++f)
+    for (int b = 0; b < D; ++b)
+      for (int c = 0; c < D; ++c) add(kK5, f, b, c, 0, L.W1, m_(f), A(b, c), -1);
+  for (int f = 0; f < D; ++f)
+    for (int g = f; g < D; ++g)
+      for (int b = 0; b < D; ++b)
+        for (int c = 0; c < D; ++c) add(kK6, f, g, b, c, L.W1, m_(f), m_(g), A(b, c));
+  return v;
+}
+
+// Polynomial in the update variables, keyed by the sorted variable list.
+struct Poly {
+  int nv;
+  std::vector<double> c;
+  explicit Poly(int n) : nv(n), c((size_t)(n + 1) * (n + 1) * (n + 1) * (n + 1), 0.0) {}
+  int key(int *v, int deg) const {
+    std::sort(v, v + deg);
+    int k = 0;
+    for (int q = 0; q < 4; ++q) k = k * (nv + 1) + (q < deg ? v[q] : nv);
+    return k;
+  }
+};
+
+// A signed single variable: dt_a or an entry of dR.
+struct SVar {
+  int var;
+  double sgn;
+};
+
+struct Vars {
+  int D;
+  SVar dt(int a) const { return {a, 1.0}; }
+  SVar dR(int a, int b) const {
+    if (D == 3) return {3 + 3 * a + b, 1.0};
+    // DUGMA.m:67: dR = [[r11, -r21], [r21, r11]], r11 = var 2, r21 = var 3
+    static const int vv[2][2] = {{2, 3}, {3, 2}};
+    static const double ss[2][2] = {{1.0, -1.0}, {1.0, 1.0}};
+    return {vv[a][b], ss[a][b]};
+  }
+};
+
+static void addt(Poly &P, double coef, std::initializer_list<SVar> vs) {
+  int v[4], deg = 0;
+  double s = coef;
+  for (const SVar &x : vs) {
+    v[deg++] = x.var;
+    s *= x.sgn;
+  }
+  P.c[P.key(v, deg)] += s;
+}
+
+static void expand(int D, const std::vector<Mom> &moms, const double *val, Poly &P) {
+  Vars V{D};
+  for (size_t k = 0; k < moms.size(); ++k) {
+    const Mom &m = moms[k];
+    const double x = val[k];
+    if (x == 0.0) continue;
+    switch (m.kind) {
+      case kM0:
+        addt(P, x, {});
+        break;
+      case kM1:  // - sum_ab dR_ab m_b v_a
+        addt(P, -x, {V.dR(m.i0, m.i1)});
+        break;
+      case kM2:  // - dt_a v_a
+        addt(P, -x, {V.dt(m.i0)});
+        break;
+      case kM3: {  // dR_ab dR_cd S_ac m_b m_d over ordered (a,c), (b,d)
+        const int a = m.i0, c = m.i1, b = m.i2, d = m.i3;
+        for (int p = 0; p < (a == c ? 1 : 2); ++p)
+          for (int q = 0; q < (b == d ? 1 : 2); ++q) {
+            const int aa = p ? c : a, cc = p ? a : c, bb = q ? d : b, dd = q ? b : d;
+            addt(P, x, {V.dR(aa, bb), V.dR(cc, dd)});
+          }
+        break;
+      }
+      case kM4: {  // 2 dt_a dR_cd S_ac m_d over ordered (a,c)
+        const int a = m.i0, c = m.i1, d = m.i2;
+        for (int p = 0; p < (a == c ? 1 : 2); ++p) {
+          const int aa = p ? c : a, cc = p ? a : c;
+          addt(P, 2.0 * x, {V.dt(aa), V.dR(cc, d)});
+        }
+        break;
+      }
+      case kM5: {  // dt_a dt_c S_ac over ordered (a,c)
+        const int a = m.i0, c = m.i1;
+        for (int p = 0; p < (a == c ? 1 : 2); ++p) addt(P, x, {V.dt(p ? c : a), V.dt(p ? a : c)});
+        break;
+      }
+      case kK1: {  // dR_ab dR_ec nn_ae A'_bc over ordered (a,e)
+        const int a = m.i0, e = m.i1, b = m.i2, c = m.i3;
+        for (int p = 0; p < (a == e ? 1 : 2); ++p)
+          addt(P, x, {V.dR(p ? e : a, b), V.dR(p ? a : e, c)});
+        break;
+      }
+      case kK2: {  // -(dR_xb dR_ec dt_e + dR_ab dR_xc dt_a)
+        const int xx = m.i0, b = m.i1, c = m.i2;
+        for (int e = 0; e < D; ++e) addt(P, -x, {V.dR(xx, b), V.dR(e, c), V.dt(e)});
+        for (int a = 0; a < D; ++a) addt(P, -x, {V.dR(a, b), V.dR(xx, c), V.dt(a)});
+        break;
+      }
+      case kK3: {  // -(dR_xb dR_ec dR_ef + dR_ab dR_xc dR_af)
+        const int xx = m.i0, f = m.i1, b = m.i2, c = m.i3;
+        for (int e = 0; e < D; ++e) addt(P, -x, {V.dR(xx, b), V.dR(e, c), V.dR(e, f)});
+        for (int a = 0; a < D; ++a) addt(P, -x, {V.dR(a, b), V.dR(xx, c), V.dR(a, f)});
+        break;
+      }
+      case kK4: {  // dR_ab dR_ec dt_a dt_e
+        const int b = m.i0, c = m.i1;
+        for (int a = 0; a < D; ++a)
+          for (int e = 0; e < D; ++e) addt(P, x, {V.dR(a, b), V.dR(e, c), V.dt(a), V.dt(e)});
+        break;
+      }
+      case kK5: {  // dR_ab dR_ec (dt_a dR_ef + dR_af dt_e)
+        const int f = m.i0, b = m.i1, c = m.i2;
+        for (int a = 0; a < D; ++a)
+          for (int e = 0; e < D; ++e) {
+            addt(P, x, {V.dR(a, b), V.dR(e, c), V.dt(a), V.dR(e, f)});
+            addt(P, x, {V.dR(a, b), V.dR(e, c), V.dR(a, f), V.dt(e)});
+          }
+        break;
+      }
+      case kK6: {  // dR_ab dR_ec dR_af dR_eg over ordered (f,g)
+        const int f = m.i0, g = m.i1, b = m.i2, c = m.i3;
+        for (int p = 0; p < (f == g ? 1 : 2); ++p) {
+          const int ff = p ? g : f, gg = p ? f : g;
+          for (int a = 0; a < D; ++a)
+            for (int e = 0; e < D; ++e) addt(P, x, {V.dR(a, b), V.dR(e, c), V.dR(a, ff), V.dR(e, gg)});
+        }
+        break;
+      }
+    }
+  }
+}
+
+static int check_dims(dugma_ctx *c, double N, double M, double D) {
+  if ((int)N != c->N || (int)M != c->M || (int)D != c->D)
+    return fail(DUGMA_ERR_STATE, "N, M, D do not match the context");
+  return DUGMA_OK;
+}
+
+}  // namespace dugma
 
 using namespace dugma;
 
 extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
                            double N, double M, double D, double sig,
                            double *coef) {
-  (void)N; (void)M; (void)D; (void)sig;
   if (!ctx || !R || !t || !coef) return fail(DUGMA_ERR_ARG, "dugma_compi: null argument");
-  return fail(DUGMA_ERR_STATE, "dugma_compi: not built yet");
+  int rc = check_dims(ctx, N, M, D);
+  if (rc) return rc;
+  if (!(sig > 0) || !std::isfinite(sig)) return fail(DUGMA_ERR_ARG, "dugma_compi: sig must be positive");
+  dugma_ctx *c = ctx;
+  const int Dd = c->D, n = c->N, m = c->M;
+  CompiPose ps;
+  memset(&ps, 0, sizeof(ps));
+  for (int a = 0; a < Dd * Dd; ++a) ps.R[a] = R[a];
+  for (int a = 0; a < Dd; ++a) ps.t[a] = t[a];
+  const double is = 1.0 / sig;
+  float f = (float)is;
+  if (Dd == 2) {
+    if ((double)f > is) f = std::nextafter(f, -INFINITY);  // __double2float_rd
+    ps.Isig = f;
+    ps.gsc = f;
+  } else {
+    ps.Isig = f;
+    ps.gsc = std::sqrt(f * f * f);  // FP32 as compi.cu:662
+  }
+
+  const std::vector<Mom> moms = moment_list(Dd);
+  const int nmom = (int)moms.size();
+  const int NW = Dd == 3 ? 20 : 12;
+  const size_t trec = Dd == 3 ? sizeof(TgtRec3) : sizeof(TgtRec2);
+  const int SS = Dd == 3 ? 16 : 8;
+  // launch shapes
+  const int sblocks = (m + kCBlock - 1) / kCBlock;
+  int nch = (8192 + sblocks * 4 - 1) / (sblocks * 4);
+  nch = std::max(1, std::min(nch, std::min(64, (n + kCSub - 1) / kCSub)));
+  const int chunk = (n + nch - 1) / nch;
+  const int nchunks = (n + chunk - 1) / chunk;
+  const int mchunk = std::max(64, (m + 1023) / 1024);
+  const int mblocks = (m + mchunk - 1) / mchunk;
+  // scratch: tgt records | src | feat | W partials | moment partials | moments
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_src = al(trec * n);
+  const size_t o_feat = o_src + al(sizeof(float) * SS * (size_t)m);
+  const size_t o_part = o_feat + al(sizeof(double) * kFeat * (size_t)m);
+  const size_t o_mp = o_part + al(sizeof(double) * (size_t)nchunks * m * NW);
+  const size_t o_mom = o_mp + al(sizeof(double) * (size_t)mblocks * nmom);
+  rc = c->scratch2.ensure(o_mom + al(sizeof(double) * nmom));
+  if (rc) return rc;
+  char *base = (char *)c->scratch2.ptr;
+  void *tgt = base;
+  float *src = (float *)(base + o_src);
+  double *feat = (double *)(base + o_feat);
+  double *wpart = (double *)(base + o_part);
+  double *mpart = (double *)(base + o_mp);
+  double *mom = (double *)(base + o_mom);
+  MomTab tab;
+  memset(&tab, 0xff, sizeof(tab));
+  for (int k = 0; k < nmom; ++k)
+    for (int q = 0; q < 4; ++q) tab.ix[k][q] = moms[k].ix[q];
+
+  dim3 blk(kCBlock);
+  if (Dd == 3) {
+    hipLaunchKernelGGL(k_compi_tgt<3>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
+    hipLaunchKernelGGL(k_compi_src<3>, dim3(sblocks), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_w<3>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
+    hipLaunchKernelGGL(k_compi_wsum<3>, dim3((unsigned)(((size_t)m * NW + kCBlock - 1) / kCBlock)), blk, 0, c->stream, wpart, m, nchunks, src, feat);
+  } else {
+    hipLaunchKernelGGL(k_compi_tgt<2>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
+    hipLaunchKernelGGL(k_compi_src<2>, dim3(sblocks), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_w<2>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
+    hipLaunchKernelGGL(k_compi_wsum<2>, dim3((unsigned)(((size_t)m * NW + kCBlock - 1) / kCBlock)), blk, 0, c->stream, wpart, m, nchunks, src, feat);
+  }
+  hipLaunchKernelGGL(k_compi_moments, dim3(mblocks), blk, 0, c->stream, feat, m, mchunk, nmom, tab, mpart);
+  hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCBlock - 1) / kCBlock), blk, 0, c->stream, mpart, mblocks, nmom, mom);
+  DUGMA_HIP(hipGetLastError());
+  std::vector<double> hm(nmom);
+  DUGMA_HIP(hipMemcpyAsync(hm.data(), mom, sizeof(double) * nmom, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+
+  const int nv = Dd == 3 ? 12 : 4;
+  Poly P(nv);
+  expand(Dd, moms, hm.data(), P);
+  const int nm = Dd == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
+  for (int k = 0; k < nm; ++k) {
+    const unsigned char *e = Dd == 3 ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
+    int v[4], deg = 0;
+    for (int x = 0; x < nv; ++x)
+      for (int r = 0; r < e[x]; ++r) v[deg++] = x;
+    coef[k] = P.c[P.key(v, deg)];
+  }
+  return DUGMA_OK;
 }

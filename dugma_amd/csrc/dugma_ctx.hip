@@ -146,21 +146,26 @@ static bool isotropic(int D, const double *S) {
          S[7] == 0 && S[0] == S[4] && S[0] == S[8];
 }
 
-static void split(double v, float &hi, float &lo) {
-  hi = (float)v;
-  lo = (float)(v - (double)hi);
-}
-
 static double sym(const double *S, int D, int a, int b) {
   return 0.5 * (S[a + D * b] + S[b + D * a]);
 }
 
-// Host-built target records (the target cloud does not move).
+// symmetric part of the FP32-rounded covariance
+static double sym32(const double *S, int D, int a, int b) {
+  return 0.5 * ((double)(float)S[a + D * b] + (double)(float)S[b + D * a]);
+}
+
+static double f32(double v) { return (double)(float)v; }
+
+// Host-built target records (the target cloud does not move).  Inputs are
+// rounded to FP32 first (as init.cu:192-234 does), then scaled by a power of
+// two, which is exact.
 static void make_target_records(int D, const std::vector<int> &keep,
-                                const double *X, const double *XC,
+                                const double *X, const double *XC, double sc,
                                 std::vector<char> &full, std::vector<char> &iso,
                                 bool want_iso) {
   size_t n = keep.size();
+  const double sc2 = sc * sc;
   if (D == 3) {
     full.assign(n * sizeof(Rec3F), 0);
     Rec3F *r = reinterpret_cast<Rec3F *>(full.data());
@@ -172,19 +177,18 @@ static void make_target_records(int D, const std::vector<int> &keep,
     for (size_t k = 0; k < n; ++k) {
       const double *p = X + (size_t)keep[k] * 3;
       const double *S = XC + (size_t)keep[k] * 9;
-      split(p[0], r[k].hx, r[k].lx);
-      split(p[1], r[k].hy, r[k].ly);
-      split(p[2], r[k].hz, r[k].lz);
-      r[k].s11 = (float)S[0];
-      r[k].s22 = (float)S[4];
-      r[k].s33 = (float)S[8];
-      r[k].s12 = (float)sym(S, 3, 0, 1);
-      r[k].s13 = (float)sym(S, 3, 0, 2);
-      r[k].s23 = (float)sym(S, 3, 1, 2);
+      r[k].hx = (float)(f32(p[0]) * sc);
+      r[k].hy = (float)(f32(p[1]) * sc);
+      r[k].hz = (float)(f32(p[2]) * sc);
+      r[k].s11 = (float)(f32(S[0]) * sc2);
+      r[k].s22 = (float)(f32(S[4]) * sc2);
+      r[k].s33 = (float)(f32(S[8]) * sc2);
+      r[k].s12 = (float)(sym32(S, 3, 0, 1) * sc2);
+      r[k].s13 = (float)(sym32(S, 3, 0, 2) * sc2);
+      r[k].s23 = (float)(sym32(S, 3, 1, 2) * sc2);
       if (ri) {
         ri[k].hx = r[k].hx; ri[k].hy = r[k].hy; ri[k].hz = r[k].hz;
-        ri[k].lx = r[k].lx; ri[k].ly = r[k].ly; ri[k].lz = r[k].lz;
-        ri[k].s = (float)S[0];
+        ri[k].s = r[k].s11;
       }
     }
   } else {
@@ -198,15 +202,14 @@ static void make_target_records(int D, const std::vector<int> &keep,
     for (size_t k = 0; k < n; ++k) {
       const double *p = X + (size_t)keep[k] * 2;
       const double *S = XC + (size_t)keep[k] * 4;
-      split(p[0], r[k].hx, r[k].lx);
-      split(p[1], r[k].hy, r[k].ly);
-      r[k].s11 = (float)S[0];
-      r[k].s22 = (float)S[3];
-      r[k].s12 = (float)sym(S, 2, 0, 1);
+      r[k].hx = (float)(f32(p[0]) * sc);
+      r[k].hy = (float)(f32(p[1]) * sc);
+      r[k].s11 = (float)(f32(S[0]) * sc2);
+      r[k].s22 = (float)(f32(S[3]) * sc2);
+      r[k].s12 = (float)(sym32(S, 2, 0, 1) * sc2);
       if (ri) {
         ri[k].hx = r[k].hx; ri[k].hy = r[k].hy;
-        ri[k].lx = r[k].lx; ri[k].ly = r[k].ly;
-        ri[k].s = (float)S[0];
+        ri[k].s = r[k].s11;
       }
     }
   }
@@ -220,7 +223,7 @@ static void free_ctx(dugma_ctx *c) {
   if (!c) return;
   void *ptrs[] = {c->dX, c->dXCinv, c->dY, c->dYCinv, c->dgX, c->dgY,
                   c->dTidx, c->dSidx, c->dTrecF, c->dTrecI, c->dSsrc,
-                  c->dSrecF, c->dSrecI};
+                  c->dSrecF, c->dSrecI, c->dcounter};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->part.release();
@@ -281,6 +284,12 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
       c->hpinned = nullptr;
       TRY(hip_fail(e, "hipHostMalloc"));
     }
+    c->hres = c->hpinned + dugma_ctx::kHostStage / 2;
+    e = hipHostGetDevicePointer((void **)&c->dres, c->hres, 0);
+    if (e != hipSuccess) TRY(hip_fail(e, "hipHostGetDevicePointer"));
+    TRY(dmalloc(&c->dcounter, 1));
+    e = hipMemsetAsync(c->dcounter, 0, sizeof(unsigned), c->stream);
+    if (e != hipSuccess) TRY(hip_fail(e, "hipMemsetAsync(counter)"));
 
     // ---- reference state (init.cu:264-331) ----
     const int DD = D * D;
@@ -317,23 +326,39 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     }
 
     // ---- convolved-energy state ----
+    // The energy path works on the FP32 points and covariances (as the
+    // reference's device state does), so validity is judged on those.
     std::vector<int> keepT, keepS;
     bool iso = true;
-    for (int i = 0; i < N; ++i)
-      if (valid_point(D, X + (size_t)i * D, X_C + (size_t)i * DD)) {
-        keepT.push_back(i);
-        iso = iso && isotropic(D, X_C + (size_t)i * DD);
+    double logv = 0.0;
+    long nlog = 0;
+    auto consider = [&](const double *P, const double *C, int n, std::vector<int> &keep) {
+      double p32[3], c32[9];
+      for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < D; ++a) p32[a] = f32(P[(size_t)i * D + a]);
+        for (int a = 0; a < DD; ++a) c32[a] = f32(C[(size_t)i * DD + a]);
+        if (!valid_point(D, p32, c32)) continue;
+        keep.push_back(i);
+        iso = iso && isotropic(D, c32);
+        double tr = 0;
+        for (int a = 0; a < D; ++a) tr += c32[a * (D + 1)];
+        logv += std::log2(tr / D);
+        ++nlog;
       }
-    for (int j = 0; j < M; ++j)
-      if (valid_point(D, Y + (size_t)j * D, Y_C + (size_t)j * DD)) {
-        keepS.push_back(j);
-        iso = iso && isotropic(D, Y_C + (size_t)j * DD);
-      }
+    };
+    consider(X, X_C, N, keepT);
+    consider(Y, Y_C, M, keepS);
     c->nT = (int)keepT.size();
     c->nS = (int)keepS.size();
     c->iso = iso && c->nT > 0 && c->nS > 0;
+    // s = 2^k with s^2 * (geometric-mean variance) ~ 1
+    int k = nlog ? (int)std::lround(-0.5 * logv / (double)nlog) : 0;
+    k = std::max(-60, std::min(60, k));
+    c->lscale = k;
+    c->scale = std::ldexp(1.0, k);
+    const double sc = c->scale, sc2 = sc * sc;
     std::vector<char> recF, recI;
-    make_target_records(D, keepT, X, X_C, recF, recI, c->iso);
+    make_target_records(D, keepT, X, X_C, sc, recF, recI, c->iso);
     const size_t recsz = (D == 3) ? sizeof(Rec3F) : sizeof(Rec2F);
     const size_t recszI = (D == 3) ? sizeof(Rec3I) : sizeof(Rec2I);
     const size_t nTa = (size_t)std::max(c->nT, 1), nSa = (size_t)std::max(c->nS, 1);
@@ -345,26 +370,18 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
       TRY(dmalloc((char **)&c->dTrecI, nTa * recszI));
       TRY(dmalloc((char **)&c->dSrecI, nSa * recszI));
     }
-    // source FP64 data: point (D) then symmetric covariance (D*D) per kept point
+    // source data: FP32-rounded point (D) then symmetric covariance (D*D)
+    // per kept point, scaled, held in FP64 for the per-call transform
     std::vector<double> src((size_t)c->nS * (D + DD));
-    for (int k = 0; k < c->nS; ++k) {
-      const double *p = Y + (size_t)keepS[k] * D;
-      const double *S = Y_C + (size_t)keepS[k] * DD;
-      double *o = src.data() + (size_t)k * (D + DD);
-      for (int a = 0; a < D; ++a) o[a] = p[a];
+    for (int q = 0; q < c->nS; ++q) {
+      const double *p = Y + (size_t)keepS[q] * D;
+      const double *S = Y_C + (size_t)keepS[q] * DD;
+      double *o = src.data() + (size_t)q * (D + DD);
+      for (int a = 0; a < D; ++a) o[a] = f32(p[a]) * sc;
       for (int a = 0; a < D; ++a)
-        for (int b = 0; b < D; ++b) o[D + a + D * b] = sym(S, D, a, b);
+        for (int b = 0; b < D; ++b) o[D + a + D * b] = sym32(S, D, a, b) * sc2;
     }
     TRY(dmalloc(&c->dSsrc, std::max<size_t>(src.size(), 1)));
-    c->hTpts.resize((size_t)c->nT * (D + DD));
-    for (int k = 0; k < c->nT; ++k) {
-      const double *p = X + (size_t)keepT[k] * D;
-      const double *S = X_C + (size_t)keepT[k] * DD;
-      double *o = c->hTpts.data() + (size_t)k * (D + DD);
-      for (int a = 0; a < D; ++a) o[a] = p[a];
-      for (int a = 0; a < D; ++a)
-        for (int b = 0; b < D; ++b) o[D + a + D * b] = sym(S, D, a, b);
-    }
     hipError_t e3 = hipSuccess;
     if (c->nT > 0) {
       e3 = hipMemcpy(c->dTidx, keepT.data(), sizeof(int) * c->nT, hipMemcpyHostToDevice);

@@ -33,10 +33,11 @@ struct PoseArg {
 // ---------------------------------------------------------------------------
 // Pair kernel
 // ---------------------------------------------------------------------------
-template <class Rec, int MODE, int T, bool COMP>
+template <class Rec, int MODE, int T, int COMP>
 __global__ __launch_bounds__(kBlock) void k_pairs(
     const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
-    int qchunk, double *__restrict__ out, double t0, double t1, double t2) {
+    int qchunk, double *__restrict__ out, unsigned *__restrict__ counter,
+    double *__restrict__ res, double t0, double t1, double t2) {
   using Tr = RecTraits<Rec>;
   constexpr bool GRAD = MODE != kModeEnergy;
   constexpr int NA = GRAD ? Tr::NACC : 1;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
     for (int j = j0; j < j1; ++j) {
       const Rec q = Q[j];
 #pragma unroll
-      for (int k = 0; k < T; ++k) pair_eval<GRAD, COMP>(p[k], q, acc[k]);
+      for (int k = 0; k < T; ++k) pair_eval<MODE, COMP>(p[k], q, acc[k]);
     }
 #pragma unroll
     for (int k = 0; k < T; ++k)
@@ -111,33 +112,40 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
 #pragma unroll
     for (int a = 0; a < NR; ++a) red[wave][a] = r[a];
   __syncthreads();
+  const unsigned nblocks = gridDim.x * gridDim.y;
   if (threadIdx.x < NR) {
     double s = red[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
     out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart + threadIdx.x] = s;
   }
-}
-
-// Fixed-order sum of per-workgroup pose partials (one workgroup).
-__global__ __launch_bounds__(kBlock) void k_reduce_parts(
-    const double *__restrict__ parts, int nparts, int width,
-    double *__restrict__ res) {
-  __shared__ double sh[kBlock][kPosePart];
-  double s[kPosePart];
+  // The last workgroup to finish sums every partial in a fixed order and
+  // writes the result (host-mapped), then re-arms the counter.
+  __shared__ unsigned is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == nblocks - 1u) ? 1u : 0u;
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double s2[NR];
 #pragma unroll
-  for (int a = 0; a < kPosePart; ++a) s[a] = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += kBlock)
-    for (int a = 0; a < width; ++a) s[a] += parts[(size_t)k * kPosePart + a];
+  for (int a = 0; a < NR; ++a) s2[a] = 0.0;
+  for (unsigned b = threadIdx.x; b < nblocks; b += kBlock)
 #pragma unroll
-  for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] = s[a];
+    for (int a = 0; a < NR; ++a) s2[a] += __builtin_nontemporal_load(out + (size_t)b * kPosePart + a);
+  __shared__ double fin[kBlock][NR];
+#pragma unroll
+  for (int a = 0; a < NR; ++a) fin[threadIdx.x][a] = s2[a];
   __syncthreads();
   for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
     if ((int)threadIdx.x < stride)
-      for (int a = 0; a < width; ++a) sh[threadIdx.x][a] += sh[threadIdx.x + stride][a];
+#pragma unroll
+      for (int a = 0; a < NR; ++a) fin[threadIdx.x][a] += fin[threadIdx.x + stride][a];
     __syncthreads();
   }
-  if ((int)threadIdx.x < width) res[threadIdx.x] = sh[0][threadIdx.x];
+  if (threadIdx.x < NR) res[threadIdx.x] = fin[0][threadIdx.x];
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // Per-point sums over streamed chunks, in chunk order.
@@ -267,42 +275,46 @@ static Plan make_plan(int nP, int nQ, int mode, int T) {
   return pl;
 }
 
-template <class Rec, int MODE>
-static void launch_pairs_t(const Plan &pl, bool comp, const Rec *P, int nP,
-                           const Rec *Q, int nQ, double *out, const double *t,
-                           hipStream_t s) {
+template <class Rec, int MODE, int COMP>
+static void launch_pairs_c(const Plan &pl, const Rec *P, int nP, const Rec *Q, int nQ,
+                           double *out, unsigned *counter, double *res,
+                           const double *t, hipStream_t s) {
   dim3 grid(pl.pblocks, pl.qchunks), block(kBlock);
   const double t0 = t[0], t1 = t[1], t2 = (RecTraits<Rec>::D == 3) ? t[2] : 0.0;
-  if (pl.T == 2) {
-    if (comp)
-      hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, true>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
-    else
-      hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, false>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
-  } else {
-    if (comp)
-      hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, true>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
-    else
-      hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, false>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk, out, t0, t1, t2);
-  }
-}
-
-template <class Rec>
-static void launch_pairs(int mode, const Plan &pl, bool comp, const Rec *P,
-                         int nP, const Rec *Q, int nQ, double *out,
-                         const double *t, hipStream_t s) {
-  if (mode == kModeEnergy)
-    launch_pairs_t<Rec, kModeEnergy>(pl, comp, P, nP, Q, nQ, out, t, s);
-  else if (mode == kModePose)
-    launch_pairs_t<Rec, kModePose>(pl, comp, P, nP, Q, nQ, out, t, s);
+  if (pl.T == 2)
+    hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
+                       out, counter, res, t0, t1, t2);
   else
-    launch_pairs_t<Rec, kModePoint>(pl, comp, P, nP, Q, nQ, out, t, s);
+    hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
+                       out, counter, res, t0, t1, t2);
 }
 
-static PoseArg make_pose(int D, const double *R, const double *t) {
+// comp: 0 none, 1 streamed side carries the lo part, 2 stationary side does.
+template <class Rec>
+static void launch_pairs(int mode, int comp, const Plan &pl, const Rec *P,
+                         int nP, const Rec *Q, int nQ, double *out,
+                         unsigned *counter, double *res, const double *t,
+                         hipStream_t s) {
+#define DUGMA_L(M, C) launch_pairs_c<Rec, M, C>(pl, P, nP, Q, nQ, out, counter, res, t, s)
+  if (mode == kModeEnergy) {
+    if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
+  } else if (mode == kModePose) {
+    if (comp == 1) DUGMA_L(kModePose, 1); else DUGMA_L(kModePose, 0);
+  } else {
+    if (comp == 1) DUGMA_L(kModePoint, 1);
+    else if (comp == 2) DUGMA_L(kModePoint, 2);
+    else DUGMA_L(kModePoint, 0);
+  }
+#undef DUGMA_L
+}
+
+// Device data are in scaled units: positions times s = 2^k, covariances
+// times s^2 (chosen at init so covariances are O(1)); exact in binary FP.
+static PoseArg make_pose(const dugma_ctx *c, const double *R, const double *t) {
   PoseArg ps;
   memset(&ps, 0, sizeof(ps));
-  for (int a = 0; a < D * D; ++a) ps.R[a] = R[a];
-  for (int a = 0; a < D; ++a) ps.t[a] = t[a];
+  for (int a = 0; a < c->D * c->D; ++a) ps.R[a] = R[a];
+  for (int a = 0; a < c->D; ++a) ps.t[a] = t[a] * c->scale;
   return ps;
 }
 
@@ -347,44 +359,50 @@ static double gauss_norm(int D) {
 static bool comp_enabled() { return env_int("DUGMA_COMPENSATED", 1) != 0; }
 static int lane_points() { return env_int("DUGMA_LANE_POINTS", 1) == 2 ? 2 : 1; }
 
-// Energy-only or fused pose pass; res receives [E, gpose...] scaled.
+// Energy-only or fused pose pass; res receives [E, rotation, translation]
+// in caller units.
 static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
                      int mode, double *res) {
   const int D = c->D;
   const int width = mode == kModeEnergy ? 1 : (D == 3 ? 7 : 4);
   for (int a = 0; a < width; ++a) res[a] = 0.0;
   if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
-  PoseArg ps = make_pose(D, R, t);
+  PoseArg ps = make_pose(c, R, t);
   int rc = transform_sources(c, ps, iso);
   if (rc) return rc;
   Plan pl = make_plan(c->nT, c->nS, mode, lane_points());
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
-  rc = c->part.ensure((nparts + 1) * kPosePart * sizeof(double));
+  rc = c->part.ensure(nparts * kPosePart * sizeof(double));
   if (rc) return rc;
   double *parts = (double *)c->part.ptr;
-  double *dres = parts + nparts * kPosePart;
-  const bool comp = comp_enabled();
+  const int comp = comp_enabled() ? 1 : 0;
   PairTimer timer(c);
   if (D == 3) {
     if (iso)
-      launch_pairs<Rec3I>(mode, pl, comp, (const Rec3I *)c->dTrecI, c->nT, (const Rec3I *)c->dSrecI, c->nS, parts, t, c->stream);
+      launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)c->dTrecI, c->nT, (const Rec3I *)c->dSrecI, c->nS,
+                          parts, c->dcounter, c->dres, ps.t, c->stream);
     else
-      launch_pairs<Rec3F>(mode, pl, comp, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS, parts, t, c->stream);
+      launch_pairs<Rec3F>(mode, comp, pl, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS,
+                          parts, c->dcounter, c->dres, ps.t, c->stream);
   } else {
     if (iso)
-      launch_pairs<Rec2I>(mode, pl, comp, (const Rec2I *)c->dTrecI, c->nT, (const Rec2I *)c->dSrecI, c->nS, parts, t, c->stream);
+      launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)c->dTrecI, c->nT, (const Rec2I *)c->dSrecI, c->nS,
+                          parts, c->dcounter, c->dres, ps.t, c->stream);
     else
-      launch_pairs<Rec2F>(mode, pl, comp, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS, parts, t, c->stream);
+      launch_pairs<Rec2F>(mode, comp, pl, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS,
+                          parts, c->dcounter, c->dres, ps.t, c->stream);
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, c->stream, parts, (int)nparts, width, dres);
-  DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipMemcpyAsync(c->hpinned, dres, sizeof(double) * width, hipMemcpyDeviceToHost, c->stream));
   DUGMA_HIP(hipStreamSynchronize(c->stream));
   timer.finish();
+  // unscale: E ~ s^D, rotation gradient ~ s^D, translation gradient ~ s^(D+1)
   const double cn = gauss_norm(D);
-  for (int a = 0; a < width; ++a) res[a] = cn * c->hpinned[a];
+  const double sD = std::pow(c->scale, D);
+  const int nrot = D == 3 ? 3 : 1;
+  const volatile double *hr = c->hres;
+  res[0] = cn * sD * hr[0];
+  for (int a = 1; a < width; ++a) res[a] = cn * sD * hr[a] * (a > nrot ? c->scale : 1.0);
   return DUGMA_OK;
 }
 
@@ -436,7 +454,7 @@ static void perturb(int D, const double *R, const double *t, const double *p,
 // One per-point pass (stationary set P, streamed set Q), full covariance
 // records; writes per-point FP64 sums [nP][NACC] to out (device).
 static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
-                      int nQ, double *out) {
+                      int nQ, int comp, double *out) {
   const int D = c->D;
   const int nacc = D == 3 ? kAcc3F : kAcc2F;
   Plan pl = make_plan(nP, nQ, kModePoint, 1);
@@ -445,11 +463,13 @@ static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
   if (rc) return rc;
   double *parts = (double *)c->part.ptr;
   const double t0[3] = {0, 0, 0};
-  const bool comp = comp_enabled();
+  if (!comp_enabled()) comp = 0;
   if (D == 3)
-    launch_pairs<Rec3F>(kModePoint, pl, comp, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, t0, c->stream);
+    launch_pairs<Rec3F>(kModePoint, comp, pl, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, c->dcounter,
+                        c->dres, t0, c->stream);
   else
-    launch_pairs<Rec2F>(kModePoint, pl, comp, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, t0, c->stream);
+    launch_pairs<Rec2F>(kModePoint, comp, pl, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, c->dcounter,
+                        c->dres, t0, c->stream);
   DUGMA_HIP(hipGetLastError());
   const size_t total = (size_t)nP * nacc;
   hipLaunchKernelGGL(k_reduce_points, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
@@ -549,15 +569,15 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
   if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
 
   // per-point passes use full-covariance records (isotropic points included)
-  PoseArg ps = make_pose(D, R, t);
+  PoseArg ps = make_pose(c, R, t);
   rc = transform_sources(c, ps, false);
   if (rc) return rc;
   double *acc = nullptr;
   const size_t nT = c->nT, nS = c->nS;
   DUGMA_HIP(hipMalloc(&acc, sizeof(double) * (nT + nS) * nacc));
   std::vector<double> hT(nT * nacc), hS(nS * nacc);
-  rc = point_pass(c, c->dTrecF, c->nT, c->dSrecF, c->nS, acc);
-  if (!rc) rc = point_pass(c, c->dSrecF, c->nS, c->dTrecF, c->nT, acc + nT * nacc);
+  rc = point_pass(c, c->dTrecF, c->nT, c->dSrecF, c->nS, 1, acc);
+  if (!rc) rc = point_pass(c, c->dSrecF, c->nS, c->dTrecF, c->nT, 2, acc + nT * nacc);
   if (!rc) {
     hipError_t e = hipMemcpyAsync(hT.data(), acc, sizeof(double) * nT * nacc, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(hS.data(), acc + nT * nacc, sizeof(double) * nS * nacc, hipMemcpyDeviceToHost, c->stream);
@@ -567,8 +587,10 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
   (void)hipFree(acc);
   if (rc) return rc;
 
+  // Everything below is in scaled units until the final unscaling:
+  // positions x s, covariances x s^2, energy / s^D.
   const double cn = gauss_norm(D);
-  // unpack symmetric (WW - C)/2 into a full column-major matrix
+  const double sc = c->scale, sD = std::pow(sc, D);
   auto cov_grad = [&](const double *a, double *G) {
     if (D == 3) {
       static const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
@@ -586,29 +608,33 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
       }
     }
   };
+  const double gpos = sD * sc, gcov = sD * sc * sc;  // unscaling of point / covariance gradients
   double Etot = 0.0;
   for (size_t k = 0; k < nT; ++k) {
     const double *a = hT.data() + k * nacc;
     Etot += a[0];
     const int i = c->hTidx[k];
     if (gX)
-      for (int d = 0; d < D; ++d) gX[(size_t)i * D + d] = -cn * a[1 + d];
-    if (gXC) cov_grad(a, gXC + (size_t)i * DD);
+      for (int d = 0; d < D; ++d) gX[(size_t)i * D + d] = -cn * a[1 + d] * gpos;
+    if (gXC) {
+      double G[9];
+      cov_grad(a, G);
+      for (int q = 0; q < DD; ++q) gXC[(size_t)i * DD + q] = G[q] * gcov;
+    }
   }
-  if (E) *E = cn * Etot;
+  if (E) *E = cn * Etot * sD;
 
   double gw[3] = {0, 0, 0}, gt[3] = {0, 0, 0}, gth = 0.0, G[9] = {0};
   for (size_t k = 0; k < nS; ++k) {
     const double *a = hS.data() + k * nacc;
     const double *s = c->hSsrc.data() + k * (D + DD);
-    double g[3], H[9], xp[3], p[3], Sp[9];
+    double g[3], H[9], p[3], Sp[9];
     for (int d = 0; d < D; ++d) g[d] = -cn * a[1 + d];  // dE/dx'_j
     cov_grad(a, H);                                     // dE/dS'_j
     for (int d = 0; d < D; ++d) {
       double v = 0;
       for (int b = 0; b < D; ++b) v += R[d + D * b] * s[b];
-      p[d] = v;  // R x_j
-      xp[d] = v + t[d];
+      p[d] = v;  // R x_j  (= x'_j - t)
     }
     double RS[9];
     for (int a1 = 0; a1 < D; ++a1)
@@ -623,9 +649,7 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
         for (int b = 0; b < D; ++b) v += RS[a1 + D * b] * R[c1 + D * b];
         Sp[a1 + D * c1] = v;
       }
-    (void)xp;
     for (int d = 0; d < D; ++d) gt[d] += g[d];
-    // H S' (used by the rotation and dR gradients)
     double HS[9], SH[9];
     for (int a1 = 0; a1 < D; ++a1)
       for (int c1 = 0; c1 < D; ++c1) {
@@ -658,7 +682,7 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
       for (int b = 0; b < D; ++b) {
         double v = 0;
         for (int a1 = 0; a1 < D; ++a1) v += R[a1 + D * b] * g[a1];
-        gY[(size_t)j * D + b] = v;
+        gY[(size_t)j * D + b] = v * gpos;
       }
     if (gYC) {
       double *o = gYC + (size_t)j * DD;
@@ -667,34 +691,35 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
           double v = 0;
           for (int a1 = 0; a1 < D; ++a1)
             for (int e = 0; e < D; ++e) v += R[a1 + D * b] * H[a1 + D * e] * R[e + D * c1];
-          o[b + D * c1] = v;
+          o[b + D * c1] = v * gcov;
         }
     }
   }
+  // unscale: rotation / dR parts ~ s^D, translation parts ~ s^(D+1)
   if (gpose) {
     if (D == 3) {
       for (int a = 0; a < 3; ++a) {
-        gpose[a] = gw[a];
-        gpose[3 + a] = gt[a];
+        gpose[a] = gw[a] * sD;
+        gpose[3 + a] = gt[a] * gpos;
       }
     } else {
-      gpose[0] = gth;
-      gpose[1] = gt[0];
-      gpose[2] = gt[1];
+      gpose[0] = gth * sD;
+      gpose[1] = gt[0] * gpos;
+      gpose[2] = gt[1] * gpos;
     }
   }
   if (gdT) {
     if (D == 3) {
       // energy3D.m variable order: t1 t2 t3 r11 r12 r13 r21 ... r33 (dR row-major)
-      for (int a = 0; a < 3; ++a) gdT[a] = gt[a];
+      for (int a = 0; a < 3; ++a) gdT[a] = gt[a] * gpos;
       for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) gdT[3 + 3 * a + b] = G[a + 3 * b];
+        for (int b = 0; b < 3; ++b) gdT[3 + 3 * a + b] = G[a + 3 * b] * sD;
     } else {
       // energy2D.m: t1 t2 r11 r21 with dR = [[r11, -r21], [r21, r11]]
-      gdT[0] = gt[0];
-      gdT[1] = gt[1];
-      gdT[2] = G[0] + G[3];
-      gdT[3] = G[1] - G[2];
+      gdT[0] = gt[0] * gpos;
+      gdT[1] = gt[1] * gpos;
+      gdT[2] = (G[0] + G[3]) * sD;
+      gdT[3] = (G[1] - G[2]) * sD;
     }
   }
   return DUGMA_OK;

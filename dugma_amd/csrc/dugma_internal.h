@@ -108,9 +108,16 @@ struct dugma_ctx {
   void *dSrecF = nullptr;    // nS per-call transformed source records
   void *dSrecI = nullptr;
 
+  // Positions are held times scale = 2^lscale and covariances times scale^2,
+  // so that covariances are O(1) in FP32 whatever the caller's units.
+  int lscale = 0;
+  double scale = 1.0;
+  unsigned *dcounter = nullptr;  // pair-kernel completion counter (re-armed by the kernel)
+  double *hres = nullptr;        // pinned host slot the pair kernel writes its result to
+  double *dres = nullptr;        // device address of hres
+
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc
-  std::vector<double> hTpts;      // kept targets: point (D) + covariance (D*D)
 
   // optional kernel timing (HIP events on `stream`)
   bool timing = false;

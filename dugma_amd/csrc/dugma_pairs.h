@@ -27,15 +27,23 @@ template <> struct RecTraits<Rec3I> { static constexpr int D = 3, NACC = kAcc3I;
 template <> struct RecTraits<Rec2F> { static constexpr int D = 2, NACC = kAcc2F; static constexpr bool FULL = true; };
 template <> struct RecTraits<Rec2I> { static constexpr int D = 2, NACC = kAcc2I; static constexpr bool FULL = false; };
 
+// GRAD: 0 energy only; 1 pose adjoint; 2 per-point adjoint (isotropic
+// records then also accumulate G).  COMP: how the FP32 hi/lo position split
+// enters d = p - q:  0 hi parts only; 1 only the streamed point has a lo part
+// (d = (p.h - q.h) - q.l); 2 only the stationary point has one
+// (d = (p.h - q.h) + p.l).  p.h - q.h is exact for nearby points, so d keeps
+// the low bits of the FP64 source transform.
+#define DUGMA_DIFF(c)                            \
+  float d##c = p.h##c - q.h##c;                  \
+  if (COMP == 1) d##c = d##c - q.l##c;           \
+  if (COMP == 2) d##c = d##c + p.l##c;
+
 // ---- 3-D, full covariance -------------------------------------------------
-template <bool GRAD, bool COMP>
+template <int GRAD, int COMP>
 __device__ __forceinline__ void pair_eval(const Rec3F &p, const Rec3F &q, float *acc) {
-  float dx = p.hx - q.hx, dy = p.hy - q.hy, dz = p.hz - q.hz;
-  if (COMP) {
-    dx += p.lx - q.lx;
-    dy += p.ly - q.ly;
-    dz += p.lz - q.lz;
-  }
+  DUGMA_DIFF(x)
+  DUGMA_DIFF(y)
+  DUGMA_DIFF(z)
   const float a11 = p.s11 + q.s11, a12 = p.s12 + q.s12, a13 = p.s13 + q.s13;
   const float a22 = p.s22 + q.s22, a23 = p.s23 + q.s23, a33 = p.s33 + q.s33;
   // adjugate (symmetric) and determinant
@@ -55,18 +63,19 @@ __device__ __forceinline__ void pair_eval(const Rec3F &p, const Rec3F &q, float 
   const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
   acc[0] += e;
   if (GRAD) {
-    const float w1 = u1 * r2, w2 = u2 * r2, w3 = u3 * r2;
-    const float ew1 = e * w1, ew2 = e * w2, ew3 = e * w3;
-    acc[1] += ew1;
-    acc[2] += ew2;
-    acc[3] += ew3;
-    acc[4] = fmaf(ew1, w1, acc[4]);
-    acc[5] = fmaf(ew1, w2, acc[5]);
-    acc[6] = fmaf(ew1, w3, acc[6]);
-    acc[7] = fmaf(ew2, w2, acc[7]);
-    acc[8] = fmaf(ew2, w3, acc[8]);
-    acc[9] = fmaf(ew3, w3, acc[9]);
+    // w = u/det:  EW += e w,  WW += e w w^T,  C += e adj/det
     const float er2 = e * r2;
+    acc[1] = fmaf(er2, u1, acc[1]);
+    acc[2] = fmaf(er2, u2, acc[2]);
+    acc[3] = fmaf(er2, u3, acc[3]);
+    const float f = er2 * r2;
+    const float f1 = f * u1, f2 = f * u2, f3 = f * u3;
+    acc[4] = fmaf(f1, u1, acc[4]);
+    acc[5] = fmaf(f1, u2, acc[5]);
+    acc[6] = fmaf(f1, u3, acc[6]);
+    acc[7] = fmaf(f2, u2, acc[7]);
+    acc[8] = fmaf(f2, u3, acc[8]);
+    acc[9] = fmaf(f3, u3, acc[9]);
     acc[10] = fmaf(er2, c11, acc[10]);
     acc[11] = fmaf(er2, c12, acc[11]);
     acc[12] = fmaf(er2, c13, acc[12]);
@@ -77,14 +86,11 @@ __device__ __forceinline__ void pair_eval(const Rec3F &p, const Rec3F &q, float 
 }
 
 // ---- 3-D, isotropic covariance --------------------------------------------
-template <bool GRAD, bool COMP>
+template <int GRAD, int COMP>
 __device__ __forceinline__ void pair_eval(const Rec3I &p, const Rec3I &q, float *acc) {
-  float dx = p.hx - q.hx, dy = p.hy - q.hy, dz = p.hz - q.hz;
-  if (COMP) {
-    dx += p.lx - q.lx;
-    dy += p.ly - q.ly;
-    dz += p.lz - q.lz;
-  }
+  DUGMA_DIFF(x)
+  DUGMA_DIFF(y)
+  DUGMA_DIFF(z)
   const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   const float s = p.s + q.s;
   const float rs = frsq(s);
@@ -96,18 +102,15 @@ __device__ __forceinline__ void pair_eval(const Rec3I &p, const Rec3I &q, float 
     acc[1] = fmaf(einv, dx, acc[1]);
     acc[2] = fmaf(einv, dy, acc[2]);
     acc[3] = fmaf(einv, dz, acc[3]);
-    acc[4] = fmaf(einv, fmaf(d2, inv, -3.0f), acc[4]);
+    if (GRAD == 2) acc[4] = fmaf(einv, fmaf(d2, inv, -3.0f), acc[4]);
   }
 }
 
 // ---- 2-D, full covariance -------------------------------------------------
-template <bool GRAD, bool COMP>
+template <int GRAD, int COMP>
 __device__ __forceinline__ void pair_eval(const Rec2F &p, const Rec2F &q, float *acc) {
-  float dx = p.hx - q.hx, dy = p.hy - q.hy;
-  if (COMP) {
-    dx += p.lx - q.lx;
-    dy += p.ly - q.ly;
-  }
+  DUGMA_DIFF(x)
+  DUGMA_DIFF(y)
   const float a11 = p.s11 + q.s11, a12 = p.s12 + q.s12, a22 = p.s22 + q.s22;
   const float det = fmaf(a11, a22, -(a12 * a12));
   const float u1 = fmaf(a22, dx, -(a12 * dy));
@@ -118,14 +121,14 @@ __device__ __forceinline__ void pair_eval(const Rec2F &p, const Rec2F &q, float 
   const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
   acc[0] += e;
   if (GRAD) {
-    const float w1 = u1 * r2, w2 = u2 * r2;
-    const float ew1 = e * w1, ew2 = e * w2;
-    acc[1] += ew1;
-    acc[2] += ew2;
-    acc[3] = fmaf(ew1, w1, acc[3]);
-    acc[4] = fmaf(ew1, w2, acc[4]);
-    acc[5] = fmaf(ew2, w2, acc[5]);
     const float er2 = e * r2;
+    acc[1] = fmaf(er2, u1, acc[1]);
+    acc[2] = fmaf(er2, u2, acc[2]);
+    const float f = er2 * r2;
+    const float f1 = f * u1, f2 = f * u2;
+    acc[3] = fmaf(f1, u1, acc[3]);
+    acc[4] = fmaf(f1, u2, acc[4]);
+    acc[5] = fmaf(f2, u2, acc[5]);
     acc[6] = fmaf(er2, a22, acc[6]);
     acc[7] = fmaf(-er2, a12, acc[7]);
     acc[8] = fmaf(er2, a11, acc[8]);
@@ -133,13 +136,10 @@ __device__ __forceinline__ void pair_eval(const Rec2F &p, const Rec2F &q, float 
 }
 
 // ---- 2-D, isotropic covariance --------------------------------------------
-template <bool GRAD, bool COMP>
+template <int GRAD, int COMP>
 __device__ __forceinline__ void pair_eval(const Rec2I &p, const Rec2I &q, float *acc) {
-  float dx = p.hx - q.hx, dy = p.hy - q.hy;
-  if (COMP) {
-    dx += p.lx - q.lx;
-    dy += p.ly - q.ly;
-  }
+  DUGMA_DIFF(x)
+  DUGMA_DIFF(y)
   const float d2 = fmaf(dx, dx, dy * dy);
   const float inv = frcp(p.s + q.s);
   const float e = fexp2((d2 * inv) * kNegHalfLog2e) * inv;  // s^-1
@@ -148,9 +148,11 @@ __device__ __forceinline__ void pair_eval(const Rec2I &p, const Rec2I &q, float 
     const float einv = e * inv;
     acc[1] = fmaf(einv, dx, acc[1]);
     acc[2] = fmaf(einv, dy, acc[2]);
-    acc[3] = fmaf(einv, fmaf(d2, inv, -2.0f), acc[3]);
+    if (GRAD == 2) acc[3] = fmaf(einv, fmaf(d2, inv, -2.0f), acc[3]);
   }
 }
+
+#undef DUGMA_DIFF
 
 // ---- pose-gradient contribution of one stationary target ------------------
 // The energy is invariant under one rigid motion of both clouds, so the

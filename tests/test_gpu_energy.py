@@ -24,9 +24,15 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
 def as_oracle(d):
-    """MATLAB-layout workload -> oracle arrays (point-major, (n,D,D))."""
-    return d["X"].T, W.unblocks(d["X_C"]), d["Y"].T, W.unblocks(d["Y_C"])
+    """MATLAB-layout workload -> oracle arrays (point-major, (n,D,D)).
+    The library evaluates the energy of the FP32 points and covariances (the
+    reference's device state after init); the pose stays FP64."""
+    return f32(d["X"].T), f32(W.unblocks(d["X_C"])), f32(d["Y"].T), f32(W.unblocks(d["Y_C"]))
 
 
 def perturbed_pose(d, deg=3.0, dt=0.01, seed=0):
@@ -39,14 +45,14 @@ def perturbed_pose(d, deg=3.0, dt=0.01, seed=0):
     return R, np.ravel(d["t_G"]) + rng.uniform(-dt, dt, size=D)
 
 
-def check_energy(h, d, R, t, oracle, cov="auto"):
+def check_energy(h, d, R, t, oracle, cov="auto", e_tol=E_TOL, g_tol=G_TOL):
     Y, SY, X, SX = as_oracle(d)
     ref = oracle.energy(Y, SY, X, SX, R, t, grads=True)
     E = dg.energy(h, R, t, cov)
-    assert abs(E / ref["E"] - 1) <= E_TOL, (E, ref["E"])
+    assert abs(E / ref["E"] - 1) <= e_tol, (E, ref["E"])
     E2, g = dg.energy_grad(h, R, t, cov)
-    assert abs(E2 / ref["E"] - 1) <= E_TOL
-    assert rel(g, ref["gpose"]) <= G_TOL, (g, ref["gpose"])
+    assert abs(E2 / ref["E"] - 1) <= e_tol
+    assert rel(g, ref["gpose"]) <= g_tol, (g, ref["gpose"])
     return ref
 
 
@@ -158,6 +164,9 @@ def test_energy_invalid_points_left_out(oracle):
 
 @pytest.mark.parametrize("n,m", [(1, 1), (1, 777), (333, 1), (257, 65), (4099, 129)])
 def test_energy_ragged_sizes(oracle, n, m):
+    """Every pair counted once at sizes off the 64/256 grid.  With a handful
+    of pairs nothing averages: the FP32 exp argument error (~q * 2^-24 for
+    Mahalanobis distance q) is the whole error, so the bar here is 2e-5."""
     rng = np.random.default_rng(n * 7 + m)
     X = rng.normal(size=(3, n)) * 0.2
     Y = rng.normal(size=(3, m)) * 0.2
@@ -168,7 +177,8 @@ def test_energy_ragged_sizes(oracle, n, m):
     d = dict(X=X, X_C=W.blocks(CX), Y=Y, Y_C=W.blocks(CY), R_G=np.eye(3), t_G=np.zeros(3))
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
-        check_energy(h, d, W.random_rotation(rng, 10), rng.normal(size=3) * 0.01, oracle)
+        check_energy(h, d, W.random_rotation(rng, 10), rng.normal(size=3) * 0.01, oracle,
+                     e_tol=2e-5, g_tol=2e-5)
     finally:
         dg.fr_m(h)
 

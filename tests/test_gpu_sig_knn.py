@@ -26,10 +26,10 @@ def test_init_handles_bit_exact(oracle, k):
         assert ok == 1.0
         np.testing.assert_array_equal(h.read_handle(0), d["X"].T.astype(np.float32).ravel())
         np.testing.assert_array_equal(h.read_handle(2), d["Y"].T.astype(np.float32).ravel())
-        for which, P in ((1, d["X_C"]), (3, d["Y_C"])):
+        for which, gwhich, P in ((1, 4, d["X_C"]), (3, 5, d["Y_C"])):
             inv, g, _ = oracle.prep_inverse(W.unblocks(P), 3)
             np.testing.assert_array_equal(h.read_handle(which), inv.ravel())
-            np.testing.assert_array_equal(h.read_handle(which + 3), g)
+            np.testing.assert_array_equal(h.read_handle(gwhich), g)
     finally:
         dg.fr_m(h)
 
