@@ -85,6 +85,7 @@ def bench_fused(hip, dg, d, steps, warmup, label):
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
         R, t = start_pose(d)
+        plan = h.autotune()
         for _ in range(warmup):
             dg.energy_grad(h, R, t)
         ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps)
@@ -95,10 +96,10 @@ def bench_fused(hip, dg, d, steps, warmup, label):
         kms, launches = h.pair_time()
         h.set_timing(False)
         nT, nS, iso = h.energy_info()
-        log("%s: %.3f ms/step, pair kernel %.3f ms/launch (%d launches), iso=%s"
-            % (label, ms / steps, kms / max(launches, 1), launches, iso))
+        log("%s: %.3f ms/step, pair kernel %.3f ms/launch (%d launches), iso=%s, plan T=%d waves=%d"
+            % (label, ms / steps, kms / max(launches, 1), launches, iso, plan[0], plan[1]))
         return dict(ms_per_step=ms / steps, kernel_ms=kms / max(launches, 1), pairs=nT * nS, iso=iso,
-                    h=None)
+                    plan={"lane_points": plan[0], "target_waves": plan[1]})
     finally:
         dg.fr_m(h)
 
@@ -107,6 +108,7 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
         R, t = start_pose(d)
+        h.autotune()
         dg.energy_grad_fd(h, R, t, step_h)
         ms = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, step_h), steps)
         nT, nS, _ = h.energy_info()
@@ -176,7 +178,8 @@ def main():
                    "target_points": 6000, "source_points": 6000, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED},
+                     "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED,
+                     "launch": fused["plan"]},
     }
     if not args.no_plain:
         plain = bench_plain(hip, dg, d2, max(3, args.steps // 10))

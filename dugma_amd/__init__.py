@@ -71,6 +71,13 @@ class Context:
         _lib.check(_lib.load().dugma_ctx_pair_time(self.ptr, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def autotune(self, cov="auto"):
+        """Time candidate launch shapes of the energy passes; keep the fastest."""
+        _lib.check(_lib.load().dugma_autotune(self.ptr, _cov_mode(cov)))
+        T, w = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().dugma_ctx_plan(self.ptr, _cov_mode(cov), ctypes.byref(T), ctypes.byref(w)))
+        return T.value, w.value
+
     def energy_info(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(_lib.load().dugma_ctx_energy_info(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))

@@ -27,6 +27,8 @@ SIGNATURES = {
     "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),
     "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),
     "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),
+    "dugma_autotune": (_ci, [_vp, _ci]),
+    "dugma_ctx_plan": (_ci, [_vp, _ci, _i, _i]),
     "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
     "dugma_read_handle": (_ci, [_vp, _ci, _f]),
     "dugma_ctx_dims": (_ci, [_vp, _i, _i, _i]),

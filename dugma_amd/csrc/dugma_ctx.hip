@@ -223,7 +223,7 @@ static void free_ctx(dugma_ctx *c) {
   if (!c) return;
   void *ptrs[] = {c->dX, c->dXCinv, c->dY, c->dYCinv, c->dgX, c->dgY,
                   c->dTidx, c->dSidx, c->dTrecF, c->dTrecI, c->dSsrc,
-                  c->dSrecF, c->dSrecI, c->dcounter};
+                  c->dSrecF, c->dSrecI};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->part.release();
@@ -287,9 +287,7 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     c->hres = c->hpinned + dugma_ctx::kHostStage / 2;
     e = hipHostGetDevicePointer((void **)&c->dres, c->hres, 0);
     if (e != hipSuccess) TRY(hip_fail(e, "hipHostGetDevicePointer"));
-    TRY(dmalloc(&c->dcounter, 1));
-    e = hipMemsetAsync(c->dcounter, 0, sizeof(unsigned), c->stream);
-    if (e != hipSuccess) TRY(hip_fail(e, "hipMemsetAsync(counter)"));
+
 
     // ---- reference state (init.cu:264-331) ----
     const int DD = D * D;

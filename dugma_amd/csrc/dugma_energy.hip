@@ -36,8 +36,7 @@ struct PoseArg {
 template <class Rec, int MODE, int T, int COMP>
 __global__ __launch_bounds__(kBlock) void k_pairs(
     const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
-    int qchunk, double *__restrict__ out, unsigned *__restrict__ counter,
-    double *__restrict__ res, double t0, double t1, double t2) {
+    int qchunk, double *__restrict__ out, double t0, double t1, double t2) {
   using Tr = RecTraits<Rec>;
   constexpr bool GRAD = MODE != kModeEnergy;
   constexpr int NA = GRAD ? Tr::NACC : 1;
@@ -112,40 +111,42 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
 #pragma unroll
     for (int a = 0; a < NR; ++a) red[wave][a] = r[a];
   __syncthreads();
-  const unsigned nblocks = gridDim.x * gridDim.y;
-  if (threadIdx.x < NR) {
-    double s = red[0][threadIdx.x];
+  if (threadIdx.x < kPosePart) {
+    double s = 0.0;
+    if ((int)threadIdx.x < NR) {
+      s = red[0][threadIdx.x];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+      for (int w = 1; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    }
     out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart + threadIdx.x] = s;
   }
-  // The last workgroup to finish sums every partial in a fixed order and
-  // writes the result (host-mapped), then re-arms the counter.
-  __shared__ unsigned is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == nblocks - 1u) ? 1u : 0u;
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double s2[NR];
+}
+
+// Fixed-order sum of the per-workgroup partials (one workgroup of 1024).
+// The result goes straight to the context's pinned host slot.
+__global__ __launch_bounds__(1024) void k_reduce_parts(
+    const double *__restrict__ parts, int nparts, int width,
+    double *__restrict__ res) {
+  __shared__ double sh[1024][kPosePart];
+  double s[kPosePart];
 #pragma unroll
-  for (int a = 0; a < NR; ++a) s2[a] = 0.0;
-  for (unsigned b = threadIdx.x; b < nblocks; b += kBlock)
+  for (int a = 0; a < kPosePart; ++a) s[a] = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += 1024) {
+    const double4 *p = reinterpret_cast<const double4 *>(parts + (size_t)k * kPosePart);
+    const double4 u = p[0], v = p[1];
+    s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
+    s[4] += v.x; s[5] += v.y; s[6] += v.z; s[7] += v.w;
+  }
 #pragma unroll
-    for (int a = 0; a < NR; ++a) s2[a] += __builtin_nontemporal_load(out + (size_t)b * kPosePart + a);
-  __shared__ double fin[kBlock][NR];
-#pragma unroll
-  for (int a = 0; a < NR; ++a) fin[threadIdx.x][a] = s2[a];
+  for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] = s[a];
   __syncthreads();
-  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+  for (int stride = 512; stride > 0; stride >>= 1) {
     if ((int)threadIdx.x < stride)
 #pragma unroll
-      for (int a = 0; a < NR; ++a) fin[threadIdx.x][a] += fin[threadIdx.x + stride][a];
+      for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] += sh[threadIdx.x + stride][a];
     __syncthreads();
   }
-  if (threadIdx.x < NR) res[threadIdx.x] = fin[0][threadIdx.x];
-  if (threadIdx.x == 0) *counter = 0u;
+  if ((int)threadIdx.x < width) res[threadIdx.x] = sh[0][threadIdx.x];
 }
 
 // Per-point sums over streamed chunks, in chunk order.
@@ -258,13 +259,14 @@ static int env_int(const char *name, int dflt) {
   return (v && *v) ? atoi(v) : dflt;
 }
 
-// Enough workgroups to keep every SIMD of the 256 CUs several waves deep,
-// with streamed chunks of at least kSub points.
-static Plan make_plan(int nP, int nQ, int mode, int T) {
+// Launch shape: T stationary points per lane and a target number of waves
+// (streamed chunks are sized to reach it).  A context may carry tuned values
+// (dugma_autotune); otherwise: T = 2 and 4096 waves, 16384 for very large
+// problems, so the grid covers the 256 CUs several times.
+static Plan make_plan(int nP, int nQ, int mode, int T, int target_waves) {
   Plan pl;
   pl.T = T;
   pl.pblocks = (nP + kBlock * T - 1) / (kBlock * T);
-  const int target_waves = env_int("DUGMA_TARGET_WAVES", 8192);
   const int waves_p = pl.pblocks * (kBlock / 64);
   int qc = (target_waves + waves_p - 1) / waves_p;
   int maxc = (nQ + kSub - 1) / kSub;
@@ -275,27 +277,37 @@ static Plan make_plan(int nP, int nQ, int mode, int T) {
   return pl;
 }
 
+static Plan ctx_plan(const dugma_ctx *c, int nP, int nQ, int mode, bool iso) {
+  int T = 2, tw = ((double)nP * nQ >= 1e9) ? 16384 : 4096;
+  const dugma_ctx::Tuned &tu = c->tuned[mode == kModeEnergy ? 0 : 1][iso ? 1 : 0];
+  if (tu.target_waves > 0) {
+    T = tu.T;
+    tw = tu.target_waves;
+  }
+  T = env_int("DUGMA_LANE_POINTS", T) == 1 ? 1 : 2;
+  tw = env_int("DUGMA_TARGET_WAVES", tw);
+  return make_plan(nP, nQ, mode, T, tw);
+}
+
 template <class Rec, int MODE, int COMP>
 static void launch_pairs_c(const Plan &pl, const Rec *P, int nP, const Rec *Q, int nQ,
-                           double *out, unsigned *counter, double *res,
-                           const double *t, hipStream_t s) {
+                           double *out, const double *t, hipStream_t s) {
   dim3 grid(pl.pblocks, pl.qchunks), block(kBlock);
   const double t0 = t[0], t1 = t[1], t2 = (RecTraits<Rec>::D == 3) ? t[2] : 0.0;
   if (pl.T == 2)
     hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
-                       out, counter, res, t0, t1, t2);
+                       out, t0, t1, t2);
   else
     hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
-                       out, counter, res, t0, t1, t2);
+                       out, t0, t1, t2);
 }
 
 // comp: 0 none, 1 streamed side carries the lo part, 2 stationary side does.
 template <class Rec>
 static void launch_pairs(int mode, int comp, const Plan &pl, const Rec *P,
                          int nP, const Rec *Q, int nQ, double *out,
-                         unsigned *counter, double *res, const double *t,
-                         hipStream_t s) {
-#define DUGMA_L(M, C) launch_pairs_c<Rec, M, C>(pl, P, nP, Q, nQ, out, counter, res, t, s)
+                         const double *t, hipStream_t s) {
+#define DUGMA_L(M, C) launch_pairs_c<Rec, M, C>(pl, P, nP, Q, nQ, out, t, s)
   if (mode == kModeEnergy) {
     if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
   } else if (mode == kModePose) {
@@ -357,7 +369,6 @@ static double gauss_norm(int D) {
 }
 
 static bool comp_enabled() { return env_int("DUGMA_COMPENSATED", 1) != 0; }
-static int lane_points() { return env_int("DUGMA_LANE_POINTS", 1) == 2 ? 2 : 1; }
 
 // Energy-only or fused pose pass; res receives [E, rotation, translation]
 // in caller units.
@@ -370,7 +381,7 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   PoseArg ps = make_pose(c, R, t);
   int rc = transform_sources(c, ps, iso);
   if (rc) return rc;
-  Plan pl = make_plan(c->nT, c->nS, mode, lane_points());
+  Plan pl = ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
   rc = c->part.ensure(nparts * kPosePart * sizeof(double));
   if (rc) return rc;
@@ -380,19 +391,21 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   if (D == 3) {
     if (iso)
       launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)c->dTrecI, c->nT, (const Rec3I *)c->dSrecI, c->nS,
-                          parts, c->dcounter, c->dres, ps.t, c->stream);
+                          parts, ps.t, c->stream);
     else
       launch_pairs<Rec3F>(mode, comp, pl, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS,
-                          parts, c->dcounter, c->dres, ps.t, c->stream);
+                          parts, ps.t, c->stream);
   } else {
     if (iso)
       launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)c->dTrecI, c->nT, (const Rec2I *)c->dSrecI, c->nS,
-                          parts, c->dcounter, c->dres, ps.t, c->stream);
+                          parts, ps.t, c->stream);
     else
       launch_pairs<Rec2F>(mode, comp, pl, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS,
-                          parts, c->dcounter, c->dres, ps.t, c->stream);
+                          parts, ps.t, c->stream);
   }
   timer.stop();
+  DUGMA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
   DUGMA_HIP(hipGetLastError());
   DUGMA_HIP(hipStreamSynchronize(c->stream));
   timer.finish();
@@ -457,7 +470,7 @@ static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
                       int nQ, int comp, double *out) {
   const int D = c->D;
   const int nacc = D == 3 ? kAcc3F : kAcc2F;
-  Plan pl = make_plan(nP, nQ, kModePoint, 1);
+  Plan pl = make_plan(nP, nQ, kModePoint, 1, 8192);
   const size_t nparts = (size_t)pl.qchunks * nP * nacc;
   int rc = c->part.ensure(nparts * sizeof(double));
   if (rc) return rc;
@@ -465,11 +478,9 @@ static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
   const double t0[3] = {0, 0, 0};
   if (!comp_enabled()) comp = 0;
   if (D == 3)
-    launch_pairs<Rec3F>(kModePoint, comp, pl, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, c->dcounter,
-                        c->dres, t0, c->stream);
+    launch_pairs<Rec3F>(kModePoint, comp, pl, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, t0, c->stream);
   else
-    launch_pairs<Rec2F>(kModePoint, comp, pl, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, c->dcounter,
-                        c->dres, t0, c->stream);
+    launch_pairs<Rec2F>(kModePoint, comp, pl, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, t0, c->stream);
   DUGMA_HIP(hipGetLastError());
   const size_t total = (size_t)nP * nacc;
   hipLaunchKernelGGL(k_reduce_points, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
@@ -722,5 +733,62 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
       gdT[3] = (G[1] - G[2]) * sD;
     }
   }
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_autotune: null context");
+  bool iso;
+  int rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  if (ctx->nT == 0 || ctx->nS == 0) return DUGMA_OK;
+  const int D = ctx->D;
+  double R[9] = {0}, t[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) R[a * (D + 1)] = 1.0;
+  hipEvent_t e0, e1;
+  DUGMA_HIP(hipEventCreate(&e0));
+  DUGMA_HIP(hipEventCreate(&e1));
+  static const int kT[2] = {1, 2};
+  static const int kW[5] = {2048, 4096, 8192, 16384, 32768};
+  const double pairs = (double)ctx->nT * ctx->nS;
+  const int reps = pairs > 4e9 ? 1 : (pairs > 1e8 ? 3 : 10);
+  for (int m = 0; m < 2; ++m) {
+    const int mode = m == 0 ? kModeEnergy : kModePose;
+    dugma_ctx::Tuned best;
+    float best_ms = 1e30f;
+    for (int ti = 0; ti < 2; ++ti)
+      for (int wi = 0; wi < 5; ++wi) {
+        dugma_ctx::Tuned &tu = ctx->tuned[m][iso ? 1 : 0];
+        tu.T = kT[ti];
+        tu.target_waves = kW[wi];
+        double res[kPosePart];
+        rc = pose_pass(ctx, R, t, iso, mode, res);  // warm
+        if (rc) break;
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = pose_pass(ctx, R, t, iso, mode, res);
+        (void)hipEventRecord(e1, ctx->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (!rc && ms < best_ms) {
+          best_ms = ms;
+          best = tu;
+        }
+      }
+    ctx->tuned[m][iso ? 1 : 0] = best;
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+extern "C" int dugma_ctx_plan(const dugma_ctx *ctx, int cov_mode, int *lane_points,
+                              int *target_waves) {
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_ctx_plan: null context");
+  const bool iso = cov_mode == DUGMA_COV_ISO || (cov_mode == DUGMA_COV_AUTO && ctx->iso);
+  const dugma_ctx::Tuned &tu = ctx->tuned[1][iso ? 1 : 0];
+  if (lane_points) *lane_points = tu.T;
+  if (target_waves) *target_waves = tu.target_waves;
   return DUGMA_OK;
 }

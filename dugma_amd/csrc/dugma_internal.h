@@ -112,12 +112,18 @@ struct dugma_ctx {
   // so that covariances are O(1) in FP32 whatever the caller's units.
   int lscale = 0;
   double scale = 1.0;
-  unsigned *dcounter = nullptr;  // pair-kernel completion counter (re-armed by the kernel)
-  double *hres = nullptr;        // pinned host slot the pair kernel writes its result to
+  double *hres = nullptr;        // pinned host slot the reduction writes its result to
   double *dres = nullptr;        // device address of hres
 
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc
+
+  // launch shapes chosen by dugma_autotune: [energy / pose][full / iso]
+  struct Tuned {
+    int T = 2;
+    int target_waves = 0;  // 0: not tuned
+  };
+  Tuned tuned[2][2];
 
   // optional kernel timing (HIP events on `stream`)
   bool timing = false;

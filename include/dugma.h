@@ -111,6 +111,15 @@ int dugma_energy_grad_points(dugma_ctx *ctx, const double *R, const double *t,
 int dugma_energy_grad_fd(dugma_ctx *ctx, const double *R, const double *t,
                          int cov_mode, double h, double *E, double *gpose);
 
+/* Choose the launch shape (points per lane, grid depth) of the energy-only
+ * and fused pose passes for this context's sizes by timing candidates on the
+ * device; later calls with the same covariance mode use the fastest.
+ * dugma_ctx_plan reports the chosen shape of the fused pass (0 waves: not
+ * tuned yet). */
+int dugma_autotune(dugma_ctx *ctx, int cov_mode);
+int dugma_ctx_plan(const dugma_ctx *ctx, int cov_mode, int *lane_points,
+                   int *target_waves);
+
 /* ---- k nearest neighbours ---------------------------------------------- */
 
 enum {
