@@ -117,6 +117,49 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         dg.fr_m(h)
 
 
+def bench_batched(dg, W, ntrials, baseline_trials, max_iter=80):
+    """4,096 independent registrations: one persistent pair launch per
+    iteration over a (trial, tile) queue, iterations replayed from a hipGraph;
+    against the same kernels one trial per launch on a subset."""
+    t0 = time.time()
+    trials = W.config4(ntrials=ntrials)
+    log("generated %d trials in %.1f s" % (ntrials, time.time() - t0))
+    b = dg.Batch(trials)
+    try:
+        info = b.info()
+        b.register(max_iter=2)  # warm-up: code objects, graph path
+        t1 = time.perf_counter()
+        out = b.register(max_iter=max_iter, use_graph=True, check_every=10)
+        wall = time.perf_counter() - t1
+    finally:
+        b.free()
+    errs = np.array([W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
+    sub = trials[:baseline_trials]
+    b1 = dg.Batch(sub)
+    try:
+        b1.register(max_iter=2, one_trial_per_launch=True)
+        t2 = time.perf_counter()
+        out1 = b1.register(max_iter=max_iter, one_trial_per_launch=True)
+        wall1 = time.perf_counter() - t2
+    finally:
+        b1.free()
+    same = bool(np.array_equal(out1["R"], out["R"][:baseline_trials]))
+    rps = ntrials / wall
+    rps1 = baseline_trials / wall1
+    log("batched: %d trials in %.3f s (%.1f reg/s), one-trial-per-launch %.1f reg/s, success %.3f"
+        % (ntrials, wall, rps, rps1, float((errs < 2.0).mean())))
+    return {"workload": "4,096 trials, N and M log-uniform in [500, 8000], rotation U(0,90) deg, "
+                        "noise U(0.001,0.02), outliers U(0,0.3), isotropic 8-NN sigma",
+            "value": round(rps, 2), "unit": "registrations/s", "wall_s": round(wall, 4),
+            "iterations_mean": float(out["iters"].mean()), "max_iter": max_iter,
+            "pairs_per_iteration": info["pairs_per_iteration"], "work_items": info["items"],
+            "success_rate": round(float((errs < 2.0).mean()), 4),
+            "rotation_error_median_deg": round(float(np.median(errs)), 4),
+            "one_trial_per_launch": {"value": round(rps1, 2), "trials": baseline_trials,
+                                     "wall_s": round(wall1, 4), "same_poses": same},
+            "speedup": round(rps / rps1, 2)}
+
+
 def cpu_baseline(d, budget_s=12.0):
     from dugma_amd import workloads as W
     from oracle import oracle as O
@@ -144,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 200K x 200K measurement")
     ap.add_argument("--no-plain", action="store_true", help="skip the finite-difference measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched registrations")
+    ap.add_argument("--trials", type=int, default=4096, help="batched trials")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -199,6 +244,8 @@ def main():
                         "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
                         "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED}
+    if not args.no_batch:
+        out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(d2)

@@ -211,3 +211,68 @@ def knn(h, which, k, R=None, t=None):
 
 def version():
     return _lib.load().dugma_version().decode()
+
+
+class Batch:
+    """Many independent 3-D registrations in one device context
+    (dugma_batch_create / dugma_batch_register).  ``trials`` is a list of
+    dicts with X, X_C, Y, Y_C in the layout of ``init`` (isotropic
+    covariances)."""
+
+    def __init__(self, trials):
+        K = len(trials)
+        iso = "sX" in trials[0]
+        names = ("X", "sX", "Y", "sY") if iso else ("X", "X_C", "Y", "Y_C")
+        ptrs = {k: (ctypes.POINTER(ctypes.c_double) * K)() for k in names}
+        N = (ctypes.c_int * K)()
+        M = (ctypes.c_int * K)()
+        keep = []
+        for i, tr in enumerate(trials):
+            arrs = {k: _cm(tr[k]) for k in names}
+            keep.append(arrs)
+            N[i], M[i] = arrs["X"].shape[1], arrs["Y"].shape[1]
+            for k in names:
+                ptrs[k][i] = _dp(arrs[k])
+        out = ctypes.c_void_p()
+        L = _lib.load()
+        if iso:
+            _lib.check(L.dugma_batch_create_iso(K, N, M, ptrs["X"], ptrs["sX"], ptrs["Y"], ptrs["sY"],
+                                                ctypes.byref(out)))
+        else:
+            _lib.check(L.dugma_batch_create(K, 3, N, M, ptrs["X"], ptrs["X_C"], ptrs["Y"], ptrs["Y_C"],
+                                            ctypes.byref(out)))
+        del keep  # the library holds its own copy
+        self._ptr = out
+        self.ntrials = K
+
+    def info(self):
+        k, n, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        _lib.check(_lib.load().dugma_batch_info(self._ptr, ctypes.byref(k), ctypes.byref(n), ctypes.byref(p)))
+        return dict(ntrials=k.value, items=n.value, pairs_per_iteration=p.value)
+
+    def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
+                 tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
+                 check_every=10, grid=0):
+        K = self.ntrials
+        R0 = np.tile(np.eye(3), (K, 1, 1)) if R0 is None else np.asarray(R0, np.float64)
+        t0 = np.zeros((K, 3)) if t0 is None else np.asarray(t0, np.float64)
+        R0c = np.ascontiguousarray(np.transpose(R0, (0, 2, 1)))  # column-major per trial
+        t0c = np.ascontiguousarray(t0)
+        o = _lib.BatchOpts(int(max_iter), int(bool(anneal)), float(anneal_decay), float(tol_rot),
+                           float(tol_trans), int(bool(use_graph)), int(bool(one_trial_per_launch)),
+                           int(check_every), int(grid))
+        R = np.zeros((K, 9))
+        t = np.zeros((K, 3))
+        iters = np.zeros(K, np.int32)
+        status = np.zeros(K, np.int32)
+        E = np.zeros(K)
+        ip = ctypes.POINTER(ctypes.c_int)
+        _lib.check(_lib.load().dugma_batch_register(self._ptr, _dp(R0c), _dp(t0c), ctypes.byref(o), _dp(R),
+                                                    _dp(t), iters.ctypes.data_as(ip), status.ctypes.data_as(ip),
+                                                    _dp(E)))
+        return dict(R=np.transpose(R.reshape(K, 3, 3), (0, 2, 1)), t=t, iters=iters, status=status, E=E)
+
+    def free(self):
+        if self._ptr and self._ptr.value:
+            _lib.check(_lib.load().dugma_batch_free(self._ptr))
+            self._ptr = ctypes.c_void_p(0)

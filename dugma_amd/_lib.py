@@ -16,6 +16,17 @@ _vp = ctypes.c_void_p
 _ci = ctypes.c_int
 _cd = ctypes.c_double
 
+class BatchOpts(ctypes.Structure):
+    """dugma_batch_opts of include/dugma.h."""
+    _fields_ = [("max_iter", ctypes.c_int), ("anneal", ctypes.c_int),
+                ("anneal_decay", ctypes.c_double), ("tol_rot", ctypes.c_double),
+                ("tol_trans", ctypes.c_double), ("use_graph", ctypes.c_int),
+                ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("grid", ctypes.c_int)]
+
+
+_pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+
 # name: (restype, argtypes) -- one entry per declaration of include/dugma.h
 SIGNATURES = {
     "dugma_init": (_ci, [_d, _d, _d, _d, _ci, _ci, _ci, ctypes.POINTER(_vp), _d]),
@@ -29,6 +40,11 @@ SIGNATURES = {
     "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),
     "dugma_autotune": (_ci, [_vp, _ci]),
     "dugma_ctx_plan": (_ci, [_vp, _ci, _i, _i]),
+    "dugma_batch_create": (_ci, [_ci, _ci, _i, _i, _pp, _pp, _pp, _pp, ctypes.POINTER(_vp)]),
+    "dugma_batch_create_iso": (_ci, [_ci, _i, _i, _pp, _pp, _pp, _pp, ctypes.POINTER(_vp)]),
+    "dugma_batch_register": (_ci, [_vp, _d, _d, ctypes.POINTER(BatchOpts), _d, _d, _i, _i, _d]),
+    "dugma_batch_info": (_ci, [_vp, _i, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "dugma_batch_free": (_ci, [_vp]),
     "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
     "dugma_read_handle": (_ci, [_vp, _ci, _f]),
     "dugma_ctx_dims": (_ci, [_vp, _i, _i, _i]),

@@ -1,0 +1,694 @@
+// dugma_batch.hip -- thousands of independent rigid registrations of uneven
+// sizes on one GPU.
+//
+// Each trial maximises the convolved-uncertainty energy
+//   E(R,t) = sum_ij N(y_i - R x_j - t; (s_i + s_j + a) I)
+// (isotropic covariances s, annealing variance a -> 0) by
+// majorise-minimise steps: with e_ij at the current pose, exp(-u) >= e0 (1 -
+// (u - u0)) makes E grow whenever sum_ij (e_ij / s_ij) |y_i - R x_j - t|^2
+// shrinks, and that weighted least-squares problem has the closed-form
+// (Horn) solution.  One iteration for all trials is three kernels:
+//   k_batch_transform  x'_j = R_k x_j + t_k - c_k (FP64 -> FP32 hi/lo), s_j + a_k
+//   k_batch_pairs      persistent; workgroups pop (trial, target tile, source
+//                      chunk) items from a queue ordered longest first and
+//                      write FP64 sufficient statistics per item
+//   k_batch_update     per trial: fixed-order sum of its items, Horn's
+//                      quaternion solution (4x4 Jacobi, FP64), pose update,
+//                      annealing, convergence; re-arms the queue
+// captured once into a hipGraph and replayed.  The same kernels run one
+// trial per launch for the baseline.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "dugma_internal.h"
+#include "dugma_pairs.h"
+
+namespace dugma {
+
+constexpr int kBBlock = 256;
+constexpr int kBSub = 64;
+constexpr int kStat = 20;  // per-item statistics (doubles)
+
+struct BItem {
+  int trial, t0, tn, s0, sn, pad0, pad1, pad2;
+};
+
+struct BTrial {
+  double R[9];  // column-major
+  double t[3];
+  double c[3];  // target centroid (targets are stored relative to it)
+  double a;     // annealing variance
+  double a_stop;
+  double E;     // energy at the last iteration (without the (2 pi)^-3/2 factor)
+  double dang, dtr;  // last rotation / translation step
+  int iters;
+  int status;   // 0 running, 1 converged, 2 iteration limit, 3 degenerate
+  int pad[2];
+};
+
+struct BParams {
+  double beta;       // annealing decay per iteration
+  double tol_rot;    // radians
+  double tol_trans;  // in the trial's own units
+  int max_iter;
+  int pad;
+};
+
+__global__ __launch_bounds__(kBBlock) void k_batch_transform(
+    const double *__restrict__ src, const int *__restrict__ src_trial, int s0,
+    int s1, const BTrial *__restrict__ trials, const int *__restrict__ done,
+    Rec3I *__restrict__ S) {
+  const int j = s0 + blockIdx.x * kBBlock + threadIdx.x;
+  if (j >= s1) return;
+  const int k = src_trial[j];
+  if (done[k]) return;
+  const BTrial &tr = trials[k];
+  const double *x = src + (size_t)j * 4;
+  Rec3I r;
+  double v[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double s = tr.t[a] - tr.c[a];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) s = fma(tr.R[a + 3 * b], x[b], s);
+    v[a] = s;
+  }
+  r.hx = (float)v[0];
+  r.hy = (float)v[1];
+  r.hz = (float)v[2];
+  r.lx = (float)(v[0] - (double)r.hx);
+  r.ly = (float)(v[1] - (double)r.hy);
+  r.lz = (float)(v[2] - (double)r.hz);
+  r.s = (float)(x[3] + tr.a);
+  r.pad0 = 0.0f;
+  S[j] = r;
+}
+
+// Statistics of one item, per target i (lane): W_i = sum_j w_ij,
+// B_i = sum_j w_ij x'_j, E_i = sum_j e_ij, Q_i = sum_j w_ij |d_ij|^2 with
+// w = e / s.  Item totals: [E, W, sum W_i y_i (3), sum B_i (3),
+// sum y_i B_i^T (9, row a = y_a), sum Q_i].
+__global__ __launch_bounds__(kBBlock) void k_batch_pairs(
+    const Rec3I *__restrict__ T, const Rec3I *__restrict__ S,
+    const BItem *__restrict__ items, const int *__restrict__ order, int nitems,
+    unsigned *__restrict__ queue, const int *__restrict__ done,
+    double *__restrict__ part) {
+  __shared__ int s_p;
+  __shared__ double red[kBBlock / 64][kStat];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (;;) {
+    if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(s_p);
+    __syncthreads();
+    if (p >= nitems) break;
+    const int idx = __builtin_amdgcn_readfirstlane(order[p]);
+    const BItem it = items[idx];
+    if (done[it.trial]) continue;
+    const int ti = it.t0 + wave * 64 + lane;
+    const bool valid = (wave * 64 + lane) < it.tn;
+    const Rec3I pt = T[valid ? ti : it.t0];
+    double dE = 0.0, dW = 0.0, dB[3] = {0.0, 0.0, 0.0}, dQ = 0.0;
+    const int s_end = it.s0 + it.sn;
+    for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
+      float fE = 0.0f, fW = 0.0f, fB0 = 0.0f, fB1 = 0.0f, fB2 = 0.0f, fQ = 0.0f;
+      const int j1 = min(j0 + kBSub, s_end);
+#pragma unroll 2
+      for (int j = j0; j < j1; ++j) {
+        const Rec3I q = S[j];
+        const float dx = (pt.hx - q.hx) - q.lx;
+        const float dy = (pt.hy - q.hy) - q.ly;
+        const float dz = (pt.hz - q.hz) - q.lz;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float rs = frsq(pt.s + q.s);
+        const float inv = rs * rs;
+        const float e = fexp2((d2 * inv) * kNegHalfLog2e) * (rs * inv);
+        const float w = e * inv;
+        fE += e;
+        fW += w;
+        fB0 = fmaf(w, q.hx, fB0);
+        fB1 = fmaf(w, q.hy, fB1);
+        fB2 = fmaf(w, q.hz, fB2);
+        fQ = fmaf(w, d2, fQ);
+      }
+      dE += fE;
+      dW += fW;
+      dB[0] += fB0;
+      dB[1] += fB1;
+      dB[2] += fB2;
+      dQ += fQ;
+    }
+    double r[kStat];
+#pragma unroll
+    for (int a = 0; a < kStat; ++a) r[a] = 0.0;
+    if (valid) {
+      const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
+      r[0] = dE;
+      r[1] = dW;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        r[2 + a] = dW * y[a];
+        r[5 + a] = dB[a];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) r[8 + 3 * a + b] = y[a] * dB[b];
+      }
+      r[17] = dQ;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int a = 0; a < 18; ++a) r[a] += __shfl_xor(r[a], off);
+    if (lane == 0)
+#pragma unroll
+      for (int a = 0; a < kStat; ++a) red[wave][a] = r[a];
+    __syncthreads();
+    if (threadIdx.x < kStat) {
+      double s = red[0][threadIdx.x];
+#pragma unroll
+      for (int w = 1; w < kBBlock / 64; ++w) s += red[w][threadIdx.x];
+      part[(size_t)idx * kStat + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Largest-eigenvalue eigenvector of a symmetric 4x4 (cyclic Jacobi, FP64).
+__device__ void eig4_max(double A[4][4], double q[4]) {
+  double V[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) V[a][b] = a == b ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        tot += A[a][b] * A[a][b];
+        if (a != b) off += A[a][b] * A[a][b];
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < 3; ++p)
+      for (int r = p + 1; r < 4; ++r) {
+        if (A[p][r] == 0.0) continue;
+        const double th = 0.5 * (A[r][r] - A[p][p]) / A[p][r];
+        const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
+        for (int k = 0; k < 4; ++k) {  // A <- A J
+          const double akp = A[k][p], akr = A[k][r];
+          A[k][p] = c * akp - s * akr;
+          A[k][r] = s * akp + c * akr;
+        }
+        for (int k = 0; k < 4; ++k) {  // A <- J^T A
+          const double apk = A[p][k], ark = A[r][k];
+          A[p][k] = c * apk - s * ark;
+          A[r][k] = s * apk + c * ark;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double vkp = V[k][p], vkr = V[k][r];
+          V[k][p] = c * vkp - s * vkr;
+          V[k][r] = s * vkp + c * vkr;
+        }
+      }
+  }
+  int best = 0;
+  for (int a = 1; a < 4; ++a)
+    if (A[a][a] > A[best][best]) best = a;
+  for (int a = 0; a < 4; ++a) q[a] = V[a][best];
+}
+
+__global__ __launch_bounds__(kBBlock) void k_batch_update(
+    const double *__restrict__ part, const int *__restrict__ titem, int k0,
+    int k1, BTrial *__restrict__ trials, int *__restrict__ done,
+    unsigned *__restrict__ queue, BParams bp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+  const int k = k0 + blockIdx.x * kBBlock + threadIdx.x;
+  if (k >= k1 || done[k]) return;
+  BTrial tr = trials[k];
+  double st[kStat];
+  for (int a = 0; a < kStat; ++a) st[a] = 0.0;
+  for (int i = titem[k]; i < titem[k + 1]; ++i)
+    for (int a = 0; a < 18; ++a) st[a] += part[(size_t)i * kStat + a];
+  const double W = st[1];
+  tr.E = st[0];
+  tr.iters += 1;
+  if (!(W > 0.0) || !isfinite(W)) {
+    tr.status = 3;  // no pair has any weight: nothing to align
+    trials[k] = tr;
+    done[k] = 1;
+    return;
+  }
+  double my[3], mx[3], H[3][3];
+  for (int a = 0; a < 3; ++a) {
+    my[a] = st[2 + a] / W;
+    mx[a] = st[5 + a] / W;
+  }
+  // H_ab = sum w (x'_a - mx_a)(y_b - my_b); st[8+3a+b] = sum y_a x'_b
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) H[a][b] = st[8 + 3 * b + a] - W * mx[a] * my[b];
+  const double Sxx = H[0][0], Sxy = H[0][1], Sxz = H[0][2];
+  const double Syx = H[1][0], Syy = H[1][1], Syz = H[1][2];
+  const double Szx = H[2][0], Szy = H[2][1], Szz = H[2][2];
+  double N4[4][4] = {{Sxx + Syy + Szz, Syz - Szy, Szx - Sxz, Sxy - Syx},
+                     {Syz - Szy, Sxx - Syy - Szz, Sxy + Syx, Szx + Sxz},
+                     {Szx - Sxz, Sxy + Syx, -Sxx + Syy - Szz, Syz + Szy},
+                     {Sxy - Syx, Szx + Sxz, Syz + Szy, -Sxx - Syy + Szz}};
+  double q[4];
+  eig4_max(N4, q);
+  const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  for (int a = 0; a < 4; ++a) q[a] /= qn;
+  const double w0 = q[0], x = q[1], y = q[2], z = q[3];
+  double dR[3][3] = {{w0 * w0 + x * x - y * y - z * z, 2 * (x * y - w0 * z), 2 * (x * z + w0 * y)},
+                     {2 * (x * y + w0 * z), w0 * w0 - x * x + y * y - z * z, 2 * (y * z - w0 * x)},
+                     {2 * (x * z - w0 * y), 2 * (y * z + w0 * x), w0 * w0 - x * x - y * y + z * z}};
+  double dt[3];
+  for (int a = 0; a < 3; ++a) dt[a] = my[a] - (dR[a][0] * mx[0] + dR[a][1] * mx[1] + dR[a][2] * mx[2]);
+  // x'_new = dR x' + dt with x' = R x + t - c
+  double Rn[9], tc[3];
+  for (int a = 0; a < 3; ++a) {
+    for (int b = 0; b < 3; ++b)
+      Rn[a + 3 * b] = dR[a][0] * tr.R[0 + 3 * b] + dR[a][1] * tr.R[1 + 3 * b] + dR[a][2] * tr.R[2 + 3 * b];
+    tc[a] = tr.t[a] - tr.c[a];
+  }
+  double dtr2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double v = dR[a][0] * tc[0] + dR[a][1] * tc[1] + dR[a][2] * tc[2] + dt[a];
+    const double tn = tr.c[a] + v;
+    dtr2 += (tn - tr.t[a]) * (tn - tr.t[a]);
+    tr.t[a] = tn;
+  }
+  for (int a = 0; a < 9; ++a) tr.R[a] = Rn[a];
+  const double ca = fmin(1.0, fmax(-1.0, 0.5 * (dR[0][0] + dR[1][1] + dR[2][2] - 1.0)));
+  tr.dang = acos(ca);
+  tr.dtr = sqrt(dtr2);
+  const bool annealing = tr.a > 0.0;
+  tr.a *= bp.beta;
+  if (tr.a < tr.a_stop) tr.a = 0.0;
+  if (!annealing && tr.dang < bp.tol_rot && tr.dtr < bp.tol_trans) {
+    tr.status = 1;
+    done[k] = 1;
+  } else if (tr.iters >= bp.max_iter) {
+    tr.status = 2;
+    done[k] = 1;
+  }
+  trials[k] = tr;
+}
+
+}  // namespace dugma
+
+using namespace dugma;
+
+struct dugma_batch {
+  int ntrials = 0;
+  hipStream_t stream = nullptr;
+  int ntgt = 0, nsrc = 0, nitems = 0;
+  Rec3I *dT = nullptr;       // targets (all trials), relative to each trial's centroid
+  Rec3I *dS = nullptr;       // per-iteration source records
+  double *dsrc = nullptr;    // sources: x y z s per point (FP64, FP32-rounded inputs)
+  int *dsrc_trial = nullptr;
+  BItem *ditems = nullptr;
+  int *dorder = nullptr;     // queue position -> item, longest first
+  int *dtitem = nullptr;     // CSR: items of trial k are [titem[k], titem[k+1])
+  double *dpart = nullptr;   // per-item statistics
+  BTrial *dtrials = nullptr;
+  int *ddone = nullptr;
+  unsigned *dqueue = nullptr;
+  int *dident = nullptr;     // identity order for one-trial launches
+  std::vector<int> src_off, titem;
+  std::vector<double> a0, astop, cent;  // per trial
+  long long pairs = 0;                  // target x source pairs of one full iteration
+  int grid = 2048;
+};
+
+static void free_batch(dugma_batch *b) {
+  if (!b) return;
+  void *ptrs[] = {b->dT, b->dS, b->dsrc, b->dsrc_trial, b->ditems, b->dorder, b->dtitem,
+                  b->dpart, b->dtrials, b->ddone, b->dqueue, b->dident};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+template <class T>
+static int bmalloc(T **p, size_t n) {
+  *p = nullptr;
+  hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, "hipMalloc(batch)");
+  }
+  return DUGMA_OK;
+}
+
+// Variance of point i of trial k's target (which = 0) or source (1) cloud:
+// > 0 for an isotropic positive covariance, -1 for a point to leave out,
+// -2 for a covariance that is not s I.
+typedef double (*VarFn)(const void *ctx, int k, int which, int i);
+
+struct BlockVars {
+  const double *const *XC, *const *YC;
+};
+static double block_var(const void *c, int k, int which, int i) {
+  const BlockVars *v = (const BlockVars *)c;
+  const double *C = (which ? v->YC[k] : v->XC[k]) + (size_t)i * 9;
+  for (int a = 0; a < 9; ++a)
+    if (!std::isfinite(C[a])) return -1.0;
+  if (C[1] != 0 || C[2] != 0 || C[3] != 0 || C[5] != 0 || C[6] != 0 || C[7] != 0 ||
+      C[0] != C[4] || C[0] != C[8])
+    return -2.0;
+  const float c0 = (float)C[0];
+  return c0 > 0 ? (double)c0 : -1.0;
+}
+
+struct IsoVars {
+  const double *const *sX, *const *sY;
+};
+static double iso_var(const void *c, int k, int which, int i) {
+  const IsoVars *v = (const IsoVars *)c;
+  const double s = (which ? v->sY[k] : v->sX[k])[i];
+  const float s32 = (float)s;
+  return (std::isfinite(s) && s32 > 0) ? (double)s32 : -1.0;
+}
+
+static int build_batch(int ntrials, const int *N, const int *M, const double *const *X,
+                       const double *const *Y, VarFn var, const void *vctx,
+                       dugma_batch **out) {
+  dugma_batch *b = new dugma_batch();
+  b->ntrials = ntrials;
+  std::vector<Rec3I> tg;
+  std::vector<double> sr;
+  std::vector<int> strial;
+  std::vector<BItem> items;
+  b->titem.assign(ntrials + 1, 0);
+  b->a0.resize(ntrials);
+  b->astop.resize(ntrials);
+  b->cent.resize(3 * (size_t)ntrials);
+  const int tile_t = kBBlock, tile_s = 2048;
+  for (int k = 0; k < ntrials; ++k) {
+    if (N[k] < 1 || M[k] < 1 || !X[k] || !Y[k]) {
+      free_batch(b);
+      return fail(DUGMA_ERR_ARG, "dugma_batch_create: empty trial");
+    }
+    std::vector<double> ty;
+    std::vector<double> tv;
+    double c[3] = {0, 0, 0};
+    for (int i = 0; i < N[k]; ++i) {
+      const double *p = X[k] + (size_t)i * 3;
+      const double v = var(vctx, k, 0, i);
+      if (v == -2.0) {
+        free_batch(b);
+        return fail(DUGMA_ERR_ARG, "dugma_batch_create: covariances must be isotropic (s I)");
+      }
+      if (v < 0 || !std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+      for (int a = 0; a < 3; ++a) {
+        ty.push_back((double)(float)p[a]);
+        c[a] += (double)(float)p[a];
+      }
+      tv.push_back(v);
+    }
+    const int nt = (int)tv.size();
+    double r2 = 0.0, vbar = 0.0;
+    if (nt > 0)
+      for (int a = 0; a < 3; ++a) c[a] /= nt;
+    const int tbase = (int)tg.size();
+    for (int i = 0; i < nt; ++i) {
+      Rec3I r;
+      memset(&r, 0, sizeof(r));
+      r.hx = (float)(ty[3 * i] - c[0]);
+      r.hy = (float)(ty[3 * i + 1] - c[1]);
+      r.hz = (float)(ty[3 * i + 2] - c[2]);
+      r.s = (float)tv[i];
+      r2 += (double)r.hx * r.hx + (double)r.hy * r.hy + (double)r.hz * r.hz;
+      vbar += tv[i];
+      tg.push_back(r);
+    }
+    const int sbase = (int)(sr.size() / 4);
+    int ns = 0;
+    for (int j = 0; j < M[k]; ++j) {
+      const double *p = Y[k] + (size_t)j * 3;
+      const double v = var(vctx, k, 1, j);
+      if (v == -2.0) {
+        free_batch(b);
+        return fail(DUGMA_ERR_ARG, "dugma_batch_create: covariances must be isotropic (s I)");
+      }
+      if (v < 0 || !std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+      for (int a = 0; a < 3; ++a) sr.push_back((double)(float)p[a]);
+      sr.push_back(v);
+      strial.push_back(k);
+      vbar += v;
+      ++ns;
+    }
+    for (int a = 0; a < 3; ++a) b->cent[3 * k + a] = c[a];
+    // annealing from a quarter of the target's mean squared radius down to
+    // a hundredth of the mean point variance
+    b->pairs += (long long)nt * ns;
+    b->a0[k] = nt ? 0.25 * r2 / nt : 0.0;
+    b->astop[k] = (nt + ns) ? 0.01 * vbar / (nt + ns) : 0.0;
+    b->titem[k] = (int)items.size();
+    for (int t0 = 0; t0 < nt; t0 += tile_t)
+      for (int s0 = 0; s0 < ns; s0 += tile_s) {
+        BItem it;
+        memset(&it, 0, sizeof(it));
+        it.trial = k;
+        it.t0 = tbase + t0;
+        it.tn = std::min(tile_t, nt - t0);
+        it.s0 = sbase + s0;
+        it.sn = std::min(tile_s, ns - s0);
+        items.push_back(it);
+      }
+  }
+  b->titem[ntrials] = (int)items.size();
+  b->ntgt = (int)tg.size();
+  b->nsrc = (int)(sr.size() / 4);
+  b->nitems = (int)items.size();
+  // queue order: most work first
+  std::vector<int> order(items.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+    return (long long)items[u].tn * items[u].sn > (long long)items[v].tn * items[v].sn;
+  });
+  std::vector<int> ident(items.size());
+  std::iota(ident.begin(), ident.end(), 0);
+  int rc = DUGMA_OK;
+#define BTRY(x)                     \
+  do {                              \
+    rc = (x);                       \
+    if (rc != DUGMA_OK) goto fail_; \
+  } while (0)
+  {
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) BTRY(hip_fail(e, "hipStreamCreateWithFlags"));
+    BTRY(bmalloc(&b->dT, tg.size()));
+    BTRY(bmalloc(&b->dS, (size_t)b->nsrc));
+    BTRY(bmalloc(&b->dsrc, sr.size()));
+    BTRY(bmalloc(&b->dsrc_trial, strial.size()));
+    BTRY(bmalloc(&b->ditems, items.size()));
+    BTRY(bmalloc(&b->dorder, order.size()));
+    BTRY(bmalloc(&b->dident, ident.size()));
+    BTRY(bmalloc(&b->dtitem, b->titem.size()));
+    BTRY(bmalloc(&b->dpart, items.size() * kStat));
+    BTRY(bmalloc(&b->dtrials, (size_t)ntrials));
+    BTRY(bmalloc(&b->ddone, (size_t)ntrials));
+    BTRY(bmalloc(&b->dqueue, 1));
+    struct Up {
+      void *d;
+      const void *h;
+      size_t n;
+    } ups[] = {{b->dT, tg.data(), tg.size() * sizeof(Rec3I)},
+               {b->dsrc, sr.data(), sr.size() * sizeof(double)},
+               {b->dsrc_trial, strial.data(), strial.size() * sizeof(int)},
+               {b->ditems, items.data(), items.size() * sizeof(BItem)},
+               {b->dorder, order.data(), order.size() * sizeof(int)},
+               {b->dident, ident.data(), ident.size() * sizeof(int)},
+               {b->dtitem, b->titem.data(), b->titem.size() * sizeof(int)}};
+    for (const Up &u : ups)
+      if (u.n) {
+        e = hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) BTRY(hip_fail(e, "hipMemcpy(batch)"));
+      }
+    e = hipMemset(b->dqueue, 0, sizeof(unsigned));
+    if (e != hipSuccess) BTRY(hip_fail(e, "hipMemset(queue)"));
+  }
+#undef BTRY
+  *out = b;
+  return DUGMA_OK;
+fail_:
+  free_batch(b);
+  return rc;
+}
+
+extern "C" int dugma_batch_create(int ntrials, int D, const int *N, const int *M,
+                                  const double *const *X, const double *const *X_C,
+                                  const double *const *Y, const double *const *Y_C,
+                                  dugma_batch **out) {
+  if (out) *out = nullptr;
+  if (!out || !N || !M || !X || !X_C || !Y || !Y_C || ntrials < 1)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_create: bad argument");
+  if (D != 3) return fail(DUGMA_ERR_ARG, "dugma_batch_create: batched registration is 3-D");
+  for (int k = 0; k < ntrials; ++k)
+    if (!X_C[k] || !Y_C[k]) return fail(DUGMA_ERR_ARG, "dugma_batch_create: null covariance");
+  BlockVars v{X_C, Y_C};
+  return build_batch(ntrials, N, M, X, Y, block_var, &v, out);
+}
+
+extern "C" int dugma_batch_create_iso(int ntrials, const int *N, const int *M,
+                                      const double *const *X, const double *const *sX,
+                                      const double *const *Y, const double *const *sY,
+                                      dugma_batch **out) {
+  if (out) *out = nullptr;
+  if (!out || !N || !M || !X || !sX || !Y || !sY || ntrials < 1)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_create_iso: bad argument");
+  for (int k = 0; k < ntrials; ++k)
+    if (!sX[k] || !sY[k]) return fail(DUGMA_ERR_ARG, "dugma_batch_create_iso: null variances");
+  IsoVars v{sX, sY};
+  return build_batch(ntrials, N, M, X, Y, iso_var, &v, out);
+}
+
+extern "C" int dugma_batch_free(dugma_batch *b) {
+  if (!b) return fail(DUGMA_ERR_ARG, "dugma_batch_free: null batch");
+  (void)hipStreamSynchronize(b->stream);
+  free_batch(b);
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
+                                long long *pairs_per_iteration) {
+  if (!b) return fail(DUGMA_ERR_ARG, "dugma_batch_info: null batch");
+  if (ntrials) *ntrials = b->ntrials;
+  if (nitems) *nitems = b->nitems;
+  if (pairs_per_iteration) *pairs_per_iteration = b->pairs;
+  return DUGMA_OK;
+}
+
+static int launch_iteration(dugma_batch *b, int k0, int k1, const BParams &bp,
+                            bool one_trial) {
+  // sources of trials [k0, k1)
+  int s0 = 0, s1 = b->nsrc;
+  const int *order = b->dorder;
+  int i0 = 0, ni = b->nitems;
+  if (one_trial) {
+    // items of trial k0 are contiguous: [titem[k0], titem[k0+1])
+    i0 = b->titem[k0];
+    ni = b->titem[k0 + 1] - i0;
+    order = b->dident + i0;
+    s0 = b->src_off[k0];
+    s1 = b->src_off[k0 + 1];
+  }
+  if (s1 > s0)
+    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + kBBlock - 1) / kBBlock), dim3(kBBlock), 0, b->stream,
+                       b->dsrc, b->dsrc_trial, s0, s1, b->dtrials, b->ddone, b->dS);
+  if (ni > 0)
+    hipLaunchKernelGGL(k_batch_pairs, dim3(std::min(b->grid, ni)), dim3(kBBlock), 0, b->stream, b->dT, b->dS,
+                       b->ditems, order, ni, b->dqueue, b->ddone, b->dpart);
+  hipLaunchKernelGGL(k_batch_update, dim3((k1 - k0 + kBBlock - 1) / kBBlock), dim3(kBBlock), 0, b->stream,
+                     b->dpart, b->dtitem, k0, k1, b->dtrials, b->ddone, b->dqueue, bp);
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
+                                    const double *t0, const dugma_batch_opts *opts,
+                                    double *R, double *t, int *iters,
+                                    int *status, double *E) {
+  if (!b || !R0 || !t0 || !opts) return fail(DUGMA_ERR_ARG, "dugma_batch_register: bad argument");
+  if (opts->max_iter < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_register: max_iter must be >= 1");
+  const int K = b->ntrials;
+  // source offsets per trial (host), for one-trial launches
+  if (b->src_off.empty()) {
+    std::vector<int> strial(b->nsrc);
+    hipError_t e = hipMemcpy(strial.data(), b->dsrc_trial, sizeof(int) * b->nsrc, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(src_trial)");
+    b->src_off.assign(K + 1, 0);
+    for (int j = 0; j < b->nsrc; ++j) b->src_off[strial[j] + 1]++;
+    for (int k = 0; k < K; ++k) b->src_off[k + 1] += b->src_off[k];
+  }
+  std::vector<BTrial> tr(K);
+  for (int k = 0; k < K; ++k) {
+    memset(&tr[k], 0, sizeof(BTrial));
+    for (int a = 0; a < 9; ++a) tr[k].R[a] = R0[9 * (size_t)k + a];
+    for (int a = 0; a < 3; ++a) {
+      tr[k].t[a] = t0[3 * (size_t)k + a];
+      tr[k].c[a] = b->cent[3 * (size_t)k + a];
+    }
+    tr[k].a = opts->anneal ? b->a0[k] : 0.0;
+    tr[k].a_stop = b->astop[k];
+  }
+  std::vector<int> done(K, 0);
+  DUGMA_HIP(hipMemcpy(b->dtrials, tr.data(), sizeof(BTrial) * K, hipMemcpyHostToDevice));
+  DUGMA_HIP(hipMemcpy(b->ddone, done.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+  BParams bp;
+  memset(&bp, 0, sizeof(bp));
+  bp.beta = opts->anneal_decay > 0 ? opts->anneal_decay : 0.85;
+  bp.tol_rot = opts->tol_rot;
+  bp.tol_trans = opts->tol_trans;
+  bp.max_iter = opts->max_iter;
+  b->grid = opts->grid > 0 ? opts->grid : 2048;
+  int rc = DUGMA_OK;
+  if (opts->one_trial_per_launch) {
+    // baseline: each trial on its own, 3 launches and a host check per iteration
+    int *hdone = nullptr;
+    DUGMA_HIP(hipHostMalloc((void **)&hdone, sizeof(int), hipHostMallocDefault));
+    for (int k = 0; k < K && !rc; ++k) {
+      for (int it = 0; it < opts->max_iter && !rc; ++it) {
+        rc = launch_iteration(b, k, k + 1, bp, true);
+        if (rc) break;
+        hipError_t e = hipMemcpyAsync(hdone, b->ddone + k, sizeof(int), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "one-trial iteration");
+        if (*hdone) break;
+      }
+    }
+    (void)hipHostFree(hdone);
+  } else if (opts->use_graph) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    DUGMA_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_iteration(b, 0, K, bp, false);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    if (!rc) {
+      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+    const int check = opts->check_every > 0 ? opts->check_every : 10;
+    for (int it = 0; it < opts->max_iter && !rc; ++it) {
+      e = hipGraphLaunch(ge, b->stream);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "hipGraphLaunch");
+        break;
+      }
+      if ((it + 1) % check == 0) {
+        // all trials finished?  (done flags summed on the host)
+        std::vector<int> hd(K);
+        e = hipMemcpyAsync(hd.data(), b->ddone, sizeof(int) * K, hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+          rc = hip_fail(e, "graph iteration");
+          break;
+        }
+        int nd = 0;
+        for (int v : hd) nd += v;
+        if (nd == K) break;
+      }
+    }
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  } else {
+    for (int it = 0; it < opts->max_iter && !rc; ++it) rc = launch_iteration(b, 0, K, bp, false);
+  }
+  if (rc) return rc;
+  DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
+  DUGMA_HIP(hipStreamSynchronize(b->stream));
+  const double cn = 1.0 / (6.283185307179586476925 * std::sqrt(6.283185307179586476925));
+  for (int k = 0; k < K; ++k) {
+    if (R)
+      for (int a = 0; a < 9; ++a) R[9 * (size_t)k + a] = tr[k].R[a];
+    if (t)
+      for (int a = 0; a < 3; ++a) t[3 * (size_t)k + a] = tr[k].t[a];
+    if (iters) iters[k] = tr[k].iters;
+    if (status) status[k] = tr[k].status;
+    if (E) E[k] = cn * tr[k].E;
+  }
+  return DUGMA_OK;
+}

@@ -69,13 +69,12 @@ class Surface:
         ph = np.linspace(0, 2 * math.pi, nph + 1)[:-1]
         TH, PH = np.meshgrid(th, ph, indexing="ij")
         V = self.points(TH.ravel(), PH.ravel()).reshape(nt + 1, nph, 3)
-        tris = []
-        for a in range(nt):
-            for b in range(nph):
-                b2 = (b + 1) % nph
-                tris.append((V[a, b], V[a + 1, b], V[a + 1, b2]))
-                tris.append((V[a, b], V[a + 1, b2], V[a, b2]))
-        T = np.array(tris)  # (nt*nph*2, 3, 3)
+        a = np.arange(nt)[:, None]
+        b = np.arange(nph)[None, :]
+        b2 = (b + 1) % nph
+        t1 = np.stack([V[a, b], V[a + 1, b], V[a + 1, b2]], axis=2).reshape(-1, 3, 3)
+        t2 = np.stack([V[a, b], V[a + 1, b2], V[a, b2]], axis=2).reshape(-1, 3, 3)
+        T = np.stack([t1, t2], axis=1).reshape(-1, 3, 3)  # (nt*nph*2, 3, 3)
         area = 0.5 * np.linalg.norm(np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0]), axis=1)
         keep = area > 0
         self.T = T[keep]
@@ -103,10 +102,10 @@ class Surface:
         return T[:, 0] + u[:, None] * (T[:, 1] - T[:, 0]) + v[:, None] * (T[:, 2] - T[:, 0])
 
 
-def knn_sigma(P, k=8):
+def knn_sigma(P, k=8, workers=-1):
     """Mean distance to the k nearest neighbours within the cloud (self excluded)."""
     from scipy.spatial import cKDTree
-    d, _ = cKDTree(P).query(P, k=k + 1)
+    d, _ = cKDTree(P).query(P, k=k + 1, workers=workers)
     return d[:, 1:].mean(axis=1)
 
 
@@ -224,7 +223,40 @@ def config3(trial=0, n=200000, m=200000):
                 R_G=R, t_G=t)
 
 
-def config4_trial(trial, surfaces=None):
+_C4_SURFACES = None
+
+
+def _c4_surfaces(nsurf=32):
+    global _C4_SURFACES
+    if _C4_SURFACES is None or len(_C4_SURFACES) != nsurf:
+        rng = np.random.default_rng(seed(4, 10 ** 6))
+        _C4_SURFACES = [Surface(rng, nt=60, nph=120) for _ in range(nsurf)]
+    return _C4_SURFACES
+
+
+def _c4_chunk(ks):
+    surfs = _c4_surfaces()
+    return [config4_trial(k, surfs, workers=1) for k in ks]
+
+
+def config4(ntrials=4096, nsurf=32, procs=None):
+    """All batched trials (shared pool of surfaces); large sets are generated
+    in worker processes (at most 16)."""
+    import os
+    procs = procs if procs is not None else min(16, os.cpu_count() or 1)
+    if ntrials < 256 or procs <= 1:
+        surfs = _c4_surfaces(nsurf)
+        return [config4_trial(k, surfs) for k in range(ntrials)]
+    import multiprocessing as mp
+    chunks = [list(range(a, min(a + 64, ntrials))) for a in range(0, ntrials, 64)]
+    with mp.get_context("fork").Pool(procs) as pool:
+        out = []
+        for part in pool.imap(_c4_chunk, chunks):
+            out.extend(part)
+    return out
+
+
+def config4_trial(trial, surfaces=None, workers=-1):
     """One batched trial: N, M log-uniform in [500, 8000], angle U(0, 90) deg,
     noise sigma U(0.001, 0.02), outlier fraction U(0, 0.3)."""
     rng = np.random.default_rng(seed(4, trial))
@@ -243,8 +275,10 @@ def config4_trial(trial, surfaces=None):
         lo, hi = tgt.min(axis=0), tgt.max(axis=0)
         idx = rng.choice(n, size=k, replace=False)
         tgt[idx] = rng.uniform(lo, hi, size=(k, 3))
-    sx, sy = knn_sigma(tgt), knn_sigma(src)
-    return dict(X=tgt.T.copy(), X_C=iso_blocks(sx ** 2), Y=src.T.copy(), Y_C=iso_blocks(sy ** 2),
+    sx, sy = knn_sigma(tgt, workers=workers), knn_sigma(src, workers=workers)
+    # isotropic variances per point (Batch takes them directly; iso_blocks
+    # turns them into the D x (D*n) layout of init)
+    return dict(X=tgt.T.copy(), sX=sx ** 2, Y=src.T.copy(), sY=sy ** 2,
                 R_G=R, t_G=t, noise=noise, outliers=outl)
 
 

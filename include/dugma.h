@@ -136,6 +136,48 @@ enum {
 int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
               const double *t, int *idx, float *d2);
 
+/* ---- batched registration (no reference counterpart) -------------------- */
+
+/* Many independent 3-D rigid registrations of uneven sizes at once; each
+ * maximises the convolved energy with isotropic covariances (every X_C / Y_C
+ * block must be s I) by majorise-minimise steps (weighted Procrustes, FP64)
+ * under an annealed extra variance a I, a -> 0.  The DUGMA.m:57 loop for one
+ * trial is the special case ntrials = 1. */
+typedef struct dugma_batch dugma_batch;
+
+typedef struct {
+  int max_iter;              /* iteration limit per trial */
+  int anneal;                /* 1: start at a = mean squared target radius / 4 */
+  double anneal_decay;       /* a <- a * decay per iteration (0: 0.85) */
+  double tol_rot;            /* converged when a = 0 and the step is below these */
+  double tol_trans;
+  int use_graph;             /* replay one captured iteration (hipGraph) */
+  int one_trial_per_launch;  /* baseline: trials one after another, host check per step */
+  int check_every;           /* graph mode: host checks for completion every n steps */
+  int grid;                  /* workgroups of the persistent pair kernel (0: 2048) */
+} dugma_batch_opts;
+
+/* Arrays of ntrials pointers, each in the layout of dugma_init. */
+int dugma_batch_create(int ntrials, int D, const int *N, const int *M,
+                       const double *const *X, const double *const *X_C,
+                       const double *const *Y, const double *const *Y_C,
+                       dugma_batch **out);
+/* The same with one variance per point (covariance s I): sX[k] has N[k]
+ * entries, sY[k] has M[k]. */
+int dugma_batch_create_iso(int ntrials, const int *N, const int *M,
+                           const double *const *X, const double *const *sX,
+                           const double *const *Y, const double *const *sY,
+                           dugma_batch **out);
+/* R0 / R: ntrials x 9 column-major, t0 / t: ntrials x 3; iters, status
+ * (1 converged, 2 iteration limit, 3 no weight), E: ntrials each (any output
+ * may be NULL). */
+int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
+                         const dugma_batch_opts *opts, double *R, double *t,
+                         int *iters, int *status, double *E);
+int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
+                     long long *pairs_per_iteration);
+int dugma_batch_free(dugma_batch *b);
+
 /* ---- the six reference handles ------------------------------------------ */
 /* Device pointers of the state the reference keeps behind its six MEX handles
  * (init.cu:326-331, in that order): X, inverse X_C, Y, inverse Y_C, g_X, g_Y
