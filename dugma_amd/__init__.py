@@ -143,6 +143,50 @@ def fr_m(h):
     return h.free()
 
 
+def poly_eval(coef, dT, grad=False):
+    """F = energy3D(dT) / energy2D(dT) (and dF/ddT) for compi's coefficients."""
+    coef = np.ascontiguousarray(coef, np.float64)
+    dT = np.ascontiguousarray(dT, np.float64)
+    D = 3 if coef.size == 637 else 2
+    g = np.zeros(dT.size) if grad else None
+    F = _lib.load().dugma_poly_eval(D, _dp(coef), _dp(dT), _dp(g) if grad else None)
+    return (F, g) if grad else F
+
+
+def poly_minimize(coef):
+    """dT minimising the surrogate over rotations dR and translations dt
+    (what fmincon('energy3D', ..., 'fcontr3D') returns in DUGMA.m:73)."""
+    coef = np.ascontiguousarray(coef, np.float64)
+    D = 3 if coef.size == 637 else 2
+    dT = np.zeros(12 if D == 3 else 4)
+    F = ctypes.c_double()
+    _lib.check(_lib.load().dugma_poly_minimize(D, _dp(coef), _dp(dT), ctypes.byref(F)))
+    return dT, F.value
+
+
+def DUGMA(X, Y, X_C, Y_C, R0, t0, Max_Iteration=300, accuracy_rotation=1e-4,
+          accuracy_translation=1e-8, accuracy_sig=1e-9, verbose=False):
+    """[R, t] = DUGMA(X, Y, X_C, Y_C, R0, t0, ...) with the argument order of
+    DUGMA.m:19; returns (R, t, iterations, sig)."""
+    X = _cm(X)
+    D, N = X.shape
+    Y = _cm(Y)
+    M = Y.shape[1]
+    X_C = _cm(X_C, (D, D * N))
+    Y_C = _cm(Y_C, (D, D * M))
+    R0 = _cm(R0, (D, D))
+    t0 = _cm(np.ravel(t0))
+    o = _lib.RegisterOpts(int(Max_Iteration), float(accuracy_rotation), float(accuracy_translation),
+                          float(accuracy_sig), int(bool(verbose)))
+    R = np.zeros((D, D), order="F")
+    t = np.zeros(D)
+    it = ctypes.c_int()
+    sig = ctypes.c_double()
+    _lib.check(_lib.load().dugma_register(_dp(X), _dp(X_C), _dp(Y), _dp(Y_C), N, M, D, _dp(R0), _dp(t0),
+                                          ctypes.byref(o), _dp(R), _dp(t), ctypes.byref(it), ctypes.byref(sig)))
+    return np.ascontiguousarray(R), t, it.value, sig.value
+
+
 def energy(h, R, t, cov="auto"):
     R = _cm(R, (h.D, h.D))
     t = _cm(np.ravel(t))

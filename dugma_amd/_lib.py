@@ -25,6 +25,13 @@ class BatchOpts(ctypes.Structure):
                 ("grid", ctypes.c_int)]
 
 
+class RegisterOpts(ctypes.Structure):
+    """dugma_register_opts of include/dugma.h."""
+    _fields_ = [("max_iter", ctypes.c_int), ("acc_rot", ctypes.c_double),
+                ("acc_trans", ctypes.c_double), ("acc_sig", ctypes.c_double),
+                ("verbose", ctypes.c_int)]
+
+
 _pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
 
 # name: (restype, argtypes) -- one entry per declaration of include/dugma.h
@@ -33,6 +40,9 @@ SIGNATURES = {
     "dugma_update_sig": (_ci, [_vp, _d, _d, _cd, _cd, _cd, _d]),
     "dugma_compi": (_ci, [_vp, _d, _d, _cd, _cd, _cd, _cd, _d]),
     "dugma_fr_m": (_ci, [_vp, _d]),
+    "dugma_poly_eval": (_cd, [_ci, _d, _d, _d]),
+    "dugma_poly_minimize": (_ci, [_ci, _d, _d, _d]),
+    "dugma_register": (_ci, [_d, _d, _d, _d, _ci, _ci, _ci, _d, _d, ctypes.POINTER(RegisterOpts), _d, _d, _i, _d]),
     "dugma_energy": (_ci, [_vp, _d, _d, _ci, _d]),
     "dugma_energy_grad": (_ci, [_vp, _d, _d, _ci, _d, _d]),
     "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdugma_hip.so")
 SOURCES = ["dugma_ctx.hip", "dugma_energy.hip", "dugma_sig.hip", "dugma_compi.hip",
-           "dugma_batch.hip"]
+           "dugma_batch.hip", "dugma_register.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

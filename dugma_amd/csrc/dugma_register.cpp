@@ -1,0 +1,368 @@
+// dugma_register.cpp -- host side of the reference's registration loop
+// (DUGMA.m:19-137, normalize.m:3-26) over the C ABI, and the surrogate
+// polynomial's objective / gradient / minimiser (energy3D.m, energy2D.m,
+// fcontr3D.m, fcontr2D.m) that MATLAB's fmincon provides in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dugma_internal.h"
+#include "dugma_monomials.inc"
+
+namespace dugma {
+
+// ---- the polynomial of energy3D.m / energy2D.m ---------------------------
+static void mono_table(int D, int *nm, int *nv) {
+  *nm = D == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
+  *nv = D == 3 ? 12 : 4;
+}
+
+static const unsigned char *mono(int D, int k) {
+  return D == 3 ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
+}
+
+// F = sum_k coef_k prod_v dT_v^e_kv and, if grad, dF/ddT.
+static double poly_value(int D, const double *coef, const double *dT, double *grad) {
+  int nm, nv;
+  mono_table(D, &nm, &nv);
+  double pw[12][5];
+  for (int v = 0; v < nv; ++v) {
+    pw[v][0] = 1.0;
+    for (int e = 1; e < 5; ++e) pw[v][e] = pw[v][e - 1] * dT[v];
+  }
+  if (grad)
+    for (int v = 0; v < nv; ++v) grad[v] = 0.0;
+  double F = 0.0;
+  for (int k = 0; k < nm; ++k) {
+    const unsigned char *e = mono(D, k);
+    double m = coef[k];
+    for (int v = 0; v < nv; ++v) m *= pw[v][e[v]];
+    F += m;
+    if (grad) {
+      for (int v = 0; v < nv; ++v) {
+        if (!e[v]) continue;
+        double g = coef[k] * e[v] * pw[v][e[v] - 1];
+        for (int u = 0; u < nv; ++u)
+          if (u != v) g *= pw[u][e[u]];
+        grad[v] += g;
+      }
+    }
+  }
+  return F;
+}
+
+// dT of a pose increment: 3-D [dt (3), dR row-major (9)]; 2-D [dt, r11, r21].
+static void pack_dT(int D, const double *dR, const double *dt, double *dT) {
+  if (D == 3) {
+    for (int a = 0; a < 3; ++a) dT[a] = dt[a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) dT[3 + 3 * a + b] = dR[a + 3 * b];
+  } else {
+    dT[0] = dt[0];
+    dT[1] = dt[1];
+    dT[2] = dR[0];
+    dT[3] = dR[1];
+  }
+}
+
+static void rot_exp3(const double *w, double *R) {
+  const double th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double A = th < 1e-12 ? 1.0 : std::sin(th) / th;
+  const double B = th < 1e-12 ? 0.5 : (1.0 - std::cos(th)) / (th * th);
+  const double K[9] = {0, w[2], -w[1], -w[2], 0, w[0], w[1], -w[0], 0};
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double k2 = 0;
+      for (int b = 0; b < 3; ++b) k2 += K[a + 3 * b] * K[b + 3 * c];
+      R[a + 3 * c] = (a == c ? 1.0 : 0.0) + A * K[a + 3 * c] + B * k2;
+    }
+}
+
+static void matmul(int D, const double *A, const double *B, double *C) {
+  for (int a = 0; a < D; ++a)
+    for (int c = 0; c < D; ++c) {
+      double s = 0;
+      for (int b = 0; b < D; ++b) s += A[a + D * b] * B[b + D * c];
+      C[a + D * c] = s;
+    }
+}
+
+// Pose increment from parameters p (3-D: w, dt; 2-D: theta, dt) applied on
+// the left of dR0:  dR = exp([w]x) dR0, dt = dt0 + p_t.
+static void apply(int D, const double *dR0, const double *dt0, const double *p,
+                  double *dR, double *dt) {
+  if (D == 3) {
+    double E[9];
+    rot_exp3(p, E);
+    matmul(3, E, dR0, dR);
+    for (int a = 0; a < 3; ++a) dt[a] = dt0[a] + p[3 + a];
+  } else {
+    const double c = std::cos(p[0]), s = std::sin(p[0]);
+    const double E[4] = {c, s, -s, c};
+    matmul(2, E, dR0, dR);
+    dt[0] = dt0[0] + p[1];
+    dt[1] = dt0[1] + p[2];
+  }
+}
+
+// Objective on the constraint set (dR a rotation) as a function of p, with
+// its gradient in p from the polynomial gradient by the chain rule.
+static double obj_p(int D, const double *coef, const double *dR0, const double *dt0,
+                    const double *p, double *gp) {
+  double dR[9], dt[3], dT[12], g[12];
+  apply(D, dR0, dt0, p, dR, dt);
+  pack_dT(D, dR, dt, dT);
+  const double F = poly_value(D, coef, dT, gp ? g : nullptr);
+  if (!gp) return F;
+  if (D == 3) {
+    // d dR / d w_a at p (left perturbation): [e_a]x dR
+    for (int a = 0; a < 3; ++a) {
+      double ea[3] = {0, 0, 0};
+      ea[a] = 1.0;
+      const double K[9] = {0, ea[2], -ea[1], -ea[2], 0, ea[0], ea[1], -ea[0], 0};
+      double KR[9];
+      matmul(3, K, dR, KR);
+      double s = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s += g[3 + 3 * r + c] * KR[r + 3 * c];
+      gp[a] = s;
+    }
+    for (int a = 0; a < 3; ++a) gp[3 + a] = g[a];
+  } else {
+    // dR = [[r11, -r21], [r21, r11]]; d/dtheta of Rot(theta) dR0 = J dR
+    gp[0] = g[2] * (-dR[1]) + g[3] * dR[0];
+    gp[1] = g[0];
+    gp[2] = g[1];
+  }
+  return F;
+}
+
+// Damped Newton on p with the Hessian from central differences of the
+// analytic gradient; re-centred on the accepted iterate each step.
+static int minimise(int D, const double *coef, double *dR, double *dt, double *Fout) {
+  const int np = D == 3 ? 6 : 3;
+  for (int a = 0; a < D * D; ++a) dR[a] = (a % (D + 1) == 0) ? 1.0 : 0.0;
+  for (int a = 0; a < D; ++a) dt[a] = 0.0;
+  double zero[6] = {0, 0, 0, 0, 0, 0};
+  double F = obj_p(D, coef, dR, dt, zero, nullptr);
+  if (!std::isfinite(F)) return -1;
+  double lambda = 1e-3;
+  for (int it = 0; it < 100; ++it) {
+    double g[6], H[36];
+    obj_p(D, coef, dR, dt, zero, g);
+    double gn = 0;
+    for (int a = 0; a < np; ++a) gn += g[a] * g[a];
+    const double h = 1e-6;
+    for (int a = 0; a < np; ++a) {
+      double p[6] = {0, 0, 0, 0, 0, 0}, gpl[6], gmi[6];
+      p[a] = h;
+      obj_p(D, coef, dR, dt, p, gpl);
+      p[a] = -h;
+      obj_p(D, coef, dR, dt, p, gmi);
+      for (int b = 0; b < np; ++b) H[a * np + b] = (gpl[b] - gmi[b]) / (2 * h);
+    }
+    for (int a = 0; a < np; ++a)
+      for (int b = a + 1; b < np; ++b) {
+        const double m = 0.5 * (H[a * np + b] + H[b * np + a]);
+        H[a * np + b] = H[b * np + a] = m;
+      }
+    bool accepted = false;
+    for (int tries = 0; tries < 30 && !accepted; ++tries) {
+      // solve (H + lambda (|diag| + 1e-12) I) s = -g by Gaussian elimination
+      double A[6][7];
+      for (int a = 0; a < np; ++a) {
+        for (int b = 0; b < np; ++b) A[a][b] = H[a * np + b];
+        A[a][a] += lambda * (std::fabs(H[a * np + a]) + 1e-12);
+        A[a][np] = -g[a];
+      }
+      bool ok = true;
+      for (int c = 0; c < np && ok; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < np; ++r)
+          if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+        if (std::fabs(A[piv][c]) < 1e-300) {
+          ok = false;
+          break;
+        }
+        for (int k = 0; k <= np; ++k) std::swap(A[c][k], A[piv][k]);
+        for (int r = 0; r < np; ++r) {
+          if (r == c) continue;
+          const double f = A[r][c] / A[c][c];
+          for (int k = c; k <= np; ++k) A[r][k] -= f * A[c][k];
+        }
+      }
+      if (ok) {
+        double s[6];
+        for (int a = 0; a < np; ++a) s[a] = A[a][np] / A[a][a];
+        const double Fn = obj_p(D, coef, dR, dt, s, nullptr);
+        if (std::isfinite(Fn) && Fn <= F) {
+          double nR[9], nt[3];
+          apply(D, dR, dt, s, nR, nt);
+          memcpy(dR, nR, sizeof(double) * D * D);
+          memcpy(dt, nt, sizeof(double) * D);
+          double sn = 0;
+          for (int a = 0; a < np; ++a) sn += s[a] * s[a];
+          const double dF = F - Fn;
+          F = Fn;
+          lambda = std::max(lambda * 0.3, 1e-12);
+          accepted = true;
+          if (sn < 1e-28 || dF <= 1e-15 * std::fabs(F)) {
+            *Fout = F;
+            return 1;
+          }
+          break;
+        }
+      }
+      lambda *= 10.0;
+    }
+    if (!accepted) break;  // no decrease possible: at a (local) minimum
+    (void)gn;
+  }
+  *Fout = F;
+  return 1;
+}
+
+// rotm2eul, 'ZYX' (MATLAB Robotics Toolbox convention): R = Rz(a) Ry(b) Rx(c)
+static void rotm2eul_zyx(const double *R, double *eul) {
+  const double r11 = R[0], r21 = R[1], r31 = R[2], r32 = R[5], r33 = R[8];
+  const double r12 = R[3], r22 = R[4];
+  const double sy = std::sqrt(r11 * r11 + r21 * r21);
+  if (sy > 1e-12) {
+    eul[0] = std::atan2(r21, r11);
+    eul[1] = std::atan2(-r31, sy);
+    eul[2] = std::atan2(r32, r33);
+  } else {
+    eul[0] = std::atan2(-r12, r22);
+    eul[1] = std::atan2(-r31, sy);
+    eul[2] = 0.0;
+  }
+}
+
+}  // namespace dugma
+
+using namespace dugma;
+
+extern "C" double dugma_poly_eval(int D, const double *coef, const double *dT, double *grad) {
+  if ((D != 2 && D != 3) || !coef || !dT) return NAN;
+  return poly_value(D, coef, dT, grad);
+}
+
+extern "C" int dugma_poly_minimize(int D, const double *coef, double *dT, double *F) {
+  if ((D != 2 && D != 3) || !coef || !dT) return fail(DUGMA_ERR_ARG, "dugma_poly_minimize: bad argument");
+  double dR[9], dt[3], Fv = 0;
+  const int ok = minimise(D, coef, dR, dt, &Fv);
+  pack_dT(D, dR, dt, dT);
+  if (F) *F = Fv;
+  return ok > 0 ? DUGMA_OK : fail(DUGMA_ERR_STATE, "dugma_poly_minimize: objective is not finite");
+}
+
+extern "C" int dugma_register(const double *X, const double *X_C, const double *Y,
+                              const double *Y_C, int N, int M, int D,
+                              const double *R0, const double *t0,
+                              const dugma_register_opts *o, double *R, double *t,
+                              int *iterations, double *sig_out) {
+  if (!X || !X_C || !Y || !Y_C || !R0 || !t0 || !o || !R || !t)
+    return fail(DUGMA_ERR_ARG, "dugma_register: null argument");
+  if (D != 2 && D != 3) return fail(DUGMA_ERR_ARG, "dugma_register: D must be 2 or 3");
+  // normalize.m:3-26: centre each cloud, divide both by the larger RMS radius
+  std::vector<double> x(X, X + (size_t)N * D), y(Y, Y + (size_t)M * D);
+  double xd[3] = {0, 0, 0}, yd[3] = {0, 0, 0};
+  for (int i = 0; i < N; ++i)
+    for (int a = 0; a < D; ++a) xd[a] += x[(size_t)i * D + a];
+  for (int j = 0; j < M; ++j)
+    for (int a = 0; a < D; ++a) yd[a] += y[(size_t)j * D + a];
+  for (int a = 0; a < D; ++a) {
+    xd[a] /= N;
+    yd[a] /= M;
+  }
+  double xs = 0, ys = 0;
+  for (int i = 0; i < N; ++i)
+    for (int a = 0; a < D; ++a) {
+      x[(size_t)i * D + a] -= xd[a];
+      xs += x[(size_t)i * D + a] * x[(size_t)i * D + a];
+    }
+  for (int j = 0; j < M; ++j)
+    for (int a = 0; a < D; ++a) {
+      y[(size_t)j * D + a] -= yd[a];
+      ys += y[(size_t)j * D + a] * y[(size_t)j * D + a];
+    }
+  xs = std::sqrt(xs / N);
+  ys = std::sqrt(ys / M);
+  const double scale = xs > ys ? xs : ys;
+  for (double &v : x) v /= scale;
+  for (double &v : y) v /= scale;
+  // DUGMA.m:44-47 (covariances are passed unscaled, as the reference does)
+  dugma_ctx *ctx = nullptr;
+  double ok = 0;
+  int rc = dugma_init(x.data(), X_C, y.data(), Y_C, N, M, D, &ctx, &ok);
+  if (rc) return rc;
+  double Rc[9], tc[3];
+  memcpy(Rc, R0, sizeof(double) * D * D);
+  memcpy(tc, t0, sizeof(double) * D);
+  double sig = 0;
+  rc = dugma_update_sig(ctx, Rc, tc, N, M, D, &sig);
+  double angle[3] = {10000, 10000, 10000}, angle0[3];
+  double tprev[3];
+  memcpy(tprev, tc, sizeof(tc));
+  bool con_angle = true, failed = false;
+  int iteration = 1;
+  std::vector<double> coef(D == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2);
+  auto tdiff2 = [&]() {
+    double s = 0;
+    for (int a = 0; a < D; ++a) s += (tprev[a] - tc[a]) * (tprev[a] - tc[a]);
+    return s;
+  };
+  // DUGMA.m:57 loop condition
+  while (!rc && iteration <= o->max_iter && (con_angle || tdiff2() > o->acc_trans) && sig > o->acc_sig) {
+    rc = dugma_compi(ctx, Rc, tc, N, M, D, sig, coef.data());
+    if (rc) break;
+    double dR[9], dt[3], F = 0;
+    if (minimise(D, coef.data(), dR, dt, &F) < 0) {
+      failed = true;  // DUGMA.m:84-87: exitflag < 0
+      break;
+    }
+    // DUGMA.m:67-78: R <- dR R, t <- dt + t
+    double Rn[9];
+    matmul(D, dR, Rc, Rn);
+    memcpy(Rc, Rn, sizeof(double) * D * D);
+    memcpy(tprev, tc, sizeof(tc));
+    for (int a = 0; a < D; ++a) tc[a] += dt[a];
+    memcpy(angle0, angle, sizeof(angle));
+    if (D == 3) {
+      rotm2eul_zyx(Rc, angle);
+      con_angle = std::fabs(angle0[0] - angle[0]) > o->acc_rot ||
+                  std::fabs(angle0[1] - angle[1]) > o->acc_rot ||
+                  std::fabs(angle0[2] - angle[2]) > o->acc_rot;
+    } else {
+      // DUGMA.m:71: asind(R(2)) in degrees
+      angle[0] = std::asin(std::max(-1.0, std::min(1.0, Rc[1]))) * 57.29577951308232;
+      con_angle = std::fabs(angle0[0] - angle[0]) > o->acc_rot;
+    }
+    rc = dugma_update_sig(ctx, Rc, tc, N, M, D, &sig);
+    if (o->verbose)
+      std::printf("Iteration=%d Sigma=%g F=%g\n", iteration, sig, F);
+    ++iteration;
+  }
+  double okf = 0;
+  dugma_fr_m(ctx, &okf);
+  if (rc) return rc;
+  // DUGMA.m:116-123
+  if (failed) {
+    for (int a = 0; a < D * D; ++a) R[a] = (a % (D + 1) == 0) ? 1.0 : 0.0;
+    for (int a = 0; a < D; ++a) t[a] = 0.0;
+  } else {
+    memcpy(R, Rc, sizeof(double) * D * D);
+    for (int a = 0; a < D; ++a) {
+      double ry = 0;
+      for (int b = 0; b < D; ++b) ry += Rc[a + D * b] * yd[b];
+      t[a] = xd[a] + tc[a] * scale - ry;
+    }
+  }
+  if (iterations) *iterations = iteration - 1;
+  if (sig_out) *sig_out = sig;
+  return DUGMA_OK;
+}

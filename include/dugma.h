@@ -70,6 +70,40 @@ int dugma_compi(dugma_ctx *ctx, const double *R, const double *t, double N,
  * device memory.  Unlike the reference it does not reset the device. */
 int dugma_fr_m(dugma_ctx *ctx, double *ok);
 
+/* ---- the surrogate polynomial (energy3D.m / energy2D.m) ----------------- */
+
+/* F(dT) = coef . B(dT) over the 637 (3-D) or 45 (2-D) monomials of
+ * energy3D.m:15 / energy2D.m:7, dT = [t1 t2 t3 r11 .. r33] (dR row-major) or
+ * [t1 t2 r11 r21].  grad (12 or 4 doubles, may be NULL) receives dF/ddT:
+ * the objective gradient fmincon would otherwise take by finite differences.
+ * Returns NaN for bad arguments. */
+double dugma_poly_eval(int D, const double *coef, const double *dT, double *grad);
+
+/* Minimise F over dR in SO(D) (fcontr3D.m / fcontr2D.m constraints, kept by
+ * construction) and dt, from dT = identity: damped Newton on a left
+ * rotation perturbation and dt.  Replaces fmincon (DUGMA.m:65,73); writes the
+ * minimiser to dT and the value to *F. */
+int dugma_poly_minimize(int D, const double *coef, double *dT, double *F);
+
+/* ---- the registration loop (DUGMA.m) ------------------------------------ */
+
+typedef struct {
+  int max_iter;        /* Max_Iteration */
+  double acc_rot;      /* accuracy_rotation (radians of ZYX Euler angles; degrees in 2-D) */
+  double acc_trans;    /* accuracy_translation (squared step) */
+  double acc_sig;      /* accuracy_sig */
+  int verbose;         /* print one line per iteration like DUGMA.m:109 */
+} dugma_register_opts;
+
+/* [R, t] = DUGMA(X, Y, X_C, Y_C, R0, t0, ...)   (DUGMA.m:19-137): normalise
+ * the points (normalize.m), init, update_sig, then compi -> minimise the
+ * surrogate -> R <- dR R, t <- t + dt -> Euler-angle convergence ->
+ * update_sig, and denormalise.  Layouts as dugma_init; R0, R column-major. */
+int dugma_register(const double *X, const double *X_C, const double *Y,
+                   const double *Y_C, int N, int M, int D, const double *R0,
+                   const double *t0, const dugma_register_opts *opts,
+                   double *R, double *t, int *iterations, double *sig);
+
 /* ---- convolved-uncertainty energy (no reference counterpart) ----------- */
 
 /* Evaluation modes for the energy entry points. */
