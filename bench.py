@@ -45,6 +45,23 @@ class Hip:
         self.lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
         self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
 
+    def copy_calibration(self, mbytes):
+        """One device-to-device copy of mbytes MiB (a blit kernel of known
+        traffic, for scaling memory counters)."""
+        n = int(mbytes) << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.lib.hipFree.argtypes = [ctypes.c_void_p]
+        assert self.lib.hipMalloc(ctypes.byref(a), n) == 0 and self.lib.hipMalloc(ctypes.byref(b), n) == 0
+        self.lib.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        self.lib.hipMemset(a, 1, n)
+        self.sync()
+        assert self.lib.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+        self.sync()
+        self.lib.hipFree(a)
+        self.lib.hipFree(b)
+
     def event(self):
         e = ctypes.c_void_p()
         assert self.lib.hipEventCreate(ctypes.byref(e)) == 0
@@ -200,6 +217,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the finite-difference measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched registrations")
     ap.add_argument("--trials", type=int, default=4096, help="batched trials")
+    ap.add_argument("--only", choices=["fused", "plain", "large", "batch"], default=None,
+                    help="run one measurement (for profiler runs)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first do one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -210,6 +231,22 @@ def main():
     import dugma_amd as dg
     from dugma_amd import workloads as W
     hip = Hip()
+    if args.copy_mb:
+        hip.copy_calibration(args.copy_mb)
+    if args.only:
+        only = args.only
+        args.no_plain = only != "plain"
+        args.no_large = only != "large"
+        args.no_batch = only != "batch"
+        args.no_cpu = True
+        if only in ("large", "batch"):
+            if only == "large":
+                d3 = W.config3()
+                big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
+                print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"]}))
+            else:
+                print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(16, args.trials))}))
+            return
 
     log("generating 6000 x 6000 workload")
     d2 = W.config2(n=6000, m=6000)
@@ -237,6 +274,9 @@ def main():
                      "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED,
                      "launch": fused["plan"]},
     }
+    if args.only == "fused":
+        print(json.dumps(out), flush=True)
+        return
     if not args.no_plain:
         plain = bench_plain(hip, dg, d2, max(3, args.steps // 10))
         pv = plain["pairs"] / (plain["ms_per_gradient"] / 1e3) / 1e9
