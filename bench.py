@@ -98,11 +98,15 @@ def start_pose(d, deg=2.0, dt=0.003, seed=0):
     return W.random_rotation(rng, deg) @ d["R_G"], np.ravel(d["t_G"]) + rng.uniform(-dt, dt, size=3)
 
 
+TUNE = True
+
+
 def bench_fused(hip, dg, d, steps, warmup, label):
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
         R, t = start_pose(d)
-        plan = h.autotune()
+        plan = h.autotune() if TUNE else (int(os.environ.get("DUGMA_LANE_POINTS", 0)),
+                                          int(os.environ.get("DUGMA_TARGET_WAVES", 0)))
         for _ in range(warmup):
             dg.energy_grad(h, R, t)
         ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps)
@@ -125,7 +129,8 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
         R, t = start_pose(d)
-        h.autotune()
+        if TUNE:
+            h.autotune()
         dg.energy_grad_fd(h, R, t, step_h)
         ms = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, step_h), steps)
         nT, nS, _ = h.energy_info()
@@ -219,6 +224,8 @@ def main():
     ap.add_argument("--trials", type=int, default=4096, help="batched trials")
     ap.add_argument("--only", choices=["fused", "plain", "large", "batch"], default=None,
                     help="run one measurement (for profiler runs)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the launch shape from DUGMA_LANE_POINTS / DUGMA_TARGET_WAVES")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first do one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -231,6 +238,8 @@ def main():
     import dugma_amd as dg
     from dugma_amd import workloads as W
     hip = Hip()
+    global TUNE
+    TUNE = not args.no_tune
     if args.copy_mb:
         hip.copy_calibration(args.copy_mb)
     if args.only:
