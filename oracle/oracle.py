@@ -204,3 +204,69 @@ def pair_weight(n, binv, gx, m, ainv, gy, R, t, sig):
     return lib().dor_pair_weight(D, _p(n, _f), _p(_c32(binv), _f), float(gx), _p(m, _f),
                                  _p(_c32(ainv), _f), float(gy), _p(_colmajor(R), _d),
                                  _p(_c64(t), _d), float(sig))
+
+
+def dugma_loop(X, Y, X_C, Y_C, R0, t0, max_iter, acc_rot=1e-4, acc_trans=1e-8, acc_sig=1e-9):
+    """DUGMA.m:19-137 restated on the CPU: normalize.m, the oracle's
+    update_sig / compi, and scipy's BFGS over a left rotation perturbation
+    and dt in place of fmincon (DUGMA.m:65,73).  Returns (R, t, iterations)."""
+    from scipy.optimize import minimize
+    D = X.shape[0]
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    xd, yd = X.mean(1), Y.mean(1)
+    x, y = X - xd[:, None], Y - yd[:, None]
+    sc = max(np.sqrt((x ** 2).sum() / x.shape[1]), np.sqrt((y ** 2).sum() / y.shape[1]))
+    Xf, Yf = (x / sc).T.astype(np.float32), (y / sc).T.astype(np.float32)
+
+    def blocks_to_pts(C):
+        n = C.shape[1] // D
+        return np.transpose(C.reshape(D, n, D), (1, 0, 2))
+
+    XCi, gX, _ = prep_inverse(blocks_to_pts(X_C), D)
+    YCi, gY, _ = prep_inverse(blocks_to_pts(Y_C), D)
+    R, t = np.array(R0, np.float64), np.array(t0, np.float64).ravel()
+    sig = update_sig(Xf, Yf, R, t)
+
+    def rot(p):
+        if D == 2:
+            c, s = np.cos(p[0]), np.sin(p[0])
+            return np.array([[c, -s], [s, c]])
+        th = np.linalg.norm(p[:3])
+        K = np.array([[0, -p[2], p[1]], [p[2], 0, -p[0]], [-p[1], p[0], 0]])
+        if th < 1e-12:
+            return np.eye(3) + K
+        return np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
+
+    def dT_of(p):
+        dR = rot(p)
+        if D == 2:
+            return np.array([p[1], p[2], dR[0, 0], dR[1, 0]])
+        return np.concatenate([p[3:], dR.ravel()])
+
+    angle = np.full(3, 10000.0)
+    tprev = t.copy()
+    it = 1
+    con = True
+    while it <= max_iter and (con or ((tprev - t) ** 2).sum() > acc_trans) and sig > acc_sig:
+        coef, _ = compi(Xf, XCi, gX, Yf, YCi, gY, R, t, sig)
+        np_ = 6 if D == 3 else 3
+        res = minimize(lambda p: poly_eval(coef, dT_of(p)), np.zeros(np_), method="BFGS",
+                       options={"gtol": 1e-12 * max(1.0, abs(poly_eval(coef, dT_of(np.zeros(np_))))),
+                                "maxiter": 500})
+        dR = rot(res.x)
+        dt = res.x[3:] if D == 3 else res.x[1:]
+        R = dR @ R
+        tprev = t.copy()
+        t = t + dt
+        a0 = angle.copy()
+        if D == 3:
+            sy = np.hypot(R[0, 0], R[1, 0])
+            angle = np.array([np.arctan2(R[1, 0], R[0, 0]), np.arctan2(-R[2, 0], sy), np.arctan2(R[2, 1], R[2, 2])])
+            con = bool(np.any(np.abs(a0 - angle) > acc_rot))
+        else:
+            angle = np.array([np.degrees(np.arcsin(np.clip(R[1, 0], -1, 1))), 0, 0])
+            con = bool(abs(a0[0] - angle[0]) > acc_rot)
+        sig = update_sig(Xf, Yf, R, t)
+        it += 1
+    return R, xd + t * sc - R @ yd, it - 1
