@@ -32,6 +32,11 @@ FLOP_3D_FULL_FUSED = 107
 FLOP_3D_FULL_FWD = 59
 FLOP_3D_ISO_FUSED = 36
 FLOP_3D_ISO_FWD = 17
+# HBM bytes per k_pairs launch on the 6,000 x 6,000 full-covariance workload,
+# keyed by launch shape (lane_points, target_waves): rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counter runs, each scaled by what the counter reads for a 256 MiB
+# copy (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).  Other shapes: null.
+MEASURED_TRAFFIC_6K_FULL = {(1, 32768): 6755000}
 
 
 class Hip:
@@ -283,6 +288,8 @@ def main():
                      "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED,
                      "launch": fused["plan"]},
     }
+    lp = fused["plan"]
+    out["roofline"]["traffic"] = MEASURED_TRAFFIC_6K_FULL.get((lp["lane_points"], lp["target_waves"]))
     if args.only == "fused":
         print(json.dumps(out), flush=True)
         return
