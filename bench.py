@@ -144,6 +144,11 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         dg.fr_m(h)
 
 
+# Batched iteration: DUGMA's sigma update (the kernel variance tracks the mean
+# nearest distance) on top of an annealed extra variance.
+BATCH_MODE = dict(anneal=True, sigma_update=True)
+
+
 def bench_batched(dg, W, ntrials, baseline_trials, max_iter=80):
     """4,096 independent registrations: one persistent pair launch per
     iteration over a (trial, tile) queue, iterations replayed from a hipGraph;
@@ -153,43 +158,47 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=80):
     log("generated %d trials in %.1f s" % (ntrials, time.time() - t0))
     b = dg.Batch(trials)
     try:
+        b.register(max_iter=2, **BATCH_MODE)  # warm-up: code objects, graph path
+        ta = time.perf_counter()
+        shape = b.autotune(sigma_update=BATCH_MODE["sigma_update"])
+        tune_s = time.perf_counter() - ta
         info = b.info()
-        b.register(max_iter=2)  # warm-up: code objects, graph path
+        b.register(max_iter=2, **BATCH_MODE)
         t1 = time.perf_counter()
-        out = b.register(max_iter=max_iter, use_graph=True, check_every=10)
+        out = b.register(max_iter=max_iter, use_graph=True, check_every=10, **BATCH_MODE)
         wall = time.perf_counter() - t1
+        # diagnostic: the same registrations started at the ground truth (how
+        # far the energy maximum itself sits from it)
+        R_G = np.array([tr["R_G"] for tr in trials])
+        t_G = np.array([tr["t_G"] for tr in trials])
+        outg = b.register(R_G, t_G, max_iter=max_iter, use_graph=True, **{**BATCH_MODE, "anneal": False})
     finally:
         b.free()
     errs = np.array([W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
-    # diagnostic: the same registrations started at the ground truth (how far
-    # the energy maximum itself sits from it on these near-spherical shapes)
-    b = dg.Batch(trials)
-    try:
-        R_G = np.array([tr["R_G"] for tr in trials])
-        t_G = np.array([tr["t_G"] for tr in trials])
-        outg = b.register(R_G, t_G, max_iter=max_iter, anneal=False, use_graph=True)
-    finally:
-        b.free()
     errs_g = np.array([W.rotation_error_deg(outg["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
     sub = trials[:baseline_trials]
     b1 = dg.Batch(sub)
     try:
-        b1.register(max_iter=2, one_trial_per_launch=True)
+        b1.set_shape(shape["waves"], shape["chunks"], shape["grid_waves"])  # same sums as the batched run
+        b1.register(max_iter=2, one_trial_per_launch=True, **BATCH_MODE)
         t2 = time.perf_counter()
-        out1 = b1.register(max_iter=max_iter, one_trial_per_launch=True)
+        out1 = b1.register(max_iter=max_iter, one_trial_per_launch=True, **BATCH_MODE)
         wall1 = time.perf_counter() - t2
     finally:
         b1.free()
     same = bool(np.array_equal(out1["R"], out["R"][:baseline_trials]))
     rps = ntrials / wall
     rps1 = baseline_trials / wall1
-    log("batched: %d trials in %.3f s (%.1f reg/s), one-trial-per-launch %.1f reg/s, success %.3f"
-        % (ntrials, wall, rps, rps1, float((errs < 2.0).mean())))
+    log("batched: %d trials in %.3f s (%.1f reg/s), one-trial-per-launch %.1f reg/s, success %.3f, shape %s"
+        % (ntrials, wall, rps, rps1, float((errs < 2.0).mean()), shape))
     return {"workload": "4,096 trials, N and M log-uniform in [500, 8000], rotation U(0,90) deg, "
                         "noise U(0.001,0.02), outliers U(0,0.3), isotropic 8-NN sigma",
             "value": round(rps, 2), "unit": "registrations/s", "wall_s": round(wall, 4),
+            "iteration": "majorise-minimise pose step; DUGMA sigma update%s" %
+                         (" + annealed extra variance" if BATCH_MODE["anneal"] else ""),
             "iterations_mean": float(out["iters"].mean()), "max_iter": max_iter,
             "pairs_per_iteration": info["pairs_per_iteration"], "work_items": info["items"],
+            "autotune": {**shape, "seconds": round(tune_s, 3)},
             "success_rate": round(float((errs < 2.0).mean()), 4),
             "rotation_error_median_deg": round(float(np.median(errs)), 4),
             "success_rate_from_ground_truth": round(float((errs_g < 2.0).mean()), 4),

@@ -294,17 +294,34 @@ class Batch:
         _lib.check(_lib.load().dugma_batch_info(self._ptr, ctypes.byref(k), ctypes.byref(n), ctypes.byref(p)))
         return dict(ntrials=k.value, items=n.value, pairs_per_iteration=p.value)
 
-    def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
-                 tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
-                 check_every=10, grid=0):
+    def _poses(self, R0, t0):
         K = self.ntrials
         R0 = np.tile(np.eye(3), (K, 1, 1)) if R0 is None else np.asarray(R0, np.float64)
         t0 = np.zeros((K, 3)) if t0 is None else np.asarray(t0, np.float64)
-        R0c = np.ascontiguousarray(np.transpose(R0, (0, 2, 1)))  # column-major per trial
-        t0c = np.ascontiguousarray(t0)
+        return np.ascontiguousarray(np.transpose(R0, (0, 2, 1))), np.ascontiguousarray(t0)  # column-major R
+
+    def autotune(self, R0=None, t0=None, sigma_update=True):
+        """dugma_batch_autotune: returns the chosen launch shape and its time
+        per iteration."""
+        R0c, t0c = self._poses(R0, t0)
+        w, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        ch = (ctypes.c_int * 4)()
+        _lib.check(_lib.load().dugma_batch_autotune(self._ptr, _dp(R0c), _dp(t0c), int(bool(sigma_update)),
+                                                    ctypes.byref(w), ch, ctypes.byref(g), ctypes.byref(ms)))
+        return dict(waves=w.value, chunks=list(ch), grid_waves=g.value, ms_per_iteration=ms.value)
+
+    def set_shape(self, waves, chunks, grid_waves):
+        ch = (ctypes.c_int * 4)(*[int(c) for c in chunks])
+        _lib.check(_lib.load().dugma_batch_set_shape(self._ptr, int(waves), ch, int(grid_waves)))
+
+    def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
+                 tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
+                 check_every=10, grid=0, sigma_update=True):
+        K = self.ntrials
+        R0c, t0c = self._poses(R0, t0)
         o = _lib.BatchOpts(int(max_iter), int(bool(anneal)), float(anneal_decay), float(tol_rot),
                            float(tol_trans), int(bool(use_graph)), int(bool(one_trial_per_launch)),
-                           int(check_every), int(grid))
+                           int(check_every), int(grid), int(bool(sigma_update)))
         R = np.zeros((K, 9))
         t = np.zeros((K, 3))
         iters = np.zeros(K, np.int32)

@@ -22,7 +22,7 @@ class BatchOpts(ctypes.Structure):
                 ("anneal_decay", ctypes.c_double), ("tol_rot", ctypes.c_double),
                 ("tol_trans", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("grid", ctypes.c_int)]
+                ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int)]
 
 
 class RegisterOpts(ctypes.Structure):
@@ -55,6 +55,8 @@ SIGNATURES = {
     "dugma_batch_register": (_ci, [_vp, _d, _d, ctypes.POINTER(BatchOpts), _d, _d, _i, _i, _d]),
     "dugma_batch_info": (_ci, [_vp, _i, _i, ctypes.POINTER(ctypes.c_longlong)]),
     "dugma_batch_free": (_ci, [_vp]),
+    "dugma_batch_autotune": (_ci, [_vp, _d, _d, _ci, _i, _i, _i, _d]),
+    "dugma_batch_set_shape": (_ci, [_vp, _ci, _i, _ci]),
     "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
     "dugma_read_handle": (_ci, [_vp, _ci, _f]),
     "dugma_ctx_dims": (_ci, [_vp, _i, _i, _i]),

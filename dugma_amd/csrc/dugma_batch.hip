@@ -2,21 +2,41 @@
 // sizes on one GPU.
 //
 // Each trial maximises the convolved-uncertainty energy
-//   E(R,t) = sum_ij N(y_i - R x_j - t; (s_i + s_j + a) I)
-// (isotropic covariances s, annealing variance a -> 0) by
-// majorise-minimise steps: with e_ij at the current pose, exp(-u) >= e0 (1 -
-// (u - u0)) makes E grow whenever sum_ij (e_ij / s_ij) |y_i - R x_j - t|^2
-// shrinks, and that weighted least-squares problem has the closed-form
-// (Horn) solution.  One iteration for all trials is three kernels:
-//   k_batch_transform  x'_j = R_k x_j + t_k - c_k (FP64 -> FP32 hi/lo), s_j + a_k
+//   E(R,t) = sum_ij N(y_i - R x_j - t; (g (s_i + s_j) + a) I)
+// (isotropic covariances s) by majorise-minimise steps: with e_ij at the
+// current pose, exp(-u) >= e0 (1 - (u - u0)) makes E grow whenever
+// sum_ij (e_ij / s_ij) |y_i - R x_j - t|^2 shrinks, and that weighted
+// least-squares problem has the closed-form (Horn) solution.
+//
+// The variance scale g follows DUGMA's sigma update: DUGMA.m:47,105 measure
+// sig = mean_i min_j |y_i - R x_j - t|^2 / D (update_sig.cu:238-260, 384-385)
+// and compi divides every inverse covariance by sig (compi.cu:642-661), i.e.
+// the kernels use sig * Sigma.  Here the point variances act as a unitless
+// shape, g = sig / sbar with sbar the trial's mean target plus mean source
+// variance, so the kernel width tracks the current mean nearest distance.
+// The nearest distances come out of the same all-pairs pass as the
+// statistics (one fminf per pair, one atomicMin per target and work item);
+// a first pass before the iterations measures the starting sig, as DUGMA.m
+// calls update_sig before its loop.  Optionally an extra variance a -> 0 is
+// annealed on top.
+//
+// One iteration for all trials is three kernels:
+//   k_batch_transform  x'_j = R_k x_j + t_k - c_k (FP64 -> FP32 hi/lo), g s_j + a
 //   k_batch_pairs      persistent; workgroups pop (trial, target tile, source
 //                      chunk) items from a queue ordered longest first and
-//                      write FP64 sufficient statistics per item
-//   k_batch_update     per trial: fixed-order sum of its items, Horn's
-//                      quaternion solution (4x4 Jacobi, FP64), pose update,
-//                      annealing, convergence; re-arms the queue
+//                      write FP64 sufficient statistics per item, plus the
+//                      per-target nearest squared distance
+//   k_batch_update     one workgroup per trial: its nearest-distance sum,
+//                      a fixed-order sum of its items, Horn's quaternion
+//                      solution (4x4 Jacobi, FP64), pose, sigma, annealing,
+//                      convergence; re-arms the queue
 // captured once into a hipGraph and replayed.  The same kernels run one
 // trial per launch for the baseline.
+//
+// Launch shape: waves per workgroup (target tile = 64 x waves), the number
+// of source chunks per target tile for each size class of trials (by source
+// count), and the resident waves of the persistent kernel are chosen by
+// dugma_batch_autotune, which times whole iterations on the batch's own data.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -28,9 +48,14 @@
 
 namespace dugma {
 
-constexpr int kBBlock = 256;
 constexpr int kBSub = 64;
-constexpr int kStat = 20;  // per-item statistics (doubles)
+constexpr int kStat = 20;         // per-item statistics (doubles)
+constexpr int kUpd = 256;         // threads of the update kernel (one workgroup per trial)
+constexpr int kClasses = 4;       // size classes by source count: < 1000, < 2000, < 4000, more
+constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
+constexpr double kGMin = 1e-6;    // smallest variance scale g
+
+static inline int size_class(int ns) { return ns < 1000 ? 0 : ns < 2000 ? 1 : ns < 4000 ? 2 : 3; }
 
 struct BItem {
   int trial, t0, tn, s0, sn, pad0, pad1, pad2;
@@ -44,9 +69,13 @@ struct BTrial {
   double a_stop;
   double E;     // energy at the last iteration (without the (2 pi)^-3/2 factor)
   double dang, dtr;  // last rotation / translation step
+  double g;     // variance scale: sig / sbar with the sigma update, else 1
+  double sig;   // mean squared nearest distance / 3 at the last pass
+  double sbar;  // mean target variance + mean source variance
   int iters;
   int status;   // 0 running, 1 converged, 2 iteration limit, 3 degenerate
-  int pad[2];
+  int sig_ready;
+  int pad;
 };
 
 struct BParams {
@@ -54,14 +83,14 @@ struct BParams {
   double tol_rot;    // radians
   double tol_trans;  // in the trial's own units
   int max_iter;
-  int pad;
+  int sigma;         // 1: DUGMA sigma update
 };
 
-__global__ __launch_bounds__(kBBlock) void k_batch_transform(
+__global__ __launch_bounds__(256) void k_batch_transform(
     const double *__restrict__ src, const int *__restrict__ src_trial, int s0,
     int s1, const BTrial *__restrict__ trials, const int *__restrict__ done,
     Rec3I *__restrict__ S) {
-  const int j = s0 + blockIdx.x * kBBlock + threadIdx.x;
+  const int j = s0 + blockIdx.x * 256 + threadIdx.x;
   if (j >= s1) return;
   const int k = src_trial[j];
   if (done[k]) return;
@@ -82,22 +111,31 @@ __global__ __launch_bounds__(kBBlock) void k_batch_transform(
   r.lx = (float)(v[0] - (double)r.hx);
   r.ly = (float)(v[1] - (double)r.hy);
   r.lz = (float)(v[2] - (double)r.hz);
-  r.s = (float)(x[3] + tr.a);
+  r.s = (float)fma(tr.g, x[3], tr.a);
   r.pad0 = 0.0f;
   S[j] = r;
 }
 
+__global__ __launch_bounds__(256) void k_batch_fill(unsigned *__restrict__ p, int n, unsigned v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Statistics of one item, per target i (lane): W_i = sum_j w_ij,
 // B_i = sum_j w_ij x'_j, E_i = sum_j e_ij, Q_i = sum_j w_ij |d_ij|^2 with
-// w = e / s.  Item totals: [E, W, sum W_i y_i (3), sum B_i (3),
-// sum y_i B_i^T (9, row a = y_a), sum Q_i].
-__global__ __launch_bounds__(kBBlock) void k_batch_pairs(
+// w = e / s_ij, s_ij = g s_i + (g s_j + a).  Item totals: [E, W, sum W_i y_i
+// (3), sum B_i (3), sum y_i B_i^T (9, row a = y_a), sum Q_i].  With SIG the
+// item's smallest |d_ij|^2 per target is folded into nn[i] (atomicMin on the
+// bits of a non-negative float, which order like the values).
+template <int W, bool SIG>
+__global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const Rec3I *__restrict__ T, const Rec3I *__restrict__ S,
     const BItem *__restrict__ items, const int *__restrict__ order, int nitems,
     unsigned *__restrict__ queue, const int *__restrict__ done,
+    const BTrial *__restrict__ trials, unsigned *__restrict__ nn,
     double *__restrict__ part) {
   __shared__ int s_p;
-  __shared__ double red[kBBlock / 64][kStat];
+  __shared__ double red[W][kStat];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
@@ -108,10 +146,13 @@ __global__ __launch_bounds__(kBBlock) void k_batch_pairs(
     const int idx = __builtin_amdgcn_readfirstlane(order[p]);
     const BItem it = items[idx];
     if (done[it.trial]) continue;
+    const float g = (float)trials[it.trial].g;
     const int ti = it.t0 + wave * 64 + lane;
     const bool valid = (wave * 64 + lane) < it.tn;
     const Rec3I pt = T[valid ? ti : it.t0];
+    const float gs = g * pt.s;
     double dE = 0.0, dW = 0.0, dB[3] = {0.0, 0.0, 0.0}, dQ = 0.0;
+    float dmin = __builtin_huge_valf();
     const int s_end = it.s0 + it.sn;
     for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
       float fE = 0.0f, fW = 0.0f, fB0 = 0.0f, fB1 = 0.0f, fB2 = 0.0f, fQ = 0.0f;
@@ -123,7 +164,8 @@ __global__ __launch_bounds__(kBBlock) void k_batch_pairs(
         const float dy = (pt.hy - q.hy) - q.ly;
         const float dz = (pt.hz - q.hz) - q.lz;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float rs = frsq(pt.s + q.s);
+        if (SIG) dmin = fminf(dmin, d2);
+        const float rs = frsq(gs + q.s);
         const float inv = rs * rs;
         const float e = fexp2((d2 * inv) * kNegHalfLog2e) * (rs * inv);
         const float w = e * inv;
@@ -141,6 +183,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch_pairs(
       dB[2] += fB2;
       dQ += fQ;
     }
+    if (SIG && valid) atomicMin(nn + ti, __float_as_uint(dmin));
     double r[kStat];
 #pragma unroll
     for (int a = 0; a < kStat; ++a) r[a] = 0.0;
@@ -168,7 +211,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch_pairs(
     if (threadIdx.x < kStat) {
       double s = red[0][threadIdx.x];
 #pragma unroll
-      for (int w = 1; w < kBBlock / 64; ++w) s += red[w][threadIdx.x];
+      for (int w = 1; w < W; ++w) s += red[w][threadIdx.x];
       part[(size_t)idx * kStat + threadIdx.x] = s;
     }
     __syncthreads();
@@ -217,14 +260,46 @@ __device__ void eig4_max(double A[4][4], double q[4]) {
   for (int a = 0; a < 4; ++a) q[a] = V[a][best];
 }
 
-__global__ __launch_bounds__(kBBlock) void k_batch_update(
-    const double *__restrict__ part, const int *__restrict__ titem, int k0,
-    int k1, BTrial *__restrict__ trials, int *__restrict__ done,
-    unsigned *__restrict__ queue, BParams bp) {
+// One workgroup per trial.  All threads fold the trial's nearest distances
+// (and reset them for the next pass); thread 0 does the rest.
+__global__ __launch_bounds__(kUpd) void k_batch_update(
+    const double *__restrict__ part, const int *__restrict__ titem,
+    const int *__restrict__ ttgt, int k0, int k1, BTrial *__restrict__ trials,
+    int *__restrict__ done, unsigned *__restrict__ queue, unsigned *__restrict__ nn,
+    BParams bp) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
-  const int k = k0 + blockIdx.x * kBBlock + threadIdx.x;
+  const int k = k0 + blockIdx.x;
   if (k >= k1 || done[k]) return;
+  __shared__ double s_nn[kUpd / 64];
+  if (bp.sigma) {
+    double snn = 0.0;
+    for (int i = ttgt[k] + threadIdx.x; i < ttgt[k + 1]; i += kUpd) {
+      snn += (double)__uint_as_float(nn[i]);
+      nn[i] = kInfBits;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) snn += __shfl_xor(snn, off);
+    if ((threadIdx.x & 63) == 0) s_nn[threadIdx.x >> 6] = snn;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   BTrial tr = trials[k];
+  double sig = -1.0;  // < 0: not measured
+  if (bp.sigma) {
+    double s = 0.0;
+    for (int w = 0; w < kUpd / 64; ++w) s += s_nn[w];
+    const int nt = ttgt[k + 1] - ttgt[k];
+    if (nt > 0 && isfinite(s)) sig = s / (3.0 * nt);
+    if (!tr.sig_ready) {  // the pass before the first iteration only measures sig
+      if (sig >= 0.0) {
+        tr.sig = sig;
+        tr.g = fmax(sig / tr.sbar, kGMin);
+      }
+      tr.sig_ready = 1;
+      trials[k] = tr;
+      return;
+    }
+  }
   double st[kStat];
   for (int a = 0; a < kStat; ++a) st[a] = 0.0;
   for (int i = titem[k]; i < titem[k + 1]; ++i)
@@ -281,6 +356,12 @@ __global__ __launch_bounds__(kBBlock) void k_batch_update(
   const double ca = fmin(1.0, fmax(-1.0, 0.5 * (dR[0][0] + dR[1][1] + dR[2][2] - 1.0)));
   tr.dang = acos(ca);
   tr.dtr = sqrt(dtr2);
+  // sig measured in this pass (at the pose before this step) sets the
+  // variance scale of the next pass
+  if (sig >= 0.0) {
+    tr.sig = sig;
+    tr.g = fmax(sig / tr.sbar, kGMin);
+  }
   const bool annealing = tr.a > 0.0;
   tr.a *= bp.beta;
   if (tr.a < tr.a_stop) tr.a = 0.0;
@@ -309,21 +390,28 @@ struct dugma_batch {
   BItem *ditems = nullptr;
   int *dorder = nullptr;     // queue position -> item, longest first
   int *dtitem = nullptr;     // CSR: items of trial k are [titem[k], titem[k+1])
+  int *dttgt = nullptr;      // CSR: targets of trial k are [ttgt[k], ttgt[k+1])
+  unsigned *dnn = nullptr;   // per target: smallest |d|^2 of the current pass (float bits)
   double *dpart = nullptr;   // per-item statistics
   BTrial *dtrials = nullptr;
   int *ddone = nullptr;
   unsigned *dqueue = nullptr;
   int *dident = nullptr;     // identity order for one-trial launches
-  std::vector<int> src_off, titem;
-  std::vector<double> a0, astop, cent;  // per trial
-  long long pairs = 0;                  // target x source pairs of one full iteration
-  int grid = 2048;
+  size_t items_cap = 0;      // capacity of ditems / dorder / dident / dpart
+  std::vector<int> tb, nt, sb, ns;  // per trial: target / source base and count
+  std::vector<int> titem;
+  std::vector<double> a0, astop, cent, sbar;  // per trial
+  long long pairs = 0;                        // target x source pairs of one full iteration
+  // launch shape
+  int waves = 4;                            // waves per workgroup: target tile = 64 * waves
+  int chunks[kClasses] = {1, 1, 2, 4};      // source chunks per target tile, per size class
+  int grid_waves = 8192;                    // resident waves of the persistent pair kernel
 };
 
 static void free_batch(dugma_batch *b) {
   if (!b) return;
-  void *ptrs[] = {b->dT, b->dS, b->dsrc, b->dsrc_trial, b->ditems, b->dorder, b->dtitem,
-                  b->dpart, b->dtrials, b->ddone, b->dqueue, b->dident};
+  void *ptrs[] = {b->dT,    b->dS,    b->dsrc,    b->dsrc_trial, b->ditems, b->dorder, b->dtitem,
+                  b->dttgt, b->dnn,   b->dpart,   b->dtrials,    b->ddone,  b->dqueue, b->dident};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -338,6 +426,62 @@ static int bmalloc(T **p, size_t n) {
     *p = nullptr;
     return hip_fail(e, "hipMalloc(batch)");
   }
+  return DUGMA_OK;
+}
+
+// Work items for the batch's current launch shape: per trial, target tiles
+// of 64 * waves points times the source chunks of its size class; queue
+// order longest first.  (Re)allocates the item arrays when they grow.
+static int make_items(dugma_batch *b) {
+  const int K = b->ntrials, tile_t = 64 * b->waves;
+  std::vector<BItem> items;
+  b->titem.assign(K + 1, 0);
+  for (int k = 0; k < K; ++k) {
+    b->titem[k] = (int)items.size();
+    const int nt = b->nt[k], ns = b->ns[k];
+    if (nt == 0 || ns == 0) continue;
+    const int nch = std::max(1, std::min(b->chunks[size_class(ns)], ns));
+    const int tile_s = (ns + nch - 1) / nch;
+    for (int t0 = 0; t0 < nt; t0 += tile_t)
+      for (int s0 = 0; s0 < ns; s0 += tile_s) {
+        BItem it;
+        memset(&it, 0, sizeof(it));
+        it.trial = k;
+        it.t0 = b->tb[k] + t0;
+        it.tn = std::min(tile_t, nt - t0);
+        it.s0 = b->sb[k] + s0;
+        it.sn = std::min(tile_s, ns - s0);
+        items.push_back(it);
+      }
+  }
+  b->titem[K] = (int)items.size();
+  std::vector<int> order(items.size()), ident(items.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::iota(ident.begin(), ident.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+    return (long long)items[u].tn * items[u].sn > (long long)items[v].tn * items[v].sn;
+  });
+  if (items.size() > b->items_cap) {
+    void *old[] = {b->ditems, b->dorder, b->dident, b->dpart};
+    for (void *p : old)
+      if (p) (void)hipFree(p);
+    b->ditems = nullptr;
+    b->dorder = b->dident = nullptr;
+    b->dpart = nullptr;
+    b->items_cap = 0;
+    int rc;
+    if ((rc = bmalloc(&b->ditems, items.size())) || (rc = bmalloc(&b->dorder, items.size())) ||
+        (rc = bmalloc(&b->dident, items.size())) || (rc = bmalloc(&b->dpart, items.size() * kStat)))
+      return rc;
+    b->items_cap = items.size();
+  }
+  b->nitems = (int)items.size();
+  if (!items.empty()) {
+    DUGMA_HIP(hipMemcpy(b->ditems, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(b->dorder, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(b->dident, ident.data(), ident.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  DUGMA_HIP(hipMemcpy(b->dtitem, b->titem.data(), b->titem.size() * sizeof(int), hipMemcpyHostToDevice));
   return DUGMA_OK;
 }
 
@@ -378,13 +522,15 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
   b->ntrials = ntrials;
   std::vector<Rec3I> tg;
   std::vector<double> sr;
-  std::vector<int> strial;
-  std::vector<BItem> items;
-  b->titem.assign(ntrials + 1, 0);
+  std::vector<int> strial, ttgt(ntrials + 1, 0);
+  b->tb.assign(ntrials, 0);
+  b->nt.assign(ntrials, 0);
+  b->sb.assign(ntrials, 0);
+  b->ns.assign(ntrials, 0);
   b->a0.resize(ntrials);
   b->astop.resize(ntrials);
+  b->sbar.resize(ntrials);
   b->cent.resize(3 * (size_t)ntrials);
-  const int tile_t = kBBlock, tile_s = 2048;
   for (int k = 0; k < ntrials; ++k) {
     if (N[k] < 1 || M[k] < 1 || !X[k] || !Y[k]) {
       free_batch(b);
@@ -408,10 +554,10 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       tv.push_back(v);
     }
     const int nt = (int)tv.size();
-    double r2 = 0.0, vbar = 0.0;
+    double r2 = 0.0, vt = 0.0, vs = 0.0;
     if (nt > 0)
       for (int a = 0; a < 3; ++a) c[a] /= nt;
-    const int tbase = (int)tg.size();
+    b->tb[k] = (int)tg.size();
     for (int i = 0; i < nt; ++i) {
       Rec3I r;
       memset(&r, 0, sizeof(r));
@@ -420,10 +566,10 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       r.hz = (float)(ty[3 * i + 2] - c[2]);
       r.s = (float)tv[i];
       r2 += (double)r.hx * r.hx + (double)r.hy * r.hy + (double)r.hz * r.hz;
-      vbar += tv[i];
+      vt += tv[i];
       tg.push_back(r);
     }
-    const int sbase = (int)(sr.size() / 4);
+    b->sb[k] = (int)(sr.size() / 4);
     int ns = 0;
     for (int j = 0; j < M[k]; ++j) {
       const double *p = Y[k] + (size_t)j * 3;
@@ -436,40 +582,22 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       for (int a = 0; a < 3; ++a) sr.push_back((double)(float)p[a]);
       sr.push_back(v);
       strial.push_back(k);
-      vbar += v;
+      vs += v;
       ++ns;
     }
+    b->nt[k] = nt;
+    b->ns[k] = ns;
+    ttgt[k + 1] = ttgt[k] + nt;
     for (int a = 0; a < 3; ++a) b->cent[3 * k + a] = c[a];
     // annealing from a quarter of the target's mean squared radius down to
     // a hundredth of the mean point variance
     b->pairs += (long long)nt * ns;
     b->a0[k] = nt ? 0.25 * r2 / nt : 0.0;
-    b->astop[k] = (nt + ns) ? 0.01 * vbar / (nt + ns) : 0.0;
-    b->titem[k] = (int)items.size();
-    for (int t0 = 0; t0 < nt; t0 += tile_t)
-      for (int s0 = 0; s0 < ns; s0 += tile_s) {
-        BItem it;
-        memset(&it, 0, sizeof(it));
-        it.trial = k;
-        it.t0 = tbase + t0;
-        it.tn = std::min(tile_t, nt - t0);
-        it.s0 = sbase + s0;
-        it.sn = std::min(tile_s, ns - s0);
-        items.push_back(it);
-      }
+    b->astop[k] = (nt + ns) ? 0.01 * (vt + vs) / (nt + ns) : 0.0;
+    b->sbar[k] = (nt ? vt / nt : 0.0) + (ns ? vs / ns : 0.0);
   }
-  b->titem[ntrials] = (int)items.size();
   b->ntgt = (int)tg.size();
   b->nsrc = (int)(sr.size() / 4);
-  b->nitems = (int)items.size();
-  // queue order: most work first
-  std::vector<int> order(items.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
-    return (long long)items[u].tn * items[u].sn > (long long)items[v].tn * items[v].sn;
-  });
-  std::vector<int> ident(items.size());
-  std::iota(ident.begin(), ident.end(), 0);
   int rc = DUGMA_OK;
 #define BTRY(x)                     \
   do {                              \
@@ -483,11 +611,9 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
     BTRY(bmalloc(&b->dS, (size_t)b->nsrc));
     BTRY(bmalloc(&b->dsrc, sr.size()));
     BTRY(bmalloc(&b->dsrc_trial, strial.size()));
-    BTRY(bmalloc(&b->ditems, items.size()));
-    BTRY(bmalloc(&b->dorder, order.size()));
-    BTRY(bmalloc(&b->dident, ident.size()));
-    BTRY(bmalloc(&b->dtitem, b->titem.size()));
-    BTRY(bmalloc(&b->dpart, items.size() * kStat));
+    BTRY(bmalloc(&b->dtitem, (size_t)ntrials + 1));
+    BTRY(bmalloc(&b->dttgt, (size_t)ntrials + 1));
+    BTRY(bmalloc(&b->dnn, tg.size()));
     BTRY(bmalloc(&b->dtrials, (size_t)ntrials));
     BTRY(bmalloc(&b->ddone, (size_t)ntrials));
     BTRY(bmalloc(&b->dqueue, 1));
@@ -498,10 +624,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
     } ups[] = {{b->dT, tg.data(), tg.size() * sizeof(Rec3I)},
                {b->dsrc, sr.data(), sr.size() * sizeof(double)},
                {b->dsrc_trial, strial.data(), strial.size() * sizeof(int)},
-               {b->ditems, items.data(), items.size() * sizeof(BItem)},
-               {b->dorder, order.data(), order.size() * sizeof(int)},
-               {b->dident, ident.data(), ident.size() * sizeof(int)},
-               {b->dtitem, b->titem.data(), b->titem.size() * sizeof(int)}};
+               {b->dttgt, ttgt.data(), ttgt.size() * sizeof(int)}};
     for (const Up &u : ups)
       if (u.n) {
         e = hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice);
@@ -509,6 +632,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       }
     e = hipMemset(b->dqueue, 0, sizeof(unsigned));
     if (e != hipSuccess) BTRY(hip_fail(e, "hipMemset(queue)"));
+    BTRY(make_items(b));
   }
 #undef BTRY
   *out = b;
@@ -561,29 +685,74 @@ extern "C" int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
   return DUGMA_OK;
 }
 
+template <int W>
+static void launch_pairs(dugma_batch *b, int grid, const int *order, int ni, bool sigma) {
+  if (sigma)
+    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, b->dT, b->dS,
+                       b->ditems, order, ni, b->dqueue, b->ddone, b->dtrials, b->dnn, b->dpart);
+  else
+    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, b->dT, b->dS,
+                       b->ditems, order, ni, b->dqueue, b->ddone, b->dtrials, b->dnn, b->dpart);
+}
+
+// grid: workgroups of the persistent pair kernel (0: from the launch shape)
 static int launch_iteration(dugma_batch *b, int k0, int k1, const BParams &bp,
-                            bool one_trial) {
-  // sources of trials [k0, k1)
+                            bool one_trial, int grid) {
   int s0 = 0, s1 = b->nsrc;
   const int *order = b->dorder;
-  int i0 = 0, ni = b->nitems;
+  int ni = b->nitems;
   if (one_trial) {
     // items of trial k0 are contiguous: [titem[k0], titem[k0+1])
-    i0 = b->titem[k0];
+    const int i0 = b->titem[k0];
     ni = b->titem[k0 + 1] - i0;
     order = b->dident + i0;
-    s0 = b->src_off[k0];
-    s1 = b->src_off[k0 + 1];
+    s0 = b->sb[k0];
+    s1 = s0 + b->ns[k0];
   }
   if (s1 > s0)
-    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + kBBlock - 1) / kBBlock), dim3(kBBlock), 0, b->stream,
-                       b->dsrc, b->dsrc_trial, s0, s1, b->dtrials, b->ddone, b->dS);
-  if (ni > 0)
-    hipLaunchKernelGGL(k_batch_pairs, dim3(std::min(b->grid, ni)), dim3(kBBlock), 0, b->stream, b->dT, b->dS,
-                       b->ditems, order, ni, b->dqueue, b->ddone, b->dpart);
-  hipLaunchKernelGGL(k_batch_update, dim3((k1 - k0 + kBBlock - 1) / kBBlock), dim3(kBBlock), 0, b->stream,
-                     b->dpart, b->dtitem, k0, k1, b->dtrials, b->ddone, b->dqueue, bp);
+    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + 255) / 256), dim3(256), 0, b->stream, b->dsrc,
+                       b->dsrc_trial, s0, s1, b->dtrials, b->ddone, b->dS);
+  if (ni > 0) {
+    if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
+    grid = std::min(grid, ni);
+    switch (b->waves) {
+      case 1: launch_pairs<1>(b, grid, order, ni, bp.sigma != 0); break;
+      case 2: launch_pairs<2>(b, grid, order, ni, bp.sigma != 0); break;
+      default: launch_pairs<4>(b, grid, order, ni, bp.sigma != 0); break;
+    }
+  }
+  hipLaunchKernelGGL(k_batch_update, dim3(k1 - k0), dim3(kUpd), 0, b->stream, b->dpart, b->dtitem, b->dttgt, k0,
+                     k1, b->dtrials, b->ddone, b->dqueue, b->dnn, bp);
   DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+// Trial states at the start of a run, uploaded with every done flag clear
+// and every nearest distance at +inf.
+static int start_trials(dugma_batch *b, const double *R0, const double *t0, bool anneal, bool sigma) {
+  const int K = b->ntrials;
+  std::vector<BTrial> tr(K);
+  for (int k = 0; k < K; ++k) {
+    memset(&tr[k], 0, sizeof(BTrial));
+    for (int a = 0; a < 9; ++a) tr[k].R[a] = R0[9 * (size_t)k + a];
+    for (int a = 0; a < 3; ++a) {
+      tr[k].t[a] = t0[3 * (size_t)k + a];
+      tr[k].c[a] = b->cent[3 * (size_t)k + a];
+    }
+    tr[k].a = anneal ? b->a0[k] : 0.0;
+    tr[k].a_stop = b->astop[k];
+    tr[k].g = 1.0;
+    tr[k].sbar = b->sbar[k] > 0 ? b->sbar[k] : 1.0;
+    tr[k].sig_ready = sigma ? 0 : 1;
+  }
+  std::vector<int> done(K, 0);
+  DUGMA_HIP(hipMemcpyAsync(b->dtrials, tr.data(), sizeof(BTrial) * K, hipMemcpyHostToDevice, b->stream));
+  DUGMA_HIP(hipMemcpyAsync(b->ddone, done.data(), sizeof(int) * K, hipMemcpyHostToDevice, b->stream));
+  if (b->ntgt > 0)
+    hipLaunchKernelGGL(k_batch_fill, dim3((b->ntgt + 255) / 256), dim3(256), 0, b->stream, b->dnn, b->ntgt,
+                       kInfBits);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
   return DUGMA_OK;
 }
 
@@ -594,44 +763,26 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   if (!b || !R0 || !t0 || !opts) return fail(DUGMA_ERR_ARG, "dugma_batch_register: bad argument");
   if (opts->max_iter < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_register: max_iter must be >= 1");
   const int K = b->ntrials;
-  // source offsets per trial (host), for one-trial launches
-  if (b->src_off.empty()) {
-    std::vector<int> strial(b->nsrc);
-    hipError_t e = hipMemcpy(strial.data(), b->dsrc_trial, sizeof(int) * b->nsrc, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(src_trial)");
-    b->src_off.assign(K + 1, 0);
-    for (int j = 0; j < b->nsrc; ++j) b->src_off[strial[j] + 1]++;
-    for (int k = 0; k < K; ++k) b->src_off[k + 1] += b->src_off[k];
-  }
-  std::vector<BTrial> tr(K);
-  for (int k = 0; k < K; ++k) {
-    memset(&tr[k], 0, sizeof(BTrial));
-    for (int a = 0; a < 9; ++a) tr[k].R[a] = R0[9 * (size_t)k + a];
-    for (int a = 0; a < 3; ++a) {
-      tr[k].t[a] = t0[3 * (size_t)k + a];
-      tr[k].c[a] = b->cent[3 * (size_t)k + a];
-    }
-    tr[k].a = opts->anneal ? b->a0[k] : 0.0;
-    tr[k].a_stop = b->astop[k];
-  }
-  std::vector<int> done(K, 0);
-  DUGMA_HIP(hipMemcpy(b->dtrials, tr.data(), sizeof(BTrial) * K, hipMemcpyHostToDevice));
-  DUGMA_HIP(hipMemcpy(b->ddone, done.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+  const bool sigma = opts->sigma_update != 0;
+  int rc = start_trials(b, R0, t0, opts->anneal != 0, sigma);
+  if (rc) return rc;
   BParams bp;
   memset(&bp, 0, sizeof(bp));
   bp.beta = opts->anneal_decay > 0 ? opts->anneal_decay : 0.85;
   bp.tol_rot = opts->tol_rot;
   bp.tol_trans = opts->tol_trans;
   bp.max_iter = opts->max_iter;
-  b->grid = opts->grid > 0 ? opts->grid : 2048;
-  int rc = DUGMA_OK;
+  bp.sigma = sigma ? 1 : 0;
+  const int grid = opts->grid;
+  // with the sigma update the first pass only measures sig
+  const int passes = opts->max_iter + (sigma ? 1 : 0);
   if (opts->one_trial_per_launch) {
     // baseline: each trial on its own, 3 launches and a host check per iteration
     int *hdone = nullptr;
     DUGMA_HIP(hipHostMalloc((void **)&hdone, sizeof(int), hipHostMallocDefault));
     for (int k = 0; k < K && !rc; ++k) {
-      for (int it = 0; it < opts->max_iter && !rc; ++it) {
-        rc = launch_iteration(b, k, k + 1, bp, true);
+      for (int it = 0; it < passes && !rc; ++it) {
+        rc = launch_iteration(b, k, k + 1, bp, true, grid);
         if (rc) break;
         hipError_t e = hipMemcpyAsync(hdone, b->ddone + k, sizeof(int), hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -644,7 +795,7 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     DUGMA_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_iteration(b, 0, K, bp, false);
+    rc = launch_iteration(b, 0, K, bp, false, grid);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
     if (!rc) {
@@ -652,7 +803,7 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
       if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
     }
     const int check = opts->check_every > 0 ? opts->check_every : 10;
-    for (int it = 0; it < opts->max_iter && !rc; ++it) {
+    for (int it = 0; it < passes && !rc; ++it) {
       e = hipGraphLaunch(ge, b->stream);
       if (e != hipSuccess) {
         rc = hip_fail(e, "hipGraphLaunch");
@@ -675,9 +826,10 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
   } else {
-    for (int it = 0; it < opts->max_iter && !rc; ++it) rc = launch_iteration(b, 0, K, bp, false);
+    for (int it = 0; it < passes && !rc; ++it) rc = launch_iteration(b, 0, K, bp, false, grid);
   }
   if (rc) return rc;
+  std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
   DUGMA_HIP(hipStreamSynchronize(b->stream));
   const double cn = 1.0 / (6.283185307179586476925 * std::sqrt(6.283185307179586476925));
@@ -691,4 +843,112 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     if (E) E[k] = cn * tr[k].E;
   }
   return DUGMA_OK;
+}
+
+// Average time of one all-trial iteration (ms) under the current launch
+// shape, from the given start poses: one untimed pass, then `reps` timed.
+static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bool sigma, int reps,
+                          hipEvent_t ev0, hipEvent_t ev1, double *ms) {
+  int rc = start_trials(b, R0, t0, false, sigma);
+  if (rc) return rc;
+  BParams bp;
+  memset(&bp, 0, sizeof(bp));
+  bp.beta = 1.0;
+  bp.tol_rot = -1.0;  // nothing converges while timing
+  bp.tol_trans = -1.0;
+  bp.max_iter = 1 << 30;
+  bp.sigma = sigma ? 1 : 0;
+  if ((rc = launch_iteration(b, 0, b->ntrials, bp, false, 0))) return rc;
+  DUGMA_HIP(hipEventRecord(ev0, b->stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = launch_iteration(b, 0, b->ntrials, bp, false, 0))) return rc;
+  DUGMA_HIP(hipEventRecord(ev1, b->stream));
+  DUGMA_HIP(hipEventSynchronize(ev1));
+  float f = 0.0f;
+  DUGMA_HIP(hipEventElapsedTime(&f, ev0, ev1));
+  *ms = (double)f / reps;
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
+                                    int sigma_update, int *waves, int *chunks, int *grid_waves,
+                                    double *ms) {
+  if (!b || !R0 || !t0) return fail(DUGMA_ERR_ARG, "dugma_batch_autotune: bad argument");
+  const bool sigma = sigma_update != 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DUGMA_HIP(hipEventCreate(&ev0));
+  hipError_t e = hipEventCreate(&ev1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(ev0);
+    return hip_fail(e, "hipEventCreate");
+  }
+  int rc = DUGMA_OK;
+  double best = 0.0, cur = 0.0;
+  bool present[kClasses] = {false, false, false, false};
+  for (int k = 0; k < b->ntrials; ++k)
+    if (b->nt[k] > 0 && b->ns[k] > 0) present[size_class(b->ns[k])] = true;
+  // 1. waves per workgroup (the target tile)
+  const int wave_opts[] = {1, 2, 4};
+  int best_w = b->waves;
+  for (int w : wave_opts) {
+    b->waves = w;
+    if ((rc = make_items(b)) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
+    if (best == 0.0 || cur < best) {
+      best = cur;
+      best_w = w;
+    }
+  }
+  b->waves = best_w;
+  // 2. source chunks per target tile, one size class at a time
+  const int chunk_opts[] = {1, 2, 4, 8};
+  for (int c = 0; c < kClasses && !rc; ++c) {
+    if (!present[c]) continue;
+    const int keep = b->chunks[c];
+    int best_n = keep;
+    for (int n : chunk_opts) {
+      if (n == keep) continue;
+      b->chunks[c] = n;
+      if ((rc = make_items(b)) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
+      if (cur < best) {
+        best = cur;
+        best_n = n;
+      }
+    }
+    b->chunks[c] = best_n;
+  }
+  // 3. resident waves of the persistent kernel
+  if (!rc) rc = make_items(b);
+  const int grid_opts[] = {4096, 8192, 16384, 32768};
+  const int keep_g = b->grid_waves;
+  int best_g = keep_g;
+  for (int gw : grid_opts) {
+    if (rc || gw == keep_g) continue;
+    b->grid_waves = gw;
+    if ((rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
+    if (cur < best) {
+      best = cur;
+      best_g = gw;
+    }
+  }
+  b->grid_waves = best_g;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (rc) return rc;
+  if (waves) *waves = b->waves;
+  if (chunks)
+    for (int c = 0; c < kClasses; ++c) chunks[c] = b->chunks[c];
+  if (grid_waves) *grid_waves = b->grid_waves;
+  if (ms) *ms = best;
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunks, int grid_waves) {
+  if (!b || (waves != 1 && waves != 2 && waves != 4) || !chunks || grid_waves < 1)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_set_shape: waves must be 1, 2 or 4, chunks >= 1, grid_waves >= 1");
+  for (int c = 0; c < kClasses; ++c)
+    if (chunks[c] < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_set_shape: chunks must be >= 1");
+  b->waves = waves;
+  for (int c = 0; c < kClasses; ++c) b->chunks[c] = chunks[c];
+  b->grid_waves = grid_waves;
+  return make_items(b);
 }

@@ -606,14 +606,14 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
     if (D == 3) {
       static const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
       for (int k = 0; k < 6; ++k) {
-        double v = 0.5 * cn * (a[4 + k] - a[10 + k]);
+        double v = 0.5 * cn * a[4 + k];
         G[ij[k][0] + 3 * ij[k][1]] = v;
         G[ij[k][1] + 3 * ij[k][0]] = v;
       }
     } else {
       static const int ij[3][2] = {{0, 0}, {0, 1}, {1, 1}};
       for (int k = 0; k < 3; ++k) {
-        double v = 0.5 * cn * (a[3 + k] - a[6 + k]);
+        double v = 0.5 * cn * a[3 + k];
         G[ij[k][0] + 2 * ij[k][1]] = v;
         G[ij[k][1] + 2 * ij[k][0]] = v;
       }

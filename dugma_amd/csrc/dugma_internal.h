@@ -41,9 +41,10 @@ struct alignas(16) Rec2I {  // 2-D, isotropic covariance s*I, 32 B
 // Per-stationary-point accumulator layout of the pair kernels (FP64):
 //  [0]      energy sum_j e_ij                    (without the (2pi)^-D/2 factor)
 //  [1..D]   EW = sum_j e_ij w_ij,  w = S^-1 d,  d = p - q
-//  full covariance:  WW = sum_j e w w^T (D(D+1)/2), C = sum_j e adj(S)/det
+//  full covariance:  B = sum_j e (w w^T - S^-1)  (D(D+1)/2 entries, upper
+//                    triangle row by row)
 //  isotropic:        G = sum_j (e/s)(|d|^2/s - D)
-enum { kAcc3F = 16, kAcc3I = 5, kAcc2F = 9, kAcc2I = 4 };
+enum { kAcc3F = 10, kAcc3I = 5, kAcc2F = 6, kAcc2I = 4 };
 
 // Pose-gradient partial per workgroup: [E, dofs...], padded to 8 doubles.
 enum { kPosePart = 8 };

@@ -5,9 +5,9 @@
 // in SGPRs): d = p - q, S = Sp + Sq,
 //   e = det(S)^-1/2 exp(-d^T S^-1 d / 2)          (times (2 pi)^-D/2 later)
 // and, with w = S^-1 d, the adjoint accumulators
-//   EW += e w,  WW += e w w^T,  C += e S^-1        (full covariance)
+//   EW += e w,  B += e (w w^T - S^-1)              (full covariance)
 //   EW += e w,  G  += (e/s)(|d|^2/s - D)          (isotropic, S = s I)
-// from which dE/dp = -c EW and dE/dSp = c (WW - C) / 2, dE/ds_p = c G / 2.
+// from which dE/dp = -c EW and dE/dSp = c B / 2, dE/ds_p = c G / 2.
 // FP32 throughout; one v_rsq (or v_rcp) and one v_exp per pair.
 #pragma once
 
@@ -63,25 +63,19 @@ __device__ __forceinline__ void pair_eval(const Rec3F &p, const Rec3F &q, float 
   const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
   acc[0] += e;
   if (GRAD) {
-    // w = u/det:  EW += e w,  WW += e w w^T,  C += e adj/det
+    // w = u/det = v:  EW += e w = er2 u,  B += e (w w^T - adj/det)
+    // = er2 (v u^T - adj), one accumulator per entry of the symmetric B
     const float er2 = e * r2;
     acc[1] = fmaf(er2, u1, acc[1]);
     acc[2] = fmaf(er2, u2, acc[2]);
     acc[3] = fmaf(er2, u3, acc[3]);
-    const float f = er2 * r2;
-    const float f1 = f * u1, f2 = f * u2, f3 = f * u3;
-    acc[4] = fmaf(f1, u1, acc[4]);
-    acc[5] = fmaf(f1, u2, acc[5]);
-    acc[6] = fmaf(f1, u3, acc[6]);
-    acc[7] = fmaf(f2, u2, acc[7]);
-    acc[8] = fmaf(f2, u3, acc[8]);
-    acc[9] = fmaf(f3, u3, acc[9]);
-    acc[10] = fmaf(er2, c11, acc[10]);
-    acc[11] = fmaf(er2, c12, acc[11]);
-    acc[12] = fmaf(er2, c13, acc[12]);
-    acc[13] = fmaf(er2, c22, acc[13]);
-    acc[14] = fmaf(er2, c23, acc[14]);
-    acc[15] = fmaf(er2, c33, acc[15]);
+    const float v1 = r2 * u1, v2 = r2 * u2, v3 = r2 * u3;
+    acc[4] = fmaf(er2, fmaf(v1, u1, -c11), acc[4]);
+    acc[5] = fmaf(er2, fmaf(v1, u2, -c12), acc[5]);
+    acc[6] = fmaf(er2, fmaf(v1, u3, -c13), acc[6]);
+    acc[7] = fmaf(er2, fmaf(v2, u2, -c22), acc[7]);
+    acc[8] = fmaf(er2, fmaf(v2, u3, -c23), acc[8]);
+    acc[9] = fmaf(er2, fmaf(v3, u3, -c33), acc[9]);
   }
 }
 
@@ -121,17 +115,14 @@ __device__ __forceinline__ void pair_eval(const Rec2F &p, const Rec2F &q, float 
   const float e = fexp2((qa * r2) * kNegHalfLog2e) * r;
   acc[0] += e;
   if (GRAD) {
+    // adj = [[a22, -a12], [-a12, a11]];  B += er2 (v u^T - adj), v = u/det
     const float er2 = e * r2;
     acc[1] = fmaf(er2, u1, acc[1]);
     acc[2] = fmaf(er2, u2, acc[2]);
-    const float f = er2 * r2;
-    const float f1 = f * u1, f2 = f * u2;
-    acc[3] = fmaf(f1, u1, acc[3]);
-    acc[4] = fmaf(f1, u2, acc[4]);
-    acc[5] = fmaf(f2, u2, acc[5]);
-    acc[6] = fmaf(er2, a22, acc[6]);
-    acc[7] = fmaf(-er2, a12, acc[7]);
-    acc[8] = fmaf(er2, a11, acc[8]);
+    const float v1 = r2 * u1, v2 = r2 * u2;
+    acc[3] = fmaf(er2, fmaf(v1, u1, -a22), acc[3]);
+    acc[4] = fmaf(er2, fmaf(v1, u2, a12), acc[4]);
+    acc[5] = fmaf(er2, fmaf(v2, u2, -a11), acc[5]);
   }
 }
 
@@ -159,7 +150,7 @@ __device__ __forceinline__ void pair_eval(const Rec2I &p, const Rec2I &q, float 
 // gradient for moving the sources (rotation about t, then translation) is
 // minus the gradient for moving the targets the same way:
 //   dE/dt = c sum_i EW_i
-//   dE/dw = c sum_i [ (y_i - t) x EW_i + vee(Sy_i B_i - B_i Sy_i) ],  B = WW - C
+//   dE/dw = c sum_i [ (y_i - t) x EW_i + vee(Sy_i B_i - B_i Sy_i) ]
 // (2-D: dE/dtheta = c sum_i [ EW^T J (y - t) - ((Sy B)_12 - (B Sy)_12) ]).
 // r: [E, rotation dofs, translation dofs].
 __device__ __forceinline__ void pose_contrib(const Rec3F &p, const double *a,
@@ -168,8 +159,8 @@ __device__ __forceinline__ void pose_contrib(const Rec3F &p, const double *a,
   const double py = (double)p.hy + (double)p.ly - t[1];
   const double pz = (double)p.hz + (double)p.lz - t[2];
   const double ex = a[1], ey = a[2], ez = a[3];
-  const double b11 = a[4] - a[10], b12 = a[5] - a[11], b13 = a[6] - a[12];
-  const double b22 = a[7] - a[13], b23 = a[8] - a[14], b33 = a[9] - a[15];
+  const double b11 = a[4], b12 = a[5], b13 = a[6];
+  const double b22 = a[7], b23 = a[8], b33 = a[9];
   const double s11 = p.s11, s12 = p.s12, s13 = p.s13, s22 = p.s22, s23 = p.s23, s33 = p.s33;
   // K = S B - B S (antisymmetric); vee(K) = (K32, K13, K21)
   const double sb32 = s13 * b12 + s23 * b22 + s33 * b23;  // (S B)_32
@@ -207,7 +198,7 @@ __device__ __forceinline__ void pose_contrib(const Rec2F &p, const double *a,
   const double px = (double)p.hx + (double)p.lx - t[0];
   const double py = (double)p.hy + (double)p.ly - t[1];
   const double ex = a[1], ey = a[2];
-  const double b11 = a[3] - a[6], b12 = a[4] - a[7], b22 = a[5] - a[8];
+  const double b11 = a[3], b12 = a[4], b22 = a[5];
   const double s11 = p.s11, s12 = p.s12, s22 = p.s22;
   r[0] += a[0];
   r[1] += ey * px - ex * py - ((s11 * b12 + s12 * b22) - (b11 * s12 + b12 * s22));

@@ -80,7 +80,10 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
       g[3] = -(a * gid[3] + b * gid[2]) / rho;
     }
   } else {
-    check(dugma_energy(h->ctx, Rn, tn, DUGMA_COV_AUTO, &E));
+    // full-covariance kernels, like the gradient pass: fmincon also evaluates
+    // dR that are not rotations, and only these kernels carry the source
+    // covariance through as (dR R) S (dR R)^T (isotropic kernels keep s I)
+    check(dugma_energy(h->ctx, Rn, tn, DUGMA_COV_FULL, &E));
   }
   plhs[0] = mxCreateDoubleMatrix(1, 1, mxREAL);
   mxGetPr(plhs[0])[0] = -E;

@@ -59,15 +59,28 @@ def _real_sph_harm(l, m, theta, phi):
     return math.sqrt(2) * norm * P * np.sin(am * phi)
 
 
+# Real spherical harmonics here are 4 pi-normalised (mean square 1 over the
+# sphere), so 0.2 c_lm is the RMS radial deformation contributed by each term.
+# With orthonormal Y_lm (mean square 1 / 4 pi) the same formula stays within
+# ~3% of a sphere, whose rotation no registration can recover to 2 degrees.
+SH_NORM = math.sqrt(4.0 * math.pi)
+R_MIN = 0.25  # coefficient draws whose radius dips below this are redrawn
+
+
 class Surface:
-    """Closed star-shaped surface r(theta, phi) = 1 + 0.2 sum c_lm Y_lm."""
+    """Closed star-shaped surface r(theta, phi) = 1 + 0.2 sum_{l=1..4,m}
+    c_lm Y_lm, c_lm ~ N(0, 1) / l^2 (l = 0 left out: 1/l^2 is undefined
+    there), Y_lm 4 pi-normalised real spherical harmonics."""
 
     def __init__(self, rng, lmax=4, nt=160, nph=320):
-        self.coef = [(l, m, rng.normal() / l ** 2) for l in range(1, lmax + 1)
-                     for m in range(-l, l + 1)]
         th = np.linspace(0, math.pi, nt + 1)
         ph = np.linspace(0, 2 * math.pi, nph + 1)[:-1]
         TH, PH = np.meshgrid(th, ph, indexing="ij")
+        while True:
+            self.coef = [(l, m, rng.normal() / l ** 2) for l in range(1, lmax + 1)
+                         for m in range(-l, l + 1)]
+            if self.radius(TH.ravel(), PH.ravel()).min() >= R_MIN:
+                break
         V = self.points(TH.ravel(), PH.ravel()).reshape(nt + 1, nph, 3)
         a = np.arange(nt)[:, None]
         b = np.arange(nph)[None, :]
@@ -84,7 +97,7 @@ class Surface:
     def radius(self, theta, phi):
         r = np.ones_like(theta)
         for l, m, c in self.coef:
-            r = r + 0.2 * c * _real_sph_harm(l, m, theta, phi)
+            r = r + 0.2 * SH_NORM * c * _real_sph_harm(l, m, theta, phi)
         return r
 
     def points(self, theta, phi):

@@ -134,7 +134,10 @@ int dugma_energy_grad(dugma_ctx *ctx, const double *R, const double *t,
  * independent variables; zero for left-out points).
  *  gdT : 12 (3-D) or 4 (2-D) doubles, dE with respect to the fmincon
  *        variables dT of energy3D.m / energy2D.m at dT = identity, where the
- *        pose is updated as R <- dR R, t <- t + dt (DUGMA.m:67-78). */
+ *        pose is updated as R <- dR R, t <- t + dt (DUGMA.m:67-78).
+ * These passes always run the full-covariance kernels (cov_mode is only
+ * checked): E and gdT describe source covariances R CY R^T also when R is
+ * not a rotation, as for dugma_energy with DUGMA_COV_FULL. */
 int dugma_energy_grad_points(dugma_ctx *ctx, const double *R, const double *t,
                              int cov_mode, double *E, double *gpose,
                              double *gX, double *gXC, double *gY, double *gYC,
@@ -172,11 +175,15 @@ int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
 
 /* ---- batched registration (no reference counterpart) -------------------- */
 
-/* Many independent 3-D rigid registrations of uneven sizes at once; each
+/* Many independent 3-D rigid registrations of uneven sizes at once (the
+ * DUGMA.m:57-123 loop for each trial, with a closed-form pose step).  Each
  * maximises the convolved energy with isotropic covariances (every X_C / Y_C
- * block must be s I) by majorise-minimise steps (weighted Procrustes, FP64)
- * under an annealed extra variance a I, a -> 0.  The DUGMA.m:57 loop for one
- * trial is the special case ntrials = 1. */
+ * block must be s I) by majorise-minimise steps (weighted Procrustes, FP64).
+ * The pair variance is g (s_i + s_j) + a: with the sigma update, g = sig /
+ * sbar where sig is DUGMA's mean squared nearest distance / 3 measured in
+ * each pass (update_sig.cu:238-260, applied as in compi.cu:642-661) and sbar
+ * the trial's mean target plus mean source variance; a is an optional
+ * annealed extra variance, a -> 0. */
 typedef struct dugma_batch dugma_batch;
 
 typedef struct {
@@ -188,7 +195,8 @@ typedef struct {
   int use_graph;             /* replay one captured iteration (hipGraph) */
   int one_trial_per_launch;  /* baseline: trials one after another, host check per step */
   int check_every;           /* graph mode: host checks for completion every n steps */
-  int grid;                  /* workgroups of the persistent pair kernel (0: 2048) */
+  int grid;                  /* workgroups of the persistent pair kernel (0: from the launch shape) */
+  int sigma_update;          /* 1: DUGMA sigma update (one extra measuring pass first) */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */
@@ -210,6 +218,18 @@ int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
                          int *iters, int *status, double *E);
 int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
                      long long *pairs_per_iteration);
+/* Launch shape of the persistent pair kernel: waves per workgroup (1, 2, 4;
+ * the target tile is 64 x waves points), source chunks per target tile for
+ * each of 4 size classes of trials (source count < 1000, < 2000, < 4000,
+ * more), and resident waves.  dugma_batch_autotune times whole iterations
+ * from the poses R0, t0 on the batch's own data, waves first, then each
+ * present size class, then the resident waves, keeps the fastest shape and
+ * reports it with its time per iteration (ms); any output may be NULL. */
+int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
+                         int sigma_update, int *waves, int *chunks,
+                         int *grid_waves, double *ms);
+int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunks,
+                          int grid_waves);
 int dugma_batch_free(dugma_batch *b);
 
 /* ---- the six reference handles ------------------------------------------ */
