@@ -216,6 +216,26 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
             "speedup": round(rps / rps1, 2)}
 
 
+def bench_knn(dg, d, k=8, reps=3):
+    """k nearest transformed sources of every target (the sigma update of the
+    200K x 200K configuration), wall time per call including the copy of the
+    N x k indices and distances to the host."""
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = d["R_G"], d["t_G"]
+        dg.knn(h, dg.KNN_TARGET_TO_SOURCE, k, R, t)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dg.knn(h, dg.KNN_TARGET_TO_SOURCE, k, R, t)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        pairs = float(h.N) * h.M
+    finally:
+        dg.fr_m(h)
+    log("knn k=%d: %.3f ms per call" % (k, ms))
+    return {"k": k, "ms_per_call": round(ms, 3), "value": round(pairs / (ms / 1e3) / 1e9, 3),
+            "unit": "Gpair-distance-evals/s"}
+
+
 def cpu_baseline(d, budget_s=15.0):
     from dugma_amd import workloads as W
     from oracle import oracle as O
@@ -329,7 +349,8 @@ def main():
         out["large"] = {"workload": "3-D 200,000 x 200,000 isotropic (8-NN sigma), fused energy + pose adjoint",
                         "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
-                        "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED}
+                        "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
+                        "knn": bench_knn(dg, d3)}
     if not args.no_batch:
         out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_cpu:

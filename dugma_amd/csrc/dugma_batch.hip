@@ -49,7 +49,7 @@
 namespace dugma {
 
 constexpr int kBSub = 64;
-constexpr int kStat = 20;         // per-item statistics (doubles)
+constexpr int kStat = 20;         // stride of the per-item statistics (doubles; 17 used)
 constexpr int kUpd = 256;         // threads of the update kernel (one workgroup per trial)
 constexpr int kClasses = 4;       // size classes by source count: < 1000, < 2000, < 4000, more
 constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
@@ -122,11 +122,15 @@ __global__ __launch_bounds__(256) void k_batch_fill(unsigned *__restrict__ p, in
 }
 
 // Statistics of one item, per target i (lane): W_i = sum_j w_ij,
-// B_i = sum_j w_ij x'_j, E_i = sum_j e_ij, Q_i = sum_j w_ij |d_ij|^2 with
-// w = e / s_ij, s_ij = g s_i + (g s_j + a).  Item totals: [E, W, sum W_i y_i
-// (3), sum B_i (3), sum y_i B_i^T (9, row a = y_a), sum Q_i].  With SIG the
-// item's smallest |d_ij|^2 per target is folded into nn[i] (atomicMin on the
-// bits of a non-negative float, which order like the values).
+// B_i = sum_j w_ij x'_j, E_i = sum_j e_ij with w = e / s_ij,
+// s_ij = g s_i + (g s_j + a).  Item totals: [E, W, sum W_i y_i (3),
+// sum B_i (3), sum y_i B_i^T (9, row a = y_a)].  With SIG the item's
+// smallest |d_ij|^2 per target is folded into nn[i] (atomicMin on the bits of
+// a non-negative float, which order like the values).  The source stream is
+// software-pipelined: record j + 1 is requested before record j is used
+// (index clamped to the item), so the scalar-load latency overlaps the
+// arithmetic.
+constexpr int kItemStat = 17;
 template <int W, bool SIG>
 __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const Rec3I *__restrict__ T, const Rec3I *__restrict__ S,
@@ -135,7 +139,7 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const BTrial *__restrict__ trials, unsigned *__restrict__ nn,
     double *__restrict__ part) {
   __shared__ int s_p;
-  __shared__ double red[W][kStat];
+  __shared__ double red[W][kItemStat];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
@@ -151,15 +155,17 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const bool valid = (wave * 64 + lane) < it.tn;
     const Rec3I pt = T[valid ? ti : it.t0];
     const float gs = g * pt.s;
-    double dE = 0.0, dW = 0.0, dB[3] = {0.0, 0.0, 0.0}, dQ = 0.0;
+    double dE = 0.0, dW = 0.0, dB[3] = {0.0, 0.0, 0.0};
     float dmin = __builtin_huge_valf();
     const int s_end = it.s0 + it.sn;
+    Rec3I qn = S[it.s0];  // it.sn >= 1
     for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
-      float fE = 0.0f, fW = 0.0f, fB0 = 0.0f, fB1 = 0.0f, fB2 = 0.0f, fQ = 0.0f;
+      float fE = 0.0f, fW = 0.0f, fB0 = 0.0f, fB1 = 0.0f, fB2 = 0.0f;
       const int j1 = min(j0 + kBSub, s_end);
 #pragma unroll 2
       for (int j = j0; j < j1; ++j) {
-        const Rec3I q = S[j];
+        const Rec3I q = qn;
+        qn = S[min(j + 1, s_end - 1)];
         const float dx = (pt.hx - q.hx) - q.lx;
         const float dy = (pt.hy - q.hy) - q.ly;
         const float dz = (pt.hz - q.hz) - q.lz;
@@ -174,19 +180,17 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
         fB0 = fmaf(w, q.hx, fB0);
         fB1 = fmaf(w, q.hy, fB1);
         fB2 = fmaf(w, q.hz, fB2);
-        fQ = fmaf(w, d2, fQ);
       }
       dE += fE;
       dW += fW;
       dB[0] += fB0;
       dB[1] += fB1;
       dB[2] += fB2;
-      dQ += fQ;
     }
     if (SIG && valid) atomicMin(nn + ti, __float_as_uint(dmin));
-    double r[kStat];
+    double r[kItemStat];
 #pragma unroll
-    for (int a = 0; a < kStat; ++a) r[a] = 0.0;
+    for (int a = 0; a < kItemStat; ++a) r[a] = 0.0;
     if (valid) {
       const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
       r[0] = dE;
@@ -198,17 +202,14 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
 #pragma unroll
         for (int b = 0; b < 3; ++b) r[8 + 3 * a + b] = y[a] * dB[b];
       }
-      r[17] = dQ;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-      for (int a = 0; a < 18; ++a) r[a] += __shfl_xor(r[a], off);
+    for (int a = 0; a < kItemStat; ++a) r[a] = wave_sum(r[a]);
     if (lane == 0)
 #pragma unroll
-      for (int a = 0; a < kStat; ++a) red[wave][a] = r[a];
+      for (int a = 0; a < kItemStat; ++a) red[wave][a] = r[a];
     __syncthreads();
-    if (threadIdx.x < kStat) {
+    if (threadIdx.x < kItemStat) {
       double s = red[0][threadIdx.x];
 #pragma unroll
       for (int w = 1; w < W; ++w) s += red[w][threadIdx.x];
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   double st[kStat];
   for (int a = 0; a < kStat; ++a) st[a] = 0.0;
   for (int i = titem[k]; i < titem[k + 1]; ++i)
-    for (int a = 0; a < 18; ++a) st[a] += part[(size_t)i * kStat + a];
+    for (int a = 0; a < kItemStat; ++a) st[a] += part[(size_t)i * kStat + a];
   const double W = st[1];
   tr.E = st[0];
   tr.iters += 1;

@@ -56,6 +56,10 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
 #pragma unroll
     for (int a = 0; a < NA; ++a) dacc[k][a] = 0.0;
 
+  // the streamed records are software-pipelined two deep: record j + 2 is
+  // requested while record j is used (q0 < q1 for every workgroup; indices
+  // clamped to the chunk)
+  Rec qa = Q[q0], qb = Q[min(q0 + 1, q1 - 1)];
   for (int j0 = q0; j0 < q1; j0 += kSub) {
     float acc[T][NA];
 #pragma unroll
@@ -65,7 +69,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
     const int j1 = min(j0 + kSub, q1);
 #pragma unroll 2
     for (int j = j0; j < j1; ++j) {
-      const Rec q = Q[j];
+      const Rec q = qa;
+      qa = qb;
+      qb = Q[min(j + 2, q1 - 1)];
 #pragma unroll
       for (int k = 0; k < T; ++k) pair_eval<MODE, COMP>(p[k], q, acc[k]);
     }
@@ -103,9 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int a = 0; a < NR; ++a) r[a] += __shfl_xor(r[a], off);
+  for (int a = 0; a < NR; ++a) r[a] = wave_sum(r[a]);
   __shared__ double red[kBlock / 64][kPosePart];
   if (lane == 0)
 #pragma unroll

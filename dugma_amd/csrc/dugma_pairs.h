@@ -21,6 +21,30 @@ __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Sum of v over the 64 lanes of a wave, returned in every lane.  DPP row
+// rotations by 8, 4, 2, 1 sum each row of 16 lanes in registers (no LDS
+// traffic), then the four row sums are read from lanes 0, 16, 32, 48 and
+// added in a fixed order, so the result is deterministic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_move<0x128>(v);  // row_ror:8
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x122>(v);  // row_ror:2
+  v += dpp_move<0x121>(v);  // row_ror:1
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const double r0 = __hiloint2double(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(lo, 0));
+  const double r1 = __hiloint2double(__builtin_amdgcn_readlane(hi, 16), __builtin_amdgcn_readlane(lo, 16));
+  const double r2 = __hiloint2double(__builtin_amdgcn_readlane(hi, 32), __builtin_amdgcn_readlane(lo, 32));
+  const double r3 = __hiloint2double(__builtin_amdgcn_readlane(hi, 48), __builtin_amdgcn_readlane(lo, 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
 template <class Rec> struct RecTraits;
 template <> struct RecTraits<Rec3F> { static constexpr int D = 3, NACC = kAcc3F; static constexpr bool FULL = true; };
 template <> struct RecTraits<Rec3I> { static constexpr int D = 3, NACC = kAcc3I; static constexpr bool FULL = false; };

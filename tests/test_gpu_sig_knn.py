@@ -113,3 +113,36 @@ def test_knn_ties_lowest_index(oracle, k):
         np.testing.assert_array_equal(d2, rd)
     finally:
         dg.fr_m(h)
+
+
+def test_knn_full_size_config3(oracle):
+    """Config 3 at its full size (200,000 x 200,000, k = 8): every row holds
+    valid indices sorted by (distance, index), self mode never returns the
+    query itself, and 256 sampled queries are bit-exact against the oracle's
+    brute force (self mode through k + 1 neighbours with the query dropped)."""
+    d = W.config3()
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = d["R_G"], d["t_G"]
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_TO_SOURCE, 8, R, t)
+        idx_s, d2_s = dg.knn(h, dg.KNN_TARGET_SELF, 8)
+    finally:
+        dg.fr_m(h)
+    Xf = d["X"].T.astype(np.float32)
+    Yt = oracle.transform_points(d["Y"].T.astype(np.float32), R, t)
+    N, M = Xf.shape[0], Yt.shape[0]
+    for I, D2, ncand in ((idx, d2, M), (idx_s, d2_s, N)):
+        assert I.min() >= 0 and I.max() < ncand
+        ordered = (D2[:, 1:] > D2[:, :-1]) | ((D2[:, 1:] == D2[:, :-1]) & (I[:, 1:] > I[:, :-1]))
+        assert ordered.all()
+    assert not (idx_s == np.arange(N)[:, None]).any()
+    q = np.random.default_rng(0).choice(N, 256, replace=False)
+    ri, rd = oracle.knn(Xf[q], Yt, 8, False)
+    np.testing.assert_array_equal(idx[q], ri)
+    np.testing.assert_array_equal(d2[q], rd)
+    ri, rd = oracle.knn(Xf[q], Xf, 9, False)
+    for r, qi in enumerate(q):
+        keep = ri[r] != qi
+        assert (~keep).sum() == 1
+        np.testing.assert_array_equal(idx_s[qi], ri[r][keep][:8])
+        np.testing.assert_array_equal(d2_s[qi], rd[r][keep][:8])
