@@ -229,6 +229,7 @@ static void free_ctx(dugma_ctx *c) {
   c->part.release();
   c->scratch2.release();
   if (c->hpinned) (void)hipHostFree(c->hpinned);
+  if (c->hparts) (void)hipHostFree(c->hparts);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -287,6 +288,14 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     c->hres = c->hpinned + dugma_ctx::kHostStage / 2;
     e = hipHostGetDevicePointer((void **)&c->dres, c->hres, 0);
     if (e != hipSuccess) TRY(hip_fail(e, "hipHostGetDevicePointer"));
+    e = hipHostMalloc((void **)&c->hparts, sizeof(double) * kPosePart * dugma_ctx::kHostParts,
+                      hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->hparts = nullptr;
+      TRY(hip_fail(e, "hipHostMalloc(parts)"));
+    }
+    e = hipHostGetDevicePointer((void **)&c->dparts, c->hparts, 0);
+    if (e != hipSuccess) TRY(hip_fail(e, "hipHostGetDevicePointer(parts)"));
 
 
     // ---- reference state (init.cu:264-331) ----

@@ -387,9 +387,15 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   if (rc) return rc;
   Plan pl = ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
-  rc = c->part.ensure(nparts * kPosePart * sizeof(double));
-  if (rc) return rc;
-  double *parts = (double *)c->part.ptr;
+  // few partials: the pair kernel writes them to pinned host memory and the
+  // host sums them after the sync; many: one device reduction launch
+  const bool host_sum = nparts <= (size_t)dugma_ctx::kHostParts;
+  double *parts = c->dparts;
+  if (!host_sum) {
+    rc = c->part.ensure(nparts * kPosePart * sizeof(double));
+    if (rc) return rc;
+    parts = (double *)c->part.ptr;
+  }
   const int comp = comp_enabled() ? 1 : 0;
   PairTimer timer(c);
   if (D == 3) {
@@ -409,17 +415,26 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
+  if (!host_sum)
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
   DUGMA_HIP(hipGetLastError());
   DUGMA_HIP(hipStreamSynchronize(c->stream));
   timer.finish();
+  double sum[kPosePart] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (host_sum) {
+    const volatile double *hp = c->hparts;
+    for (size_t q = 0; q < nparts; ++q)
+      for (int a = 0; a < width; ++a) sum[a] += hp[q * kPosePart + a];
+  } else {
+    const volatile double *hr = c->hres;
+    for (int a = 0; a < width; ++a) sum[a] = hr[a];
+  }
   // unscale: E ~ s^D, rotation gradient ~ s^D, translation gradient ~ s^(D+1)
   const double cn = gauss_norm(D);
   const double sD = std::pow(c->scale, D);
   const int nrot = D == 3 ? 3 : 1;
-  const volatile double *hr = c->hres;
-  res[0] = cn * sD * hr[0];
-  for (int a = 1; a < width; ++a) res[a] = cn * sD * hr[a] * (a > nrot ? c->scale : 1.0);
+  res[0] = cn * sD * sum[0];
+  for (int a = 1; a < width; ++a) res[a] = cn * sD * sum[a] * (a > nrot ? c->scale : 1.0);
   return DUGMA_OK;
 }
 

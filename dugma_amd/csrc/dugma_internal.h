@@ -136,4 +136,11 @@ struct dugma_ctx {
   dugma::DevBuf scratch2;    // reductions / k-NN merge
   double *hpinned = nullptr; // pinned host staging, kHostStage doubles
   static constexpr int kHostStage = 4096;
+  // pinned host buffer the pair kernel writes its per-workgroup partials to
+  // when there are at most kHostParts of them (summed on the host in a fixed
+  // order: no reduction launch).  Host reads of device-written pinned memory
+  // cost more than the reduction launch beyond a few hundred partials.
+  double *hparts = nullptr;
+  double *dparts = nullptr;  // device address of hparts
+  static constexpr int kHostParts = 256;
 };

@@ -295,6 +295,25 @@ def config4_trial(trial, surfaces=None, workers=-1):
                 R_G=R, t_G=t, noise=noise, outliers=outl)
 
 
+def kinect_like(trial=0, n=4600, m=4000, deg=30.0, var=2.25):
+    """A registration the size of the reference's Kinect scenes (3,600-5,300
+    points per cloud, 8-43 degrees apart, median 30): a rotation of `deg`
+    degrees about a random axis of an anisotropic closed surface,
+    noise 0.003, isotropic covariances of the Kinect data's size relative to
+    the normalised cloud (the regime DUGMA's Lambda / sig weights are made
+    for)."""
+    rng = np.random.default_rng(seed(6, trial))
+    surf = Surface(rng, nt=60, nph=120)
+    A = np.diag([1.6, 1.0, 0.6])
+    src = surf.sample(rng, m) @ A
+    tgt0 = surf.sample(rng, n) @ A
+    R = rot_axis_angle(rng.normal(size=3), math.radians(deg))
+    t = rng.uniform(-0.1, 0.1, 3)
+    tgt = tgt0 @ R.T + t + rng.normal(scale=0.003, size=(n, 3))
+    return dict(X=tgt.T.copy(), X_C=iso_blocks(np.full(n, var)), Y=src.T.copy(),
+                Y_C=iso_blocks(np.full(m, var)), R_G=R, t_G=t)
+
+
 def contour2d(rng, n, harmonics=20):
     """Closed 2-D contour from a random Fourier series (coefficients N(0,1)/k)."""
     s = np.sort(rng.uniform(0, 2 * math.pi, size=n))
