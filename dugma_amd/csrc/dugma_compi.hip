@@ -12,21 +12,32 @@
 // times (a feature of target i) times (a feature of source j).  So:
 //  1. k_compi_w: for every source j, W_j = sum_i P_ij G_i over the 20 (3-D)
 //     or 12 (2-D) target features G = [1, n, n n^T, n^T B n, (B+B^T) n,
-//     sym(B)]; sources sit in lanes, targets stream through SGPRs.
-//  2. k_compi_moments: 325 (85) moments sum_j W'_j,f * m'_j.. * A'_j,..
-//  3. host: the moments are expanded into the polynomial in FP64.
+//     sym(B)]; sources sit in lanes (64 per workgroup), the workgroup's
+//     target chunk is split over its 4 waves and streams through SGPRs; the
+//     waves' sums meet in LDS, one partial per (chunk, source).
+//  2. k_compi_mom: per block of 32 sources, the chunk partials are summed
+//     into the features, staged in LDS, and the 325 (85) moments
+//     sum_j W'_j,f * m'_j.. * A'_j,.. accumulated; k_compi_msum adds the
+//     block partials, one wave per moment.
+//  3. host: a fixed sparse map (built once per D from the expansion of each
+//     moment) turns the moments into the polynomial coefficients in FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "dugma_internal.h"
 #include "dugma_monomials.inc"
+#include "dugma_pairs.h"
 
 namespace dugma {
 
 constexpr int kCBlock = 256;
+constexpr int kCWaves = kCBlock / 64;  // waves per workgroup of k_compi_w
 constexpr int kCSub = 64;
+constexpr int kMomSrc = 32;            // sources per workgroup of k_compi_mom
 constexpr int kFeat = 32;  // per-source feature stride (doubles) for the moments
 constexpr float kNegHalfLog2eC = -0.72134752044448170368f;
 
@@ -38,17 +49,17 @@ struct CompiPose {
 };
 
 // Streamed target record: n, sym(B), gX, then the features n n^T, n^T B n,
-// (B+B^T) n.  3-D: 3+6+1+6+1+3 = 20 floats (padded to 32).  2-D: 2+3+1+3+1+2.
+// (B+B^T) n.  3-D: 3+6+1+6+1+3 = 20 floats (80 B).  2-D: 2+3+1+3+1+2 = 12
+// floats (48 B).
 struct alignas(16) TgtRec3 {
   float n[3], b[6], g;
   float nn[6], s, v[3];
-  float pad[12];
 };
 struct alignas(16) TgtRec2 {
   float n[2], b[3], g;
   float nn[3], s, v[2];
-  float pad[4];
 };
+static_assert(sizeof(TgtRec3) == 80 && sizeof(TgtRec2) == 48, "target record layout");
 
 // sym index of (a,b), a <= b: 3-D 00 01 02 11 12 22 ; 2-D 00 01 11
 __host__ __device__ constexpr int sidx3(int a, int b) {
@@ -96,7 +107,6 @@ __global__ __launch_bounds__(kCBlock) void k_compi_tgt(
       }
     r.g = g;
     r.s = (float)s;
-    for (int q = 0; q < 12; ++q) r.pad[q] = 0.0f;
     reinterpret_cast<TgtRec3 *>(outv)[i] = r;
   } else {
     TgtRec2 r;
@@ -111,7 +121,6 @@ __global__ __launch_bounds__(kCBlock) void k_compi_tgt(
       }
     r.g = g;
     r.s = (float)s;
-    for (int q = 0; q < 4; ++q) r.pad[q] = 0.0f;
     reinterpret_cast<TgtRec2 *>(outv)[i] = r;
   }
 }
@@ -178,14 +187,21 @@ __global__ __launch_bounds__(kCBlock) void k_compi_src(
   for (int q = 0; q < D * D; ++q) f[FA + q] = (double)(float)C[q];
 }
 
-// W_j = sum_i P_ij G_i.  Lane = source j (stationary), targets streamed.
+// W_j = sum_i P_ij G_i.  Lane = source j (stationary, 64 per workgroup); the
+// workgroup's target chunk is split into kCWaves contiguous parts, one per
+// wave, streamed through SGPRs one record ahead (index clamped to the part).
+// The waves' FP64 sums meet in LDS in a fixed order: (w0 + w2) + (w1 + w3).
 template <int D>
 __global__ __launch_bounds__(kCBlock) void k_compi_w(
     const float *__restrict__ src, int M, const void *__restrict__ tgtv, int N,
     int chunk, double *__restrict__ part) {
   constexpr int NW = D == 3 ? 20 : 12;
   constexpr int SS = D == 3 ? 16 : 8;
-  const int j = blockIdx.x * kCBlock + threadIdx.x;
+  using Tgt = typename std::conditional<D == 3, TgtRec3, TgtRec2>::type;
+  const Tgt *__restrict__ T = reinterpret_cast<const Tgt *>(tgtv);
+  // the wave index is uniform: say so, so the target stream uses scalar loads
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = blockIdx.x * 64 + lane;
   const int jj = min(j, M - 1);
   const float *o = src + (size_t)jj * SS;
   float mh[3], ml[3], C[6];
@@ -196,124 +212,158 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
   }
 #pragma unroll
   for (int q = 0; q < D * (D + 1) / 2; ++q) C[q] = o[2 * D + q];
-  const int i0 = blockIdx.y * chunk;
-  const int i1 = min(i0 + chunk, N);
+  const int c0 = blockIdx.y * chunk;
+  const int c1 = min(c0 + chunk, N);
+  const int per = (c1 - c0 + kCWaves - 1) / kCWaves;
+  const int i0 = min(c0 + wave * per, c1);
+  const int i1 = min(i0 + per, c1);
   double W[NW];
 #pragma unroll
   for (int k = 0; k < NW; ++k) W[k] = 0.0;
-  for (int s0 = i0; s0 < i1; s0 += kCSub) {
-    float w[NW];
+  if (i0 < i1) {
+    Tgt tn = T[i0];
+    for (int s0 = i0; s0 < i1; s0 += kCSub) {
+      float w[NW];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) w[k] = 0.0f;
-    const int s1 = min(s0 + kCSub, i1);
-    for (int i = s0; i < s1; ++i) {
-      if (D == 3) {
-        const TgtRec3 t = reinterpret_cast<const TgtRec3 *>(tgtv)[i];
-        const float dx = (t.n[0] - mh[0]) - ml[0];
-        const float dy = (t.n[1] - mh[1]) - ml[1];
-        const float dz = (t.n[2] - mh[2]) - ml[2];
-        // mal1 = d^T C d, mal2 = d^T B d
-        const float c1 = fmaf(C[0], dx, fmaf(C[1], dy, C[2] * dz));
-        const float c2 = fmaf(C[1], dx, fmaf(C[3], dy, C[4] * dz));
-        const float c3 = fmaf(C[2], dx, fmaf(C[4], dy, C[5] * dz));
-        const float mal1 = fmaf(dx, c1, fmaf(dy, c2, dz * c3));
-        const float b1 = fmaf(t.b[0], dx, fmaf(t.b[1], dy, t.b[2] * dz));
-        const float b2 = fmaf(t.b[1], dx, fmaf(t.b[3], dy, t.b[4] * dz));
-        const float b3 = fmaf(t.b[2], dx, fmaf(t.b[4], dy, t.b[5] * dz));
-        const float mal2 = fmaf(dx, b1, fmaf(dy, b2, dz * b3));
-        const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
-                               __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
-        w[0] += P;
+      for (int k = 0; k < NW; ++k) w[k] = 0.0f;
+      const int s1 = min(s0 + kCSub, i1);
+      for (int i = s0; i < s1; ++i) {
+        const Tgt t = tn;
+        tn = T[min(i + 1, i1 - 1)];
+        if constexpr (D == 3) {
+          const float dx = (t.n[0] - mh[0]) - ml[0];
+          const float dy = (t.n[1] - mh[1]) - ml[1];
+          const float dz = (t.n[2] - mh[2]) - ml[2];
+          // mal1 = d^T C d, mal2 = d^T B d
+          const float c1 = fmaf(C[0], dx, fmaf(C[1], dy, C[2] * dz));
+          const float c2 = fmaf(C[1], dx, fmaf(C[3], dy, C[4] * dz));
+          const float c3 = fmaf(C[2], dx, fmaf(C[4], dy, C[5] * dz));
+          const float mal1 = fmaf(dx, c1, fmaf(dy, c2, dz * c3));
+          const float b1 = fmaf(t.b[0], dx, fmaf(t.b[1], dy, t.b[2] * dz));
+          const float b2 = fmaf(t.b[1], dx, fmaf(t.b[3], dy, t.b[4] * dz));
+          const float b3 = fmaf(t.b[2], dx, fmaf(t.b[4], dy, t.b[5] * dz));
+          const float mal2 = fmaf(dx, b1, fmaf(dy, b2, dz * b3));
+          const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
+                                 __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
+          w[0] += P;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
+          for (int a = 0; a < 3; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
 #pragma unroll
-        for (int a = 0; a < 6; ++a) w[4 + a] = fmaf(P, t.nn[a], w[4 + a]);
-        w[10] = fmaf(P, t.s, w[10]);
+          for (int a = 0; a < 6; ++a) w[4 + a] = fmaf(P, t.nn[a], w[4 + a]);
+          w[10] = fmaf(P, t.s, w[10]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) w[11 + a] = fmaf(P, t.v[a], w[11 + a]);
+          for (int a = 0; a < 3; ++a) w[11 + a] = fmaf(P, t.v[a], w[11 + a]);
 #pragma unroll
-        for (int a = 0; a < 6; ++a) w[14 + a] = fmaf(P, t.b[a], w[14 + a]);
-      } else {
-        const TgtRec2 t = reinterpret_cast<const TgtRec2 *>(tgtv)[i];
-        const float dx = (t.n[0] - mh[0]) - ml[0];
-        const float dy = (t.n[1] - mh[1]) - ml[1];
-        const float mal1 = fmaf(dx, fmaf(C[0], dx, C[1] * dy), dy * fmaf(C[1], dx, C[2] * dy));
-        const float mal2 = fmaf(dx, fmaf(t.b[0], dx, t.b[1] * dy), dy * fmaf(t.b[1], dx, t.b[2] * dy));
-        const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
-                               __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
-        w[0] += P;
+          for (int a = 0; a < 6; ++a) w[14 + a] = fmaf(P, t.b[a], w[14 + a]);
+        } else {
+          const float dx = (t.n[0] - mh[0]) - ml[0];
+          const float dy = (t.n[1] - mh[1]) - ml[1];
+          const float mal1 = fmaf(dx, fmaf(C[0], dx, C[1] * dy), dy * fmaf(C[1], dx, C[2] * dy));
+          const float mal2 = fmaf(dx, fmaf(t.b[0], dx, t.b[1] * dy), dy * fmaf(t.b[1], dx, t.b[2] * dy));
+          const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
+                                 __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
+          w[0] += P;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
+          for (int a = 0; a < 2; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) w[3 + a] = fmaf(P, t.nn[a], w[3 + a]);
-        w[6] = fmaf(P, t.s, w[6]);
+          for (int a = 0; a < 3; ++a) w[3 + a] = fmaf(P, t.nn[a], w[3 + a]);
+          w[6] = fmaf(P, t.s, w[6]);
 #pragma unroll
-        for (int a = 0; a < 2; ++a) w[7 + a] = fmaf(P, t.v[a], w[7 + a]);
+          for (int a = 0; a < 2; ++a) w[7 + a] = fmaf(P, t.v[a], w[7 + a]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) w[9 + a] = fmaf(P, t.b[a], w[9 + a]);
+          for (int a = 0; a < 3; ++a) w[9 + a] = fmaf(P, t.b[a], w[9 + a]);
+        }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < NW; ++k) W[k] += (double)w[k];
+      for (int k = 0; k < NW; ++k) W[k] += (double)w[k];
+    }
   }
-  if (j < M) {
+  __shared__ double red[2][NW][64];
+  if (wave >= 2)
+#pragma unroll
+    for (int k = 0; k < NW; ++k) red[wave - 2][k][lane] = W[k];
+  __syncthreads();
+  if (wave < 2)
+#pragma unroll
+    for (int k = 0; k < NW; ++k) W[k] += red[wave][k][lane];
+  __syncthreads();
+  if (wave == 1)
+#pragma unroll
+    for (int k = 0; k < NW; ++k) red[0][k][lane] = W[k];
+  __syncthreads();
+  if (wave == 0 && j < M) {
     double *out = part + ((size_t)blockIdx.y * M + j) * NW;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) out[k] = W[k];
+    for (int k = 0; k < NW; ++k) out[k] = W[k] + red[0][k][lane];
   }
-}
-
-// Sum the W partials over target chunks (fixed order), times gY_j, into the
-// feature rows.
-template <int D>
-__global__ __launch_bounds__(kCBlock) void k_compi_wsum(
-    const double *__restrict__ part, int M, int nchunks,
-    const float *__restrict__ src, double *__restrict__ feat) {
-  constexpr int NW = D == 3 ? 20 : 12;
-  constexpr int SS = D == 3 ? 16 : 8;
-  constexpr int GI = D == 3 ? 12 : 7;  // gY position in the source record
-  const size_t idx = (size_t)blockIdx.x * kCBlock + threadIdx.x;
-  if (idx >= (size_t)M * NW) return;
-  const size_t j = idx / NW, k = idx % NW;
-  double s = 0.0;
-  for (int c = 0; c < nchunks; ++c) s += part[((size_t)c * M + j) * NW + k];
-  feat[j * kFeat + k] = s * (double)src[j * SS + GI];
 }
 
 // Moments: mom[k] = sum_j prod of up to four feature entries of source j
-// (index -1 = 1).  One thread per moment, sources in chunks per workgroup;
-// per-workgroup partials summed later in a fixed order.
+// (index -1 = 1).  One workgroup per kMomSrc sources: their features (the W
+// partials summed over the target chunks in a fixed order, times gY_j, plus
+// m' and A' from k_compi_src) are staged in LDS, then each thread
+// accumulates its moments over them.  Block partials are summed later.
 struct MomTab {
   short ix[400][4];
 };
 
-__global__ __launch_bounds__(kCBlock) void k_compi_moments(
-    const double *__restrict__ feat, int M, int chunk, int nmom, MomTab tab,
-    double *__restrict__ part) {
-  const int j0 = blockIdx.x * chunk;
-  const int j1 = min(j0 + chunk, M);
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_mom(
+    const double *__restrict__ wpart, int M, int nchunks,
+    const float *__restrict__ src, const double *__restrict__ feat, int nmom,
+    MomTab tab, double *__restrict__ mpart) {
+  constexpr int NW = D == 3 ? 20 : 12;
+  constexpr int SS = D == 3 ? 16 : 8;
+  constexpr int GI = D == 3 ? 12 : 7;  // gY position in the source record
+  __shared__ double fs[kMomSrc][kFeat];
+  const int j0 = blockIdx.x * kMomSrc;
+  for (int e = threadIdx.x; e < kMomSrc * kFeat; e += kCBlock) {
+    const int jl = e / kFeat, k = e % kFeat, j = j0 + jl;
+    double v = 0.0;
+    if (j < M) {
+      if (k < NW) {
+        const double *p = wpart + (size_t)j * NW + k;
+        const size_t stride = (size_t)M * NW;
+        double s0 = 0.0, s1 = 0.0;
+        int c = 0;
+        for (; c + 1 < nchunks; c += 2) {
+          s0 += p[(size_t)c * stride];
+          s1 += p[(size_t)(c + 1) * stride];
+        }
+        if (c < nchunks) s0 += p[(size_t)c * stride];
+        v = (s0 + s1) * (double)src[(size_t)j * SS + GI];
+      } else if (k < NW + D + D * D) {  // m' and A'
+        v = feat[(size_t)j * kFeat + k];
+      }
+    }
+    fs[jl][k] = v;
+  }
+  __syncthreads();
   for (int k = threadIdx.x; k < nmom; k += kCBlock) {
     const int a = tab.ix[k][0], b = tab.ix[k][1], c = tab.ix[k][2], d = tab.ix[k][3];
     double s = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const double *f = feat + (size_t)j * kFeat;
-      double v = f[a];
-      if (b >= 0) v *= f[b];
-      if (c >= 0) v *= f[c];
-      if (d >= 0) v *= f[d];
+    for (int jl = 0; jl < kMomSrc; ++jl) {
+      double v = fs[jl][a];
+      if (b >= 0) v *= fs[jl][b];
+      if (c >= 0) v *= fs[jl][c];
+      if (d >= 0) v *= fs[jl][d];
       s += v;
     }
-    part[(size_t)blockIdx.x * nmom + k] = s;
+    mpart[(size_t)blockIdx.x * nmom + k] = s;
   }
 }
 
+// One wave per moment: lanes take every 64th block partial, then a fixed-
+// order wave sum.
 __global__ __launch_bounds__(kCBlock) void k_compi_msum(
     const double *__restrict__ part, int nblocks, int nmom, double *__restrict__ out) {
-  const int k = blockIdx.x * kCBlock + threadIdx.x;
-  if (k >= nmom) return;
+  const int k = blockIdx.x * kCWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= nmom) return;  // uniform per wave
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nmom + k];
-  out[k] = s;
+  for (int b = lane; b < nblocks; b += 64) s += part[(size_t)b * nmom + k];
+  s = wave_sum(s);
+  if (lane == 0) out[k] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -520,6 +570,56 @@ static void expand(int D, const std::vector<Mom> &moms, const double *val, Poly 
   }
 }
 
+// The moment -> coefficient map is linear and depends only on D: coef[k] =
+// sum over (k, moment, factor) entries of factor * moment.  Built once per D
+// by expanding each moment alone with value 1; entries are ordered by
+// coefficient, then moment, so each coefficient sums its moments in order.
+struct ExpMap {
+  std::vector<int> start;  // CSR over coefficients (monomial order)
+  std::vector<int> mom;
+  std::vector<double> fac;
+};
+
+static void build_map(int D, ExpMap &E) {
+  const std::vector<Mom> moms = moment_list(D);
+  const int nv = D == 3 ? 12 : 4;
+  const int nm = D == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
+  Poly P(nv);
+  std::vector<int> key(nm);
+  for (int k = 0; k < nm; ++k) {
+    const unsigned char *e = D == 3 ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
+    int v[4], deg = 0;
+    for (int x = 0; x < nv; ++x)
+      for (int r = 0; r < e[x]; ++r) v[deg++] = x;
+    key[k] = P.key(v, deg);
+  }
+  std::vector<std::vector<std::pair<int, double>>> rows(nm);
+  std::vector<double> one(moms.size(), 0.0);
+  for (size_t m = 0; m < moms.size(); ++m) {
+    one[m] = 1.0;
+    std::fill(P.c.begin(), P.c.end(), 0.0);
+    expand(D, moms, one.data(), P);
+    one[m] = 0.0;
+    for (int k = 0; k < nm; ++k)
+      if (P.c[key[k]] != 0.0) rows[k].push_back({(int)m, P.c[key[k]]});
+  }
+  E.start.assign(nm + 1, 0);
+  for (int k = 0; k < nm; ++k) {
+    E.start[k + 1] = E.start[k] + (int)rows[k].size();
+    for (const auto &r : rows[k]) {
+      E.mom.push_back(r.first);
+      E.fac.push_back(r.second);
+    }
+  }
+}
+
+static const ExpMap &expansion_map(int D) {
+  static ExpMap maps[2];
+  static std::once_flag once[2];
+  std::call_once(once[D - 2], [D] { build_map(D, maps[D - 2]); });
+  return maps[D - 2];
+}
+
 static int check_dims(dugma_ctx *c, double N, double M, double D) {
   if ((int)N != c->N || (int)M != c->M || (int)D != c->D)
     return fail(DUGMA_ERR_STATE, "N, M, D do not match the context");
@@ -559,14 +659,14 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   const int NW = Dd == 3 ? 20 : 12;
   const size_t trec = Dd == 3 ? sizeof(TgtRec3) : sizeof(TgtRec2);
   const int SS = Dd == 3 ? 16 : 8;
-  // launch shapes
-  const int sblocks = (m + kCBlock - 1) / kCBlock;
-  int nch = (8192 + sblocks * 4 - 1) / (sblocks * 4);
-  nch = std::max(1, std::min(nch, std::min(64, (n + kCSub - 1) / kCSub)));
+  // launch shapes: 64 sources per workgroup of k_compi_w, target chunks so
+  // that about 2,048 workgroups run, each wave with at least 64 targets
+  const int sblocks = (m + 63) / 64;
+  int nch = (2048 + sblocks - 1) / sblocks;
+  nch = std::max(1, std::min(nch, std::max(1, n / (kCWaves * 64))));
   const int chunk = (n + nch - 1) / nch;
   const int nchunks = (n + chunk - 1) / chunk;
-  const int mchunk = std::max(64, (m + 1023) / 1024);
-  const int mblocks = (m + mchunk - 1) / mchunk;
+  const int mblocks = (m + kMomSrc - 1) / kMomSrc;
   // scratch: tgt records | src | feat | W partials | moment partials | moments
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t o_src = al(trec * n);
@@ -587,36 +687,32 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   memset(&tab, 0xff, sizeof(tab));
   for (int k = 0; k < nmom; ++k)
     for (int q = 0; q < 4; ++q) tab.ix[k][q] = moms[k].ix[q];
+  const ExpMap &E = expansion_map(Dd);
 
   dim3 blk(kCBlock);
+  const unsigned sgrid = (unsigned)((m + kCBlock - 1) / kCBlock);
   if (Dd == 3) {
     hipLaunchKernelGGL(k_compi_tgt<3>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
-    hipLaunchKernelGGL(k_compi_src<3>, dim3(sblocks), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_src<3>, dim3(sgrid), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
     hipLaunchKernelGGL(k_compi_w<3>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
-    hipLaunchKernelGGL(k_compi_wsum<3>, dim3((unsigned)(((size_t)m * NW + kCBlock - 1) / kCBlock)), blk, 0, c->stream, wpart, m, nchunks, src, feat);
+    hipLaunchKernelGGL(k_compi_mom<3>, dim3(mblocks), blk, 0, c->stream, wpart, m, nchunks, src, feat, nmom, tab, mpart);
   } else {
     hipLaunchKernelGGL(k_compi_tgt<2>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
-    hipLaunchKernelGGL(k_compi_src<2>, dim3(sblocks), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_src<2>, dim3(sgrid), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
     hipLaunchKernelGGL(k_compi_w<2>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
-    hipLaunchKernelGGL(k_compi_wsum<2>, dim3((unsigned)(((size_t)m * NW + kCBlock - 1) / kCBlock)), blk, 0, c->stream, wpart, m, nchunks, src, feat);
+    hipLaunchKernelGGL(k_compi_mom<2>, dim3(mblocks), blk, 0, c->stream, wpart, m, nchunks, src, feat, nmom, tab, mpart);
   }
-  hipLaunchKernelGGL(k_compi_moments, dim3(mblocks), blk, 0, c->stream, feat, m, mchunk, nmom, tab, mpart);
-  hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCBlock - 1) / kCBlock), blk, 0, c->stream, mpart, mblocks, nmom, mom);
+  hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCWaves - 1) / kCWaves), blk, 0, c->stream, mpart, mblocks, nmom, mom);
   DUGMA_HIP(hipGetLastError());
-  std::vector<double> hm(nmom);
-  DUGMA_HIP(hipMemcpyAsync(hm.data(), mom, sizeof(double) * nmom, hipMemcpyDeviceToHost, c->stream));
+  double *hm = c->hpinned;  // pinned staging (nmom <= kHostStage / 2)
+  DUGMA_HIP(hipMemcpyAsync(hm, mom, sizeof(double) * nmom, hipMemcpyDeviceToHost, c->stream));
   DUGMA_HIP(hipStreamSynchronize(c->stream));
 
-  const int nv = Dd == 3 ? 12 : 4;
-  Poly P(nv);
-  expand(Dd, moms, hm.data(), P);
   const int nm = Dd == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
   for (int k = 0; k < nm; ++k) {
-    const unsigned char *e = Dd == 3 ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
-    int v[4], deg = 0;
-    for (int x = 0; x < nv; ++x)
-      for (int r = 0; r < e[x]; ++r) v[deg++] = x;
-    coef[k] = P.c[P.key(v, deg)];
+    double v = 0.0;
+    for (int e = E.start[k]; e < E.start[k + 1]; ++e) v += E.fac[e] * hm[E.mom[e]];
+    coef[k] = v;
   }
   return DUGMA_OK;
 }
