@@ -36,9 +36,9 @@ FLOP_3D_ISO_FWD = 17
 # full-covariance workload, keyed by launch shape (lane_points, target_waves):
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by what the
 # counter reads for a 256 MiB copy (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
-MEASURED_TRAFFIC_6K_FULL = {(1, 4096): 6673000, (1, 8192): 6739000, (1, 16384): 6752000,
-                            (1, 32768): 6752000, (2, 4096): 7087000, (2, 8192): 7093000,
-                            (2, 16384): 7093000, (2, 32768): 7093000}
+MEASURED_TRAFFIC_6K_FULL = {(1, 4096): 6688000, (1, 8192): 6754000, (1, 16384): 6766000,
+                            (1, 32768): 6766000, (2, 4096): 7101000, (2, 8192): 7108000,
+                            (2, 16384): 7108000, (2, 32768): 7108000}
 
 
 class Hip:

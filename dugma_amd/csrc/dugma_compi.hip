@@ -322,16 +322,21 @@ __global__ __launch_bounds__(kCBlock) void k_compi_mom(
     double v = 0.0;
     if (j < M) {
       if (k < NW) {
+        // eight loads in flight, summed in a fixed order
         const double *p = wpart + (size_t)j * NW + k;
         const size_t stride = (size_t)M * NW;
-        double s0 = 0.0, s1 = 0.0;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         int c = 0;
-        for (; c + 1 < nchunks; c += 2) {
-          s0 += p[(size_t)c * stride];
-          s1 += p[(size_t)(c + 1) * stride];
+        for (; c + 8 <= nchunks; c += 8) {
+          double x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = p[(size_t)(c + u) * stride];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[u] += x[u];
         }
-        if (c < nchunks) s0 += p[(size_t)c * stride];
-        v = (s0 + s1) * (double)src[(size_t)j * SS + GI];
+        for (; c < nchunks; ++c) acc[0] += p[(size_t)c * stride];
+        const double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        v = sum * (double)src[(size_t)j * SS + GI];
       } else if (k < NW + D + D * D) {  // m' and A'
         v = feat[(size_t)j * kFeat + k];
       }
