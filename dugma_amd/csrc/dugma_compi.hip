@@ -71,11 +71,8 @@ __host__ __device__ constexpr int sidx(int a, int b) { return D == 3 ? sidx3(a, 
 
 // Per-call target records.  B = fl32(inv(CX) * Isig) (compi.cu:655-658).
 template <int D>
-__global__ __launch_bounds__(kCBlock) void k_compi_tgt(
-    const float *__restrict__ X, const float *__restrict__ XCinv,
-    const float *__restrict__ gX, int N, CompiPose ps, void *__restrict__ outv) {
-  const int i = blockIdx.x * kCBlock + threadIdx.x;
-  if (i >= N) return;
+__device__ void compi_tgt(int i, const float *__restrict__ X, const float *__restrict__ XCinv,
+                          const float *__restrict__ gX, const CompiPose &ps, void *__restrict__ outv) {
   float B[9];
 #pragma unroll
   for (int q = 0; q < D * D; ++q) B[q] = XCinv[(size_t)i * D * D + q] * ps.Isig;
@@ -132,12 +129,9 @@ __global__ __launch_bounds__(kCBlock) void k_compi_tgt(
 //                          2-D [mh 2, ml 2, C sym 3, gY] = 8
 // feat (doubles per source, kFeat stride): [.. W set later .., m' at FM, A' at FA]
 template <int D>
-__global__ __launch_bounds__(kCBlock) void k_compi_src(
-    const float *__restrict__ Y, const float *__restrict__ YCinv,
-    const float *__restrict__ gY, int M, CompiPose ps, float *__restrict__ src,
-    double *__restrict__ feat) {
-  const int j = blockIdx.x * kCBlock + threadIdx.x;
-  if (j >= M) return;
+__device__ void compi_src(int j, const float *__restrict__ Y, const float *__restrict__ YCinv,
+                          const float *__restrict__ gY, const CompiPose &ps, float *__restrict__ src,
+                          double *__restrict__ feat) {
   constexpr int SS = D == 3 ? 16 : 8;
   constexpr int FM = D == 3 ? 20 : 12;
   constexpr int FA = FM + D;
@@ -185,6 +179,18 @@ __global__ __launch_bounds__(kCBlock) void k_compi_src(
   for (; k < SS; ++k) o[k] = 0.0f;
 #pragma unroll
   for (int q = 0; q < D * D; ++q) f[FA + q] = (double)(float)C[q];
+}
+
+// Both per-call preparations in one launch: thread i prepares target i
+// (i < N) and source i (i < M).
+template <int D>
+__global__ __launch_bounds__(kCBlock) void k_compi_prep(
+    const float *__restrict__ X, const float *__restrict__ XCinv, const float *__restrict__ gX, int N,
+    const float *__restrict__ Y, const float *__restrict__ YCinv, const float *__restrict__ gY, int M,
+    CompiPose ps, void *__restrict__ tgt, float *__restrict__ src, double *__restrict__ feat) {
+  const int i = blockIdx.x * kCBlock + threadIdx.x;
+  if (i < N) compi_tgt<D>(i, X, XCinv, gX, ps, tgt);
+  if (i < M) compi_src<D>(i, Y, YCinv, gY, ps, src, feat);
 }
 
 // W_j = sum_i P_ij G_i.  Lane = source j (stationary, 64 per workgroup); the
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
 // Moments: mom[k] = sum_j prod of up to four feature entries of source j
 // (index -1 = 1).  One workgroup per kMomSrc sources: their features (the W
 // partials summed over the target chunks in a fixed order, times gY_j, plus
-// m' and A' from k_compi_src) are staged in LDS, then each thread
+// m' and A' from k_compi_prep) are staged in LDS, then each thread
 // accumulates its moments over them.  Block partials are summed later.
 struct MomTab {
   short ix[400][4];
@@ -687,7 +693,6 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   double *feat = (double *)(base + o_feat);
   double *wpart = (double *)(base + o_part);
   double *mpart = (double *)(base + o_mp);
-  double *mom = (double *)(base + o_mom);
   MomTab tab;
   memset(&tab, 0xff, sizeof(tab));
   for (int k = 0; k < nmom; ++k)
@@ -695,23 +700,24 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   const ExpMap &E = expansion_map(Dd);
 
   dim3 blk(kCBlock);
-  const unsigned sgrid = (unsigned)((m + kCBlock - 1) / kCBlock);
+  const unsigned pgrid = (unsigned)((std::max(n, m) + kCBlock - 1) / kCBlock);
   if (Dd == 3) {
-    hipLaunchKernelGGL(k_compi_tgt<3>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
-    hipLaunchKernelGGL(k_compi_src<3>, dim3(sgrid), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_prep<3>, dim3(pgrid), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, c->dY,
+                       c->dYCinv, c->dgY, m, ps, tgt, src, feat);
     hipLaunchKernelGGL(k_compi_w<3>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
     hipLaunchKernelGGL(k_compi_mom<3>, dim3(mblocks), blk, 0, c->stream, wpart, m, nchunks, src, feat, nmom, tab, mpart);
   } else {
-    hipLaunchKernelGGL(k_compi_tgt<2>, dim3((n + kCBlock - 1) / kCBlock), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, ps, tgt);
-    hipLaunchKernelGGL(k_compi_src<2>, dim3(sgrid), blk, 0, c->stream, c->dY, c->dYCinv, c->dgY, m, ps, src, feat);
+    hipLaunchKernelGGL(k_compi_prep<2>, dim3(pgrid), blk, 0, c->stream, c->dX, c->dXCinv, c->dgX, n, c->dY,
+                       c->dYCinv, c->dgY, m, ps, tgt, src, feat);
     hipLaunchKernelGGL(k_compi_w<2>, dim3(sblocks, nchunks), blk, 0, c->stream, src, m, tgt, n, chunk, wpart);
     hipLaunchKernelGGL(k_compi_mom<2>, dim3(mblocks), blk, 0, c->stream, wpart, m, nchunks, src, feat, nmom, tab, mpart);
   }
-  hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCWaves - 1) / kCWaves), blk, 0, c->stream, mpart, mblocks, nmom, mom);
+  // moment sums straight into the context's pinned result slots
+  hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCWaves - 1) / kCWaves), blk, 0, c->stream, mpart, mblocks, nmom,
+                     c->dres + dugma_ctx::kResMoments);
   DUGMA_HIP(hipGetLastError());
-  double *hm = c->hpinned;  // pinned staging (nmom <= kHostStage / 2)
-  DUGMA_HIP(hipMemcpyAsync(hm, mom, sizeof(double) * nmom, hipMemcpyDeviceToHost, c->stream));
   DUGMA_HIP(hipStreamSynchronize(c->stream));
+  const volatile double *hm = c->hres + dugma_ctx::kResMoments;
 
   const int nm = Dd == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
   for (int k = 0; k < nm; ++k) {

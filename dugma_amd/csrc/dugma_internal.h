@@ -113,8 +113,12 @@ struct dugma_ctx {
   // so that covariances are O(1) in FP32 whatever the caller's units.
   int lscale = 0;
   double scale = 1.0;
-  double *hres = nullptr;        // pinned host slot the reduction writes its result to
+  // pinned host slots kernels write their final results to (read after the
+  // stream sync): [0, 8) pose-pass sums, [8] the update_sig value (as a
+  // float), [kResMoments, kResMoments + 400) the compi moments
+  double *hres = nullptr;
   double *dres = nullptr;        // device address of hres
+  static constexpr int kResSig = 8, kResMoments = 64;
 
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc

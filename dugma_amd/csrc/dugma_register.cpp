@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "dugma_internal.h"
@@ -25,32 +27,72 @@ static const unsigned char *mono(int D, int k) {
   return D == 3 ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
 }
 
-// F = sum_k coef_k prod_v dT_v^e_kv and, if grad, dF/ddT.
-static double poly_value(int D, const double *coef, const double *dT, double *grad) {
-  int nm, nv;
-  mono_table(D, &nm, &nv);
-  double pw[12][5];
-  for (int v = 0; v < nv; ++v) {
-    pw[v][0] = 1.0;
-    for (int e = 1; e < 5; ++e) pw[v][e] = pw[v][e - 1] * dT[v];
-  }
+// Monomials as lists of at most four variable factors (with repetition),
+// built once per D from the exponent table.
+struct MonoList {
+  int nm = 0, nv = 0;
+  std::vector<int> deg;
+  std::vector<std::array<int, 4>> var;
+};
+
+static const MonoList &mono_list(int D) {
+  static MonoList lists[2];
+  static std::once_flag once[2];
+  std::call_once(once[D - 2], [D] {
+    MonoList &L = lists[D - 2];
+    mono_table(D, &L.nm, &L.nv);
+    L.deg.resize(L.nm);
+    L.var.resize(L.nm);
+    for (int k = 0; k < L.nm; ++k) {
+      const unsigned char *e = mono(D, k);
+      int d = 0;
+      std::array<int, 4> v{0, 0, 0, 0};
+      for (int x = 0; x < L.nv; ++x)
+        for (int r = 0; r < e[x] && d < 4; ++r) v[d++] = x;
+      L.deg[k] = d;
+      L.var[k] = v;
+    }
+  });
+  return lists[D - 2];
+}
+
+// F = sum_k coef_k prod_v dT_v^e_kv and, if asked, dF/ddT (nv) and the full
+// Hessian d2F/ddT2 (nv x nv, row-major).  Per monomial the products leave
+// out one factor (gradient) or two (Hessian); a repeated variable is counted
+// once per factor position, which gives the powers' derivatives.
+static double poly_value(int D, const double *coef, const double *dT, double *grad,
+                         double *hess = nullptr) {
+  const MonoList &L = mono_list(D);
+  const int nv = L.nv;
   if (grad)
     for (int v = 0; v < nv; ++v) grad[v] = 0.0;
+  if (hess)
+    for (int v = 0; v < nv * nv; ++v) hess[v] = 0.0;
   double F = 0.0;
-  for (int k = 0; k < nm; ++k) {
-    const unsigned char *e = mono(D, k);
-    double m = coef[k];
-    for (int v = 0; v < nv; ++v) m *= pw[v][e[v]];
-    F += m;
-    if (grad) {
-      for (int v = 0; v < nv; ++v) {
-        if (!e[v]) continue;
-        double g = coef[k] * e[v] * pw[v][e[v] - 1];
-        for (int u = 0; u < nv; ++u)
-          if (u != v) g *= pw[u][e[u]];
-        grad[v] += g;
+  for (int k = 0; k < L.nm; ++k) {
+    const double c = coef[k];
+    if (c == 0.0) continue;
+    const int d = L.deg[k];
+    const std::array<int, 4> &v = L.var[k];
+    double f[4] = {1.0, 1.0, 1.0, 1.0};
+    for (int q = 0; q < d; ++q) f[q] = dT[v[q]];
+    F += c * ((f[0] * f[1]) * (f[2] * f[3]));
+    if (grad)
+      for (int q = 0; q < d; ++q) {
+        double m = c;
+        for (int r = 0; r < d; ++r)
+          if (r != q) m *= f[r];
+        grad[v[q]] += m;
       }
-    }
+    if (hess)
+      for (int q = 0; q < d; ++q)
+        for (int r = q + 1; r < d; ++r) {
+          double m = c;
+          for (int t = 0; t < d; ++t)
+            if (t != q && t != r) m *= f[t];
+          hess[v[q] * nv + v[r]] += m;
+          hess[v[r] * nv + v[q]] += m;
+        }
   }
   return F;
 }
@@ -109,67 +151,99 @@ static void apply(int D, const double *dR0, const double *dt0, const double *p,
   }
 }
 
-// Objective on the constraint set (dR a rotation) as a function of p, with
-// its gradient in p from the polynomial gradient by the chain rule.
+// Objective on the constraint set (dR a rotation) as a function of the
+// parameters p of a left increment (3-D: w, dt; 2-D: theta, dt) of (dR, dt).
 static double obj_p(int D, const double *coef, const double *dR0, const double *dt0,
-                    const double *p, double *gp) {
-  double dR[9], dt[3], dT[12], g[12];
+                    const double *p) {
+  double dR[9], dt[3], dT[12];
   apply(D, dR0, dt0, p, dR, dt);
   pack_dT(D, dR, dt, dT);
-  const double F = poly_value(D, coef, dT, gp ? g : nullptr);
-  if (!gp) return F;
+  return poly_value(D, coef, dT, nullptr);
+}
+
+// Value, gradient and Hessian in p at p = 0, from the polynomial's own
+// gradient g and Hessian H in dT by the chain rule: with J = d dT / d p,
+// grad_p = J^T g and hess_p = J^T H J + sum_v g_v d2 dT_v / dp dp.
+// 3-D: d dR / d w_a = [e_a]x dR, d2 dR / d w_a d w_b = (K_a K_b + K_b K_a) dR / 2;
+// 2-D: dR = Rot(theta) dR0, so d (r11, r21) / d theta = (-r21, r11) and the
+// second derivative is (-r11, -r21).
+static double obj_p_newton(int D, const double *coef, const double *dR, const double *dt, double *gp,
+                           double *Hp) {
+  const int nv = D == 3 ? 12 : 4, np = D == 3 ? 6 : 3;
+  double dT[12], g[12], H[144], J[12][6];
+  pack_dT(D, dR, dt, dT);
+  const double F = poly_value(D, coef, dT, g, H);
+  for (int v = 0; v < nv; ++v)
+    for (int a = 0; a < np; ++a) J[v][a] = 0.0;
+  double S2[3][3][9];  // second derivatives of dR (column-major) in w_a, w_b
   if (D == 3) {
-    // d dR / d w_a at p (left perturbation): [e_a]x dR
+    double K[3][9], KR[3][9];
     for (int a = 0; a < 3; ++a) {
       double ea[3] = {0, 0, 0};
       ea[a] = 1.0;
-      const double K[9] = {0, ea[2], -ea[1], -ea[2], 0, ea[0], ea[1], -ea[0], 0};
-      double KR[9];
-      matmul(3, K, dR, KR);
-      double s = 0;
+      const double Ka[9] = {0, ea[2], -ea[1], -ea[2], 0, ea[0], ea[1], -ea[0], 0};
+      memcpy(K[a], Ka, sizeof(Ka));
+      matmul(3, K[a], dR, KR[a]);
       for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) s += g[3 + 3 * r + c] * KR[r + 3 * c];
-      gp[a] = s;
+        for (int c = 0; c < 3; ++c) J[3 + 3 * r + c][a] = KR[a][r + 3 * c];
+      J[a][3 + a] = 1.0;
     }
-    for (int a = 0; a < 3; ++a) gp[3 + a] = g[a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        double KaKbR[9], KbKaR[9];
+        matmul(3, K[a], KR[b], KaKbR);
+        matmul(3, K[b], KR[a], KbKaR);
+        for (int q = 0; q < 9; ++q) S2[a][b][q] = 0.5 * (KaKbR[q] + KbKaR[q]);
+      }
   } else {
-    // dR = [[r11, -r21], [r21, r11]]; d/dtheta of Rot(theta) dR0 = J dR
-    gp[0] = g[2] * (-dR[1]) + g[3] * dR[0];
-    gp[1] = g[0];
-    gp[2] = g[1];
+    J[2][0] = -dR[1];
+    J[3][0] = dR[0];
+    J[0][1] = 1.0;
+    J[1][2] = 1.0;
+  }
+  for (int a = 0; a < np; ++a) {
+    double s = 0.0;
+    for (int v = 0; v < nv; ++v) s += J[v][a] * g[v];
+    gp[a] = s;
+  }
+  for (int a = 0; a < np; ++a)
+    for (int b = 0; b < np; ++b) {
+      double s = 0.0;
+      for (int v = 0; v < nv; ++v) {
+        if (J[v][a] == 0.0) continue;
+        double hv = 0.0;
+        for (int u = 0; u < nv; ++u) hv += H[v * nv + u] * J[u][b];
+        s += J[v][a] * hv;
+      }
+      Hp[a * np + b] = s;
+    }
+  if (D == 3) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        double s = 0.0;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) s += g[3 + 3 * r + c] * S2[a][b][r + 3 * c];
+        Hp[a * np + b] += s;
+      }
+  } else {
+    Hp[0] += g[2] * (-dR[0]) + g[3] * (-dR[1]);
   }
   return F;
 }
 
-// Damped Newton on p with the Hessian from central differences of the
-// analytic gradient; re-centred on the accepted iterate each step.
+// Damped Newton on p with the analytic Hessian, re-centred on the accepted
+// iterate each step.
 static int minimise(int D, const double *coef, double *dR, double *dt, double *Fout) {
   const int np = D == 3 ? 6 : 3;
   for (int a = 0; a < D * D; ++a) dR[a] = (a % (D + 1) == 0) ? 1.0 : 0.0;
   for (int a = 0; a < D; ++a) dt[a] = 0.0;
   double zero[6] = {0, 0, 0, 0, 0, 0};
-  double F = obj_p(D, coef, dR, dt, zero, nullptr);
+  double F = obj_p(D, coef, dR, dt, zero);
   if (!std::isfinite(F)) return -1;
   double lambda = 1e-3;
   for (int it = 0; it < 100; ++it) {
     double g[6], H[36];
-    obj_p(D, coef, dR, dt, zero, g);
-    double gn = 0;
-    for (int a = 0; a < np; ++a) gn += g[a] * g[a];
-    const double h = 1e-6;
-    for (int a = 0; a < np; ++a) {
-      double p[6] = {0, 0, 0, 0, 0, 0}, gpl[6], gmi[6];
-      p[a] = h;
-      obj_p(D, coef, dR, dt, p, gpl);
-      p[a] = -h;
-      obj_p(D, coef, dR, dt, p, gmi);
-      for (int b = 0; b < np; ++b) H[a * np + b] = (gpl[b] - gmi[b]) / (2 * h);
-    }
-    for (int a = 0; a < np; ++a)
-      for (int b = a + 1; b < np; ++b) {
-        const double m = 0.5 * (H[a * np + b] + H[b * np + a]);
-        H[a * np + b] = H[b * np + a] = m;
-      }
+    obj_p_newton(D, coef, dR, dt, g, H);
     bool accepted = false;
     for (int tries = 0; tries < 30 && !accepted; ++tries) {
       // solve (H + lambda (|diag| + 1e-12) I) s = -g by Gaussian elimination
@@ -198,7 +272,7 @@ static int minimise(int D, const double *coef, double *dR, double *dt, double *F
       if (ok) {
         double s[6];
         for (int a = 0; a < np; ++a) s[a] = A[a][np] / A[a][a];
-        const double Fn = obj_p(D, coef, dR, dt, s, nullptr);
+        const double Fn = obj_p(D, coef, dR, dt, s);
         if (std::isfinite(Fn) && Fn <= F) {
           double nR[9], nt[3];
           apply(D, dR, dt, s, nR, nt);
@@ -220,7 +294,6 @@ static int minimise(int D, const double *coef, double *dR, double *dt, double *F
       lambda *= 10.0;
     }
     if (!accepted) break;  // no decrease possible: at a (local) minimum
-    (void)gn;
   }
   *Fout = F;
   return 1;

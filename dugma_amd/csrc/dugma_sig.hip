@@ -22,10 +22,14 @@ struct PoseArgC {
 };
 
 // update_sig.cu:247-252: m_j = fl32(r_a1*y1 + r_a2*y2 (+ r_a3*y3) + t_a), FP64.
+// Threads below nfill also set fill[i] = fv (the start value of the
+// quantised minimum), saving a launch.
 template <int D>
 __global__ __launch_bounds__(kSBlock) void k_transform_compat(
-    const float *__restrict__ Y, int M, PoseArgC ps, float *__restrict__ out) {
+    const float *__restrict__ Y, int M, PoseArgC ps, float *__restrict__ out,
+    unsigned *__restrict__ fill, int nfill, unsigned fv) {
   const int j = blockIdx.x * kSBlock + threadIdx.x;
+  if (j < nfill) fill[j] = fv;
   if (j >= M) return;
   const float *y = Y + (size_t)j * D;
 #pragma unroll
@@ -36,12 +40,6 @@ __global__ __launch_bounds__(kSBlock) void k_transform_compat(
     s = s + ps.t[a];
     out[(size_t)j * D + a] = (float)s;
   }
-}
-
-__global__ __launch_bounds__(kSBlock) void k_fill_u32(unsigned *__restrict__ p,
-                                                      int n, unsigned v) {
-  const int i = blockIdx.x * kSBlock + threadIdx.x;
-  if (i < n) p[i] = v;
 }
 
 // __float2uint_rn: round half to even, saturating; NaN and negatives -> 0.
@@ -82,19 +80,16 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
   if (i < N) atomicMin(qmin + i, best);
 }
 
-// update_sig.cu:79-91: fl32(q) / 1e6 (IEEE division here).
-__global__ __launch_bounds__(kSBlock) void k_convert_back(
-    const unsigned *__restrict__ q, int n, float *__restrict__ f) {
-  const int i = blockIdx.x * kSBlock + threadIdx.x;
-  if (i < n) f[i] = ((float)q[i]) / 1000000.0f;
-}
-
-// update_sig.cu:291-307 with n_c = 1: pairwise tree, level by level; at each
-// level element 2k*stride absorbs element (2k+1)*stride, an odd last element
-// is carried.  One workgroup; returns sum / N (update_sig.cu:385).
-__global__ __launch_bounds__(1024) void k_tree_sum_div(float *__restrict__ a,
-                                                       int n,
+// update_sig.cu:79-91 then 291-307 with n_c = 1: a[i] = fl32(q[i]) / 1e6
+// (IEEE division), then the pairwise tree, level by level; at each level
+// element 2k*stride absorbs element (2k+1)*stride, an odd last element is
+// carried.  One workgroup; writes sum / N (update_sig.cu:385) to out (the
+// context's pinned result slot).
+__global__ __launch_bounds__(1024) void k_tree_sum_div(const unsigned *__restrict__ q,
+                                                       float *__restrict__ a, int n,
                                                        float *__restrict__ out) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) a[k] = ((float)q[k]) / 1000000.0f;
+  __syncthreads();
   long L = n, stride = 1;
   while (L > 1) {
     const long half = (L + 1) / 2;
@@ -210,16 +205,19 @@ static int chunks_for(int nq, int np) {
 }
 
 static int transform_compat(dugma_ctx *c, const double *R, const double *t,
-                            float *out) {
+                            float *out, unsigned *fill = nullptr, int nfill = 0,
+                            unsigned fv = 0u) {
   PoseArgC ps;
   memset(&ps, 0, sizeof(ps));
   for (int a = 0; a < c->D * c->D; ++a) ps.R[a] = R[a];
   for (int a = 0; a < c->D; ++a) ps.t[a] = t[a];
-  dim3 grid((c->M + kSBlock - 1) / kSBlock);
+  dim3 grid((std::max(c->M, nfill) + kSBlock - 1) / kSBlock);
   if (c->D == 3)
-    hipLaunchKernelGGL(k_transform_compat<3>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out);
+    hipLaunchKernelGGL(k_transform_compat<3>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out, fill,
+                       nfill, fv);
   else
-    hipLaunchKernelGGL(k_transform_compat<2>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out);
+    hipLaunchKernelGGL(k_transform_compat<2>, grid, dim3(kSBlock), 0, c->stream, c->dY, c->M, ps, out, fill,
+                       nfill, fv);
   DUGMA_HIP(hipGetLastError());
   return DUGMA_OK;
 }
@@ -240,20 +238,18 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
   int rc = check_dims(ctx, N, M, D);
   if (rc) return rc;
   dugma_ctx *c = ctx;
-  // scratch: m (M*D floats) | q (N u32) | f (N floats) | result
+  // scratch: m (M*D floats) | q (N u32) | f (N floats)
   const size_t off_q = ((size_t)c->M * c->D * 4 + 255) & ~(size_t)255;
   const size_t off_f = off_q + (((size_t)c->N * 4 + 255) & ~(size_t)255);
-  const size_t off_r = off_f + (((size_t)c->N * 4 + 255) & ~(size_t)255);
-  rc = c->scratch2.ensure(off_r + 256);
+  rc = c->scratch2.ensure(off_f + (((size_t)c->N * 4 + 255) & ~(size_t)255));
   if (rc) return rc;
   char *base = (char *)c->scratch2.ptr;
   float *m = (float *)base;
   unsigned *q = (unsigned *)(base + off_q);
   float *f = (float *)(base + off_f);
-  float *res = (float *)(base + off_r);
-  rc = transform_compat(c, R, t, m);
+  // transform, with q = 4e9 (update_sig.cu:13) set in the same launch
+  rc = transform_compat(c, R, t, m, q, c->N, 4000000000u);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fill_u32, dim3((c->N + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, c->stream, q, c->N, 4000000000u);
   const int nch = chunks_for(c->N, c->M);
   const int chunk = (c->M + nch - 1) / nch;
   const int nchunks = (c->M + chunk - 1) / chunk;
@@ -262,14 +258,12 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
     hipLaunchKernelGGL(k_nn1_quant<3>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
   else
     hipLaunchKernelGGL(k_nn1_quant<2>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
-  hipLaunchKernelGGL(k_convert_back, dim3((c->N + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, c->stream, q, c->N, f);
-  hipLaunchKernelGGL(k_tree_sum_div, dim3(1), dim3(1024), 0, c->stream, f, c->N, res);
+  hipLaunchKernelGGL(k_tree_sum_div, dim3(1), dim3(1024), 0, c->stream, q, f, c->N,
+                     (float *)(c->dres + dugma_ctx::kResSig));
   DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipMemcpyAsync(c->hpinned, res, sizeof(float), hipMemcpyDeviceToHost, c->stream));
   DUGMA_HIP(hipStreamSynchronize(c->stream));
-  float s;
-  memcpy(&s, c->hpinned, sizeof(float));
-  *sig = (double)s;
+  const volatile float *hs = (const volatile float *)(c->hres + dugma_ctx::kResSig);
+  *sig = (double)hs[0];
   return DUGMA_OK;
 }
 
