@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
   float n[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) n[a] = X[(size_t)ii * D + a];
-  const float DD = (float)D;
+  constexpr float kThird = 1.0f / 3.0f;
   unsigned best = 4000000000u;
   for (int j = j0; j < j1; ++j) {
     const float *m = Mp + (size_t)j * D;
@@ -73,7 +73,16 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
       const float e2 = n[2] - m[2];
       s2 = s2 + e2 * e2;
     }
-    float s = s2 / DD;
+    // fl32(s2 / D): for D = 3 the FMA-corrected product below is the IEEE
+    // quotient for every non-negative finite float (checked exhaustively on
+    // the CPU by tests/arith/div3_exhaustive.c); for D = 2 halving is exact
+    float s;
+    if (D == 3) {
+      const float q = s2 * kThird;
+      s = fmaf(fmaf(-3.0f, q, s2), kThird, q);
+    } else {
+      s = s2 * 0.5f;
+    }
     s = s * 1000000.0f;
     best = min(best, f2u_rn(s));
   }
