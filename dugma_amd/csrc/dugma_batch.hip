@@ -82,6 +82,7 @@ struct BParams {
   double beta;       // annealing decay per iteration
   double tol_rot;    // radians
   double tol_trans;  // in the trial's own units
+  double relax;      // over-relaxation of the pose step once annealing has ended (1: none)
   int max_iter;
   int sigma;         // 1: DUGMA sigma update
 };
@@ -333,12 +334,29 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   eig4_max(N4, q);
   const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
   for (int a = 0; a < 4; ++a) q[a] /= qn;
+  const bool annealing = tr.a > 0.0;
+  // over-relaxed step (after annealing): the rotation angle times relax,
+  // and below the weighted source centroid moved relax times as far
+  const double om = (!annealing && bp.relax > 1.0) ? bp.relax : 1.0;
+  if (om != 1.0) {
+    if (q[0] < 0.0)
+      for (int a = 0; a < 4; ++a) q[a] = -q[a];
+    const double vn = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (vn > 0.0) {
+      const double ph = om * atan2(vn, q[0]);
+      const double sv = sin(ph) / vn;
+      q[0] = cos(ph);
+      for (int a = 1; a < 4; ++a) q[a] *= sv;
+    }
+  }
   const double w0 = q[0], x = q[1], y = q[2], z = q[3];
   double dR[3][3] = {{w0 * w0 + x * x - y * y - z * z, 2 * (x * y - w0 * z), 2 * (x * z + w0 * y)},
                      {2 * (x * y + w0 * z), w0 * w0 - x * x + y * y - z * z, 2 * (y * z - w0 * x)},
                      {2 * (x * z - w0 * y), 2 * (y * z + w0 * x), w0 * w0 - x * x - y * y + z * z}};
+  // dt takes the source centroid mx to mx + om (my - mx) (to my when om = 1)
   double dt[3];
-  for (int a = 0; a < 3; ++a) dt[a] = my[a] - (dR[a][0] * mx[0] + dR[a][1] * mx[1] + dR[a][2] * mx[2]);
+  for (int a = 0; a < 3; ++a)
+    dt[a] = mx[a] + om * (my[a] - mx[a]) - (dR[a][0] * mx[0] + dR[a][1] * mx[1] + dR[a][2] * mx[2]);
   // x'_new = dR x' + dt with x' = R x + t - c
   double Rn[9], tc[3];
   for (int a = 0; a < 3; ++a) {
@@ -363,7 +381,6 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
     tr.sig = sig;
     tr.g = fmax(sig / tr.sbar, kGMin);
   }
-  const bool annealing = tr.a > 0.0;
   tr.a *= bp.beta;
   if (tr.a < tr.a_stop) tr.a = 0.0;
   if (!annealing && tr.dang < bp.tol_rot && tr.dtr < bp.tol_trans) {
@@ -774,6 +791,7 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   bp.tol_trans = opts->tol_trans;
   bp.max_iter = opts->max_iter;
   bp.sigma = sigma ? 1 : 0;
+  bp.relax = opts->relax > 1.0 ? opts->relax : 1.0;
   const int grid = opts->grid;
   // with the sigma update the first pass only measures sig
   const int passes = opts->max_iter + (sigma ? 1 : 0);
@@ -858,6 +876,7 @@ static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bo
   bp.tol_rot = -1.0;  // nothing converges while timing
   bp.tol_trans = -1.0;
   bp.max_iter = 1 << 30;
+  bp.relax = 1.0;
   bp.sigma = sigma ? 1 : 0;
   if ((rc = launch_iteration(b, 0, b->ntrials, bp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev0, b->stream));

@@ -197,6 +197,8 @@ typedef struct {
   int check_every;           /* graph mode: host checks for completion every n steps */
   int grid;                  /* workgroups of the persistent pair kernel (0: from the launch shape) */
   int sigma_update;          /* 1: DUGMA sigma update (one extra measuring pass first) */
+  double relax;              /* > 1: over-relax the pose step once annealing has ended
+                                (rotation angle and centroid move times relax) */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */
