@@ -189,26 +189,25 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
       dB[2] += fB2;
     }
     if (SIG && valid) atomicMin(nn + ti, __float_as_uint(dmin));
-    double r[kItemStat];
+    // item totals, one statistic at a time (few registers live): [E, W,
+    // W y (3), B (3), y B^T (9)]
+    const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
 #pragma unroll
-    for (int a = 0; a < kItemStat; ++a) r[a] = 0.0;
-    if (valid) {
-      const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
-      r[0] = dE;
-      r[1] = dW;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        r[2 + a] = dW * y[a];
-        r[5 + a] = dB[a];
-#pragma unroll
-        for (int b = 0; b < 3; ++b) r[8 + 3 * a + b] = y[a] * dB[b];
-      }
+    for (int a = 0; a < kItemStat; ++a) {
+      double v;
+      if (a == 0)
+        v = dE;
+      else if (a == 1)
+        v = dW;
+      else if (a < 5)
+        v = dW * y[a - 2];
+      else if (a < 8)
+        v = dB[a - 5];
+      else
+        v = y[(a - 8) / 3] * dB[(a - 8) % 3];
+      v = wave_sum(valid ? v : 0.0);
+      if (lane == 0) red[wave][a] = v;
     }
-#pragma unroll
-    for (int a = 0; a < kItemStat; ++a) r[a] = wave_sum(r[a]);
-    if (lane == 0)
-#pragma unroll
-      for (int a = 0; a < kItemStat; ++a) red[wave][a] = r[a];
     __syncthreads();
     if (threadIdx.x < kItemStat) {
       double s = red[0][threadIdx.x];
