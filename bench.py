@@ -345,7 +345,8 @@ def main():
             elif only == "large":
                 d3 = W.config3()
                 big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
-                print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"]}))
+                print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"],
+                                  "knn": bench_knn(dg, d3)}))
             else:
                 print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(128, args.trials))}))
             return

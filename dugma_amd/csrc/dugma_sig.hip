@@ -65,8 +65,15 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
   for (int a = 0; a < D; ++a) n[a] = X[(size_t)ii * D + a];
   constexpr float kThird = 1.0f / 3.0f;
   unsigned best = 4000000000u;
+  // candidate j + 1 is requested before candidate j is used (j0 < j1 for
+  // every workgroup; index clamped to the chunk)
+  float mn[3] = {Mp[(size_t)j0 * D], Mp[(size_t)j0 * D + 1], D == 3 ? Mp[(size_t)j0 * D + (D - 1)] : 0.0f};
   for (int j = j0; j < j1; ++j) {
-    const float *m = Mp + (size_t)j * D;
+    const float m[3] = {mn[0], mn[1], mn[2]};
+    const size_t jn = (size_t)min(j + 1, j1 - 1) * D;
+    mn[0] = Mp[jn];
+    mn[1] = Mp[jn + 1];
+    if (D == 3) mn[2] = Mp[jn + (D - 1)];
     const float e0 = n[0] - m[0], e1 = n[1] - m[1];
     float s2 = e0 * e0 + e1 * e1;
     if (D == 3) {
@@ -206,10 +213,12 @@ __global__ __launch_bounds__(kSBlock) void k_knn_merge(
     }
 }
 
-static int chunks_for(int nq, int np) {
+// Candidate chunks so that about 8,192 waves run, each chunk at least
+// min_chunk candidates long.
+static int chunks_for(int nq, int np, int min_chunk = 256) {
   const int qblocks = (nq + kSBlock - 1) / kSBlock;
   int c = (8192 + qblocks * 4 - 1) / (qblocks * 4);
-  c = std::max(1, std::min(c, (np + 255) / 256));
+  c = std::max(1, std::min(c, (np + min_chunk - 1) / min_chunk));
   return c;
 }
 
@@ -259,7 +268,8 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
   // transform, with q = 4e9 (update_sig.cu:13) set in the same launch
   rc = transform_compat(c, R, t, m, q, c->N, 4000000000u);
   if (rc) return rc;
-  const int nch = chunks_for(c->N, c->M);
+  // chunks merge through one atomicMin per target each: short chunks are cheap
+  const int nch = chunks_for(c->N, c->M, 64);
   const int chunk = (c->M + nch - 1) / nch;
   const int nchunks = (c->M + chunk - 1) / chunk;
   dim3 grid((c->N + kSBlock - 1) / kSBlock, nchunks);
