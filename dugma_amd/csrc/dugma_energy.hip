@@ -126,12 +126,13 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
   }
 }
 
-// Fixed-order sum of the per-workgroup partials (one workgroup of 1024).
-// The result goes straight to the context's pinned host slot.
+// Fixed-order sum of the per-workgroup partials (one workgroup of 1024):
+// strided loads per thread, a DPP sum per wave, then the 16 wave sums in
+// order.  The result goes straight to the context's pinned host slot.
 __global__ __launch_bounds__(1024) void k_reduce_parts(
     const double *__restrict__ parts, int nparts, int width,
     double *__restrict__ res) {
-  __shared__ double sh[1024][kPosePart];
+  __shared__ double sh[1024 / 64][kPosePart];
   double s[kPosePart];
 #pragma unroll
   for (int a = 0; a < kPosePart; ++a) s[a] = 0.0;
@@ -141,16 +142,18 @@ __global__ __launch_bounds__(1024) void k_reduce_parts(
     s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
     s[4] += v.x; s[5] += v.y; s[6] += v.z; s[7] += v.w;
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] = s[a];
-  __syncthreads();
-  for (int stride = 512; stride > 0; stride >>= 1) {
-    if ((int)threadIdx.x < stride)
-#pragma unroll
-      for (int a = 0; a < kPosePart; ++a) sh[threadIdx.x][a] += sh[threadIdx.x + stride][a];
-    __syncthreads();
+  for (int a = 0; a < kPosePart; ++a) {
+    const double w = wave_sum(s[a]);
+    if (lane == 0) sh[wave][a] = w;
   }
-  if ((int)threadIdx.x < width) res[threadIdx.x] = sh[0][threadIdx.x];
+  __syncthreads();
+  if ((int)threadIdx.x < width) {
+    double t = 0.0;
+    for (int w = 0; w < 1024 / 64; ++w) t += sh[w][threadIdx.x];
+    res[threadIdx.x] = t;
+  }
 }
 
 // Per-point sums over streamed chunks, in chunk order.
