@@ -74,3 +74,28 @@ def test_monomial_table_matches_reference_files():
     m3, m2 = gm.tables(ref)
     text = open(os.path.join(ROOT, "dugma_amd", "csrc", "dugma_monomials.inc")).read()
     assert gm.emit(m3, m2) == text
+
+
+def test_batch_argument_errors_before_any_device_call():
+    """dugma_batch_create* validate every trial before touching the device:
+    empty trials and covariances that are not s I are status -1 with a
+    message; the other batch entry points reject null handles."""
+    import numpy as np
+    import dugma_amd as dg
+    from dugma_amd import _lib
+    from dugma_amd import workloads as W
+    X = np.random.default_rng(0).normal(size=(3, 20))
+    C = np.tile(np.eye(3), (20, 1, 1))
+    C[5, 0, 1] = C[5, 1, 0] = 0.1  # not isotropic
+    with pytest.raises(_lib.DugmaError) as e:
+        dg.Batch([dict(X=X, X_C=W.blocks(C), Y=X, Y_C=W.blocks(np.tile(np.eye(3), (20, 1, 1))))])
+    assert e.value.code == -1 and "isotropic" in str(e.value)
+    with pytest.raises(_lib.DugmaError) as e:
+        dg.Batch([dict(X=X[:, :0], sX=np.zeros(0), Y=X, sY=np.ones(20))])
+    assert e.value.code == -1 and "empty trial" in str(e.value)
+    L = _lib.load()
+    assert L.dugma_batch_register(None, None, None, None, None, None, None, None, None) == -1
+    assert L.dugma_batch_autotune(None, None, None, 1, None, None, None, None) == -1
+    assert L.dugma_batch_free(None) == -1
+    ch = (ctypes.c_int * 4)(1, 1, 1, 1)
+    assert L.dugma_batch_set_shape(None, 2, ch, 8192) == -1

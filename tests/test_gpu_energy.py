@@ -206,3 +206,64 @@ def test_deterministic_repeats():
         assert a[0] == b[0] and (a[1] == b[1]).all()
     finally:
         dg.fr_m(h)
+
+
+def test_depth_views_full_size_adjoint_identities():
+    """Configuration 5, 3-D: two 320 x 240 depth views (about 76,800 points
+    each, full 3 x 3 covariances, invalid pixels left out) at full size,
+    checked through identities that do not need an N x M reference:
+      * moving every target by v and every source by R^T v leaves E alone,
+        so sum_i dE/dy_i + R sum_j dE/dx_j = 0;
+      * the pose pass's dE/dt is -sum_i dE/dy_i and its energy agrees;
+      * scaling every covariance by lambda: dE/dlambda at 1 equals
+        sum over points of tr(dE/dS S), against central differences of E."""
+    d = W.config5_3d()
+    R, t = perturbed_pose(d, deg=2.0, dt=0.005, seed=11)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        out = dg.energy_grad_points(h, R, t)
+        E1, gp = dg.energy_grad(h, R, t, "full")
+    finally:
+        dg.fr_m(h)
+    gX, gY = out["gX"], out["gY"]
+    assert abs(out["E"] / E1 - 1) <= 2 * E_TOL
+    mass = np.abs(gX).sum() + np.abs(gY).sum()
+    assert np.linalg.norm(gX.sum(0) + R @ gY.sum(0)) <= 1e-5 * mass
+    assert rel(gp[3:], -gX.sum(0)) <= G_TOL
+    SXc, SYc = W.unblocks(d["X_C"]), W.unblocks(d["Y_C"])
+    tr = np.einsum("nab,nba->", out["gXC"], SXc) + np.einsum("nab,nba->", out["gYC"], SYc)
+    lam = 1e-3
+    Es = []
+    for s in (1 + lam, 1 - lam):
+        okk, hs = dg.init(d["X"], d["X_C"] * s, d["Y"], d["Y_C"] * s)
+        try:
+            Es.append(dg.energy(hs, R, t, "full"))
+        finally:
+            dg.fr_m(hs)
+    fd = (Es[0] - Es[1]) / (2 * lam)
+    assert abs(tr - fd) <= 1e-3 * abs(fd), (tr, fd)
+
+
+def test_depth_views_subsample_matches_oracle(oracle):
+    """The same depth views, 1,500 points of each, per-point gradients against
+    the FP64 oracle."""
+    d = W.config5_3d()
+    rng = np.random.default_rng(3)
+    n, m = d["X"].shape[1], d["Y"].shape[1]
+    it, js = np.sort(rng.choice(n, 1500, replace=False)), np.sort(rng.choice(m, 1500, replace=False))
+    sub = dict(X=d["X"][:, it].copy(), Y=d["Y"][:, js].copy(),
+               X_C=W.blocks(W.unblocks(d["X_C"])[it]), Y_C=W.blocks(W.unblocks(d["Y_C"])[js]),
+               R_G=d["R_G"], t_G=d["t_G"])
+    R, t = perturbed_pose(sub, deg=2.0, dt=0.005, seed=12)
+    Y, SY, X, SX = as_oracle(sub)
+    ref = oracle.energy(Y, SY, X, SX, R, t, grads=True)
+    ok, h = dg.init(sub["X"], sub["X_C"], sub["Y"], sub["Y_C"])
+    try:
+        out = dg.energy_grad_points(h, R, t)
+    finally:
+        dg.fr_m(h)
+    assert abs(out["E"] / ref["E"] - 1) <= E_TOL
+    assert rel(out["gX"], ref["gY"]) <= G_TOL
+    assert rel(out["gXC"], ref["gSY"]) <= G_TOL
+    assert rel(out["gY"], ref["gX"]) <= G_TOL
+    assert rel(out["gYC"], ref["gSX"]) <= G_TOL
