@@ -320,8 +320,8 @@ def main():
                     help="first do one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU only: there is no multi-GPU path (targets or whole trials "
+              "would be sharded across GPUs with one all-reduce per iteration); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,9 +1,9 @@
 /*
  * dugma_oracle.c -- CPU restatement of the DUGMA registration hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this file's
- * library.  Built with -ffp-contract=off so every FP32 operation written
+ * TEST INFRASTRUCTURE ONLY (the oracle the HIP path is tested against, never
+ * the product): only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this file's library.  Built with -ffp-contract=off so every FP32 operation written
  * below rounds exactly once, in the order written.
  *
  * Pinning (what checks each function beyond restating it):

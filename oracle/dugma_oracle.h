@@ -1,9 +1,9 @@
 /*
  * dugma_oracle.h -- CPU restatement of the DUGMA registration hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
- * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load it.  The product (libdugma_hip.so) never links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP path is
+ * tested against: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (libdugma_hip.so) never links or calls it.
  *
  * Conventions shared by every function:
  *  - D is 2 or 3.  Point arrays are point-major (x,y[,z] per point), which

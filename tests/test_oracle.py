@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (the checker of the HIP path) against closed forms,
+"""Pins the CPU oracle (what the HIP path is tested against) against closed forms,
 library routines, brute force and the reference's own fixtures."""
 import math
 import os
