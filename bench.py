@@ -109,7 +109,13 @@ TUNE = True
 
 
 def bench_fused(hip, dg, d, steps, warmup, label):
+    # init (host arrays -> device state, the one PCIe transfer): the second of
+    # two contexts, so that code objects are already loaded
+    ok, h0 = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    dg.fr_m(h0)
+    t0 = time.perf_counter()
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    init_ms = (time.perf_counter() - t0) * 1e3
     try:
         R, t = start_pose(d)
         plan = h.autotune() if TUNE else (int(os.environ.get("DUGMA_LANE_POINTS", 0)),
@@ -127,7 +133,7 @@ def bench_fused(hip, dg, d, steps, warmup, label):
         log("%s: %.3f ms/step, pair kernel %.3f ms/launch (%d launches), iso=%s, plan T=%d waves=%d"
             % (label, ms / steps, kms / max(launches, 1), launches, iso, plan[0], plan[1]))
         return dict(ms_per_step=ms / steps, kernel_ms=kms / max(launches, 1), pairs=nT * nS, iso=iso,
-                    plan={"lane_points": plan[0], "target_waves": plan[1]})
+                    plan={"lane_points": plan[0], "target_waves": plan[1]}, init_ms=init_ms)
     finally:
         dg.fr_m(h)
 
@@ -148,11 +154,15 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
 
 # Batched iteration: DUGMA's sigma update (the kernel variance tracks the mean
 # nearest distance) on top of an annealed extra variance, pose steps
-# over-relaxed by 1.9 once annealing has ended; stopping rules of the
-# reference's own driver (main_kinect_real_application.m:14-17: at most 300
-# iterations, rotation step 1e-4 rad, translation step 1e-4).
-BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=1e-4, tol_trans=1e-4, relax=1.9)
-BATCH_MAX_ITER = 300
+# over-relaxed by 1.9 once annealing has ended; the first 160 iterations on
+# every 4th target and source point (1/16 of the pairs), then the full clouds
+# under the stopping rules of the reference's own driver
+# (main_kinect_real_application.m:14-17: at most 300 iterations, rotation
+# step 1e-4 rad, translation step 1e-4).
+BATCH_COARSE_ITERS = 160
+BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=1e-4, tol_trans=1e-4, relax=1.9,
+                  coarse_iters=BATCH_COARSE_ITERS)
+BATCH_MAX_ITER = 300 + BATCH_COARSE_ITERS
 
 
 def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
@@ -177,7 +187,8 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
         # far the energy maximum itself sits from it)
         R_G = np.array([tr["R_G"] for tr in trials])
         t_G = np.array([tr["t_G"] for tr in trials])
-        outg = b.register(R_G, t_G, max_iter=max_iter, use_graph=True, **{**BATCH_MODE, "anneal": False})
+        outg = b.register(R_G, t_G, max_iter=max_iter - BATCH_COARSE_ITERS, use_graph=True,
+                          **{**BATCH_MODE, "anneal": False, "coarse_iters": 0})
     finally:
         b.free()
     errs = np.array([W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
@@ -201,10 +212,11 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
                         "noise U(0.001,0.02), outliers U(0,0.3), isotropic 8-NN sigma",
             "value": round(rps, 2), "unit": "registrations/s", "wall_s": round(wall, 4),
             "iteration": "majorise-minimise pose step; DUGMA sigma update%s; steps over-relaxed by %g after "
-                         "annealing; stop at rotation step < %g rad and translation step < %g, or %d iterations" %
+                         "annealing; first %d iterations on every 4th point; stop at rotation step < %g rad "
+                         "and translation step < %g, or %d iterations in all" %
                          (" + annealed extra variance (decay %g)" % BATCH_MODE["anneal_decay"]
-                          if BATCH_MODE["anneal"] else "", BATCH_MODE["relax"], BATCH_MODE["tol_rot"],
-                          BATCH_MODE["tol_trans"], max_iter),
+                          if BATCH_MODE["anneal"] else "", BATCH_MODE["relax"], BATCH_MODE["coarse_iters"],
+                          BATCH_MODE["tol_rot"], BATCH_MODE["tol_trans"], max_iter),
             "converged_fraction": round(float((out["status"] == 1).mean()), 4),
             "iterations_mean": float(out["iters"].mean()), "max_iter": max_iter,
             "pairs_per_iteration": info["pairs_per_iteration"], "work_items": info["items"],
@@ -376,6 +388,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                      "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED,
                      "launch": fused["plan"]},
+        "init_ms": round(fused["init_ms"], 3),
     }
     lp = fused["plan"]
     out["roofline"]["traffic"] = MEASURED_TRAFFIC_6K_FULL.get((lp["lane_points"], lp["target_waves"]))

@@ -22,7 +22,8 @@ class BatchOpts(ctypes.Structure):
                 ("anneal_decay", ctypes.c_double), ("tol_rot", ctypes.c_double),
                 ("tol_trans", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int), ("relax", ctypes.c_double)]
+                ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int), ("relax", ctypes.c_double),
+                ("coarse_iters", ctypes.c_int)]
 
 
 class RegisterOpts(ctypes.Structure):

@@ -392,13 +392,10 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   trials[k] = tr;
 }
 
-}  // namespace dugma
-
-using namespace dugma;
-
-struct dugma_batch {
-  int ntrials = 0;
-  hipStream_t stream = nullptr;
+// One resolution of every trial's clouds.  Level 0 holds all kept points;
+// level 1 every kCoarseStep-th target and source of each trial (same
+// centroid, same per-point variances), for the early, broad iterations.
+struct BLevel {
   int ntgt = 0, nsrc = 0, nitems = 0;
   Rec3I *dT = nullptr;       // targets (all trials), relative to each trial's centroid
   Rec3I *dS = nullptr;       // per-iteration source records
@@ -406,29 +403,49 @@ struct dugma_batch {
   int *dsrc_trial = nullptr;
   BItem *ditems = nullptr;
   int *dorder = nullptr;     // queue position -> item, longest first
+  int *dident = nullptr;     // identity order for one-trial launches
   int *dtitem = nullptr;     // CSR: items of trial k are [titem[k], titem[k+1])
   int *dttgt = nullptr;      // CSR: targets of trial k are [ttgt[k], ttgt[k+1])
   unsigned *dnn = nullptr;   // per target: smallest |d|^2 of the current pass (float bits)
   double *dpart = nullptr;   // per-item statistics
-  BTrial *dtrials = nullptr;
-  int *ddone = nullptr;
-  unsigned *dqueue = nullptr;
-  int *dident = nullptr;     // identity order for one-trial launches
   size_t items_cap = 0;      // capacity of ditems / dorder / dident / dpart
   std::vector<int> tb, nt, sb, ns;  // per trial: target / source base and count
   std::vector<int> titem;
+  long long pairs = 0;              // target x source pairs of one full iteration
+};
+
+constexpr int kCoarseStep = 4;
+
+}  // namespace dugma
+
+using namespace dugma;
+
+struct dugma_batch {
+  int ntrials = 0;
+  hipStream_t stream = nullptr;
+  BLevel lv[2];
+  BTrial *dtrials = nullptr;
+  int *ddone = nullptr;
+  unsigned *dqueue = nullptr;
   std::vector<double> a0, astop, cent, sbar;  // per trial
-  long long pairs = 0;                        // target x source pairs of one full iteration
   // launch shape
   int waves = 4;                            // waves per workgroup: target tile = 64 * waves
   int chunks[kClasses] = {1, 1, 2, 4};      // source chunks per target tile, per size class
   int grid_waves = 8192;                    // resident waves of the persistent pair kernel
 };
 
+static void free_level(BLevel &L) {
+  void *ptrs[] = {L.dT, L.dS, L.dsrc, L.dsrc_trial, L.ditems, L.dorder, L.dident,
+                  L.dtitem, L.dttgt, L.dnn, L.dpart};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
 static void free_batch(dugma_batch *b) {
   if (!b) return;
-  void *ptrs[] = {b->dT,    b->dS,    b->dsrc,    b->dsrc_trial, b->ditems, b->dorder, b->dtitem,
-                  b->dttgt, b->dnn,   b->dpart,   b->dtrials,    b->ddone,  b->dqueue, b->dident};
+  free_level(b->lv[0]);
+  free_level(b->lv[1]);
+  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -446,16 +463,17 @@ static int bmalloc(T **p, size_t n) {
   return DUGMA_OK;
 }
 
-// Work items for the batch's current launch shape: per trial, target tiles
-// of 64 * waves points times the source chunks of its size class; queue
-// order longest first.  (Re)allocates the item arrays when they grow.
-static int make_items(dugma_batch *b) {
+// Work items of one level for the batch's current launch shape: per trial,
+// target tiles of 64 * waves points times the source chunks of its size
+// class; queue order longest first.  (Re)allocates the item arrays when they
+// grow.
+static int make_items(dugma_batch *b, BLevel &L) {
   const int K = b->ntrials, tile_t = 64 * b->waves;
   std::vector<BItem> items;
-  b->titem.assign(K + 1, 0);
+  L.titem.assign(K + 1, 0);
   for (int k = 0; k < K; ++k) {
-    b->titem[k] = (int)items.size();
-    const int nt = b->nt[k], ns = b->ns[k];
+    L.titem[k] = (int)items.size();
+    const int nt = L.nt[k], ns = L.ns[k];
     if (nt == 0 || ns == 0) continue;
     const int nch = std::max(1, std::min(b->chunks[size_class(ns)], ns));
     const int tile_s = (ns + nch - 1) / nch;
@@ -464,41 +482,77 @@ static int make_items(dugma_batch *b) {
         BItem it;
         memset(&it, 0, sizeof(it));
         it.trial = k;
-        it.t0 = b->tb[k] + t0;
+        it.t0 = L.tb[k] + t0;
         it.tn = std::min(tile_t, nt - t0);
-        it.s0 = b->sb[k] + s0;
+        it.s0 = L.sb[k] + s0;
         it.sn = std::min(tile_s, ns - s0);
         items.push_back(it);
       }
   }
-  b->titem[K] = (int)items.size();
+  L.titem[K] = (int)items.size();
   std::vector<int> order(items.size()), ident(items.size());
   std::iota(order.begin(), order.end(), 0);
   std::iota(ident.begin(), ident.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
     return (long long)items[u].tn * items[u].sn > (long long)items[v].tn * items[v].sn;
   });
-  if (items.size() > b->items_cap) {
-    void *old[] = {b->ditems, b->dorder, b->dident, b->dpart};
+  if (items.size() > L.items_cap) {
+    void *old[] = {L.ditems, L.dorder, L.dident, L.dpart};
     for (void *p : old)
       if (p) (void)hipFree(p);
-    b->ditems = nullptr;
-    b->dorder = b->dident = nullptr;
-    b->dpart = nullptr;
-    b->items_cap = 0;
+    L.ditems = nullptr;
+    L.dorder = L.dident = nullptr;
+    L.dpart = nullptr;
+    L.items_cap = 0;
     int rc;
-    if ((rc = bmalloc(&b->ditems, items.size())) || (rc = bmalloc(&b->dorder, items.size())) ||
-        (rc = bmalloc(&b->dident, items.size())) || (rc = bmalloc(&b->dpart, items.size() * kStat)))
+    if ((rc = bmalloc(&L.ditems, items.size())) || (rc = bmalloc(&L.dorder, items.size())) ||
+        (rc = bmalloc(&L.dident, items.size())) || (rc = bmalloc(&L.dpart, items.size() * kStat)))
       return rc;
-    b->items_cap = items.size();
+    L.items_cap = items.size();
   }
-  b->nitems = (int)items.size();
+  L.nitems = (int)items.size();
   if (!items.empty()) {
-    DUGMA_HIP(hipMemcpy(b->ditems, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
-    DUGMA_HIP(hipMemcpy(b->dorder, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
-    DUGMA_HIP(hipMemcpy(b->dident, ident.data(), ident.size() * sizeof(int), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(L.ditems, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(L.dorder, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(L.dident, ident.data(), ident.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  DUGMA_HIP(hipMemcpy(b->dtitem, b->titem.data(), b->titem.size() * sizeof(int), hipMemcpyHostToDevice));
+  DUGMA_HIP(hipMemcpy(L.dtitem, L.titem.data(), L.titem.size() * sizeof(int), hipMemcpyHostToDevice));
+  return DUGMA_OK;
+}
+
+static int make_all_items(dugma_batch *b) {
+  int rc = make_items(b, b->lv[0]);
+  if (!rc) rc = make_items(b, b->lv[1]);
+  return rc;
+}
+
+// Host copies of one level's points, built per trial, then uploaded.
+struct LevelHost {
+  std::vector<Rec3I> tg;
+  std::vector<double> sr;
+  std::vector<int> strial, ttgt;
+};
+
+static int upload_level(dugma_batch *b, BLevel &L, const LevelHost &H) {
+  const int K = b->ntrials;
+  L.ntgt = (int)H.tg.size();
+  L.nsrc = (int)(H.sr.size() / 4);
+  int rc;
+  if ((rc = bmalloc(&L.dT, H.tg.size())) || (rc = bmalloc(&L.dS, (size_t)L.nsrc)) ||
+      (rc = bmalloc(&L.dsrc, H.sr.size())) || (rc = bmalloc(&L.dsrc_trial, H.strial.size())) ||
+      (rc = bmalloc(&L.dtitem, (size_t)K + 1)) || (rc = bmalloc(&L.dttgt, (size_t)K + 1)) ||
+      (rc = bmalloc(&L.dnn, H.tg.size())))
+    return rc;
+  struct Up {
+    void *d;
+    const void *h;
+    size_t n;
+  } ups[] = {{L.dT, H.tg.data(), H.tg.size() * sizeof(Rec3I)},
+             {L.dsrc, H.sr.data(), H.sr.size() * sizeof(double)},
+             {L.dsrc_trial, H.strial.data(), H.strial.size() * sizeof(int)},
+             {L.dttgt, H.ttgt.data(), H.ttgt.size() * sizeof(int)}};
+  for (const Up &u : ups)
+    if (u.n) DUGMA_HIP(hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice));
   return DUGMA_OK;
 }
 
@@ -537,13 +591,15 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
                        dugma_batch **out) {
   dugma_batch *b = new dugma_batch();
   b->ntrials = ntrials;
-  std::vector<Rec3I> tg;
-  std::vector<double> sr;
-  std::vector<int> strial, ttgt(ntrials + 1, 0);
-  b->tb.assign(ntrials, 0);
-  b->nt.assign(ntrials, 0);
-  b->sb.assign(ntrials, 0);
-  b->ns.assign(ntrials, 0);
+  LevelHost H[2];
+  for (int l = 0; l < 2; ++l) {
+    BLevel &L = b->lv[l];
+    L.tb.assign(ntrials, 0);
+    L.nt.assign(ntrials, 0);
+    L.sb.assign(ntrials, 0);
+    L.ns.assign(ntrials, 0);
+    H[l].ttgt.assign(ntrials + 1, 0);
+  }
   b->a0.resize(ntrials);
   b->astop.resize(ntrials);
   b->sbar.resize(ntrials);
@@ -553,8 +609,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       free_batch(b);
       return fail(DUGMA_ERR_ARG, "dugma_batch_create: empty trial");
     }
-    std::vector<double> ty;
-    std::vector<double> tv;
+    std::vector<double> ty, tv, sy;
     double c[3] = {0, 0, 0};
     for (int i = 0; i < N[k]; ++i) {
       const double *p = X[k] + (size_t)i * 3;
@@ -570,24 +625,6 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       }
       tv.push_back(v);
     }
-    const int nt = (int)tv.size();
-    double r2 = 0.0, vt = 0.0, vs = 0.0;
-    if (nt > 0)
-      for (int a = 0; a < 3; ++a) c[a] /= nt;
-    b->tb[k] = (int)tg.size();
-    for (int i = 0; i < nt; ++i) {
-      Rec3I r;
-      memset(&r, 0, sizeof(r));
-      r.hx = (float)(ty[3 * i] - c[0]);
-      r.hy = (float)(ty[3 * i + 1] - c[1]);
-      r.hz = (float)(ty[3 * i + 2] - c[2]);
-      r.s = (float)tv[i];
-      r2 += (double)r.hx * r.hx + (double)r.hy * r.hy + (double)r.hz * r.hz;
-      vt += tv[i];
-      tg.push_back(r);
-    }
-    b->sb[k] = (int)(sr.size() / 4);
-    int ns = 0;
     for (int j = 0; j < M[k]; ++j) {
       const double *p = Y[k] + (size_t)j * 3;
       const double v = var(vctx, k, 1, j);
@@ -596,67 +633,70 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
         return fail(DUGMA_ERR_ARG, "dugma_batch_create: covariances must be isotropic (s I)");
       }
       if (v < 0 || !std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
-      for (int a = 0; a < 3; ++a) sr.push_back((double)(float)p[a]);
-      sr.push_back(v);
-      strial.push_back(k);
-      vs += v;
-      ++ns;
+      for (int a = 0; a < 3; ++a) sy.push_back((double)(float)p[a]);
+      sy.push_back(v);
     }
-    b->nt[k] = nt;
-    b->ns[k] = ns;
-    ttgt[k + 1] = ttgt[k] + nt;
+    const int nt = (int)tv.size(), ns = (int)(sy.size() / 4);
+    if (nt > 0)
+      for (int a = 0; a < 3; ++a) c[a] /= nt;
+    double r2 = 0.0, vt = 0.0, vs = 0.0;
+    for (int l = 0; l < 2; ++l) {
+      const int step = l == 0 ? 1 : kCoarseStep;
+      BLevel &L = b->lv[l];
+      LevelHost &h = H[l];
+      L.tb[k] = (int)h.tg.size();
+      for (int i = 0; i < nt; i += step) {
+        Rec3I r;
+        memset(&r, 0, sizeof(r));
+        r.hx = (float)(ty[3 * i] - c[0]);
+        r.hy = (float)(ty[3 * i + 1] - c[1]);
+        r.hz = (float)(ty[3 * i + 2] - c[2]);
+        r.s = (float)tv[i];
+        if (l == 0) {
+          r2 += (double)r.hx * r.hx + (double)r.hy * r.hy + (double)r.hz * r.hz;
+          vt += tv[i];
+        }
+        h.tg.push_back(r);
+      }
+      L.nt[k] = (int)h.tg.size() - L.tb[k];
+      L.sb[k] = (int)(h.sr.size() / 4);
+      for (int j = 0; j < ns; j += step) {
+        for (int a = 0; a < 4; ++a) h.sr.push_back(sy[4 * (size_t)j + a]);
+        h.strial.push_back(k);
+        if (l == 0) vs += sy[4 * (size_t)j + 3];
+      }
+      L.ns[k] = (int)(h.sr.size() / 4) - L.sb[k];
+      h.ttgt[k + 1] = h.ttgt[k] + L.nt[k];
+      L.pairs += (long long)L.nt[k] * L.ns[k];
+    }
     for (int a = 0; a < 3; ++a) b->cent[3 * k + a] = c[a];
     // annealing from a quarter of the target's mean squared radius down to
     // a hundredth of the mean point variance
-    b->pairs += (long long)nt * ns;
     b->a0[k] = nt ? 0.25 * r2 / nt : 0.0;
     b->astop[k] = (nt + ns) ? 0.01 * (vt + vs) / (nt + ns) : 0.0;
     b->sbar[k] = (nt ? vt / nt : 0.0) + (ns ? vs / ns : 0.0);
   }
-  b->ntgt = (int)tg.size();
-  b->nsrc = (int)(sr.size() / 4);
   int rc = DUGMA_OK;
-#define BTRY(x)                     \
-  do {                              \
-    rc = (x);                       \
-    if (rc != DUGMA_OK) goto fail_; \
-  } while (0)
   {
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) BTRY(hip_fail(e, "hipStreamCreateWithFlags"));
-    BTRY(bmalloc(&b->dT, tg.size()));
-    BTRY(bmalloc(&b->dS, (size_t)b->nsrc));
-    BTRY(bmalloc(&b->dsrc, sr.size()));
-    BTRY(bmalloc(&b->dsrc_trial, strial.size()));
-    BTRY(bmalloc(&b->dtitem, (size_t)ntrials + 1));
-    BTRY(bmalloc(&b->dttgt, (size_t)ntrials + 1));
-    BTRY(bmalloc(&b->dnn, tg.size()));
-    BTRY(bmalloc(&b->dtrials, (size_t)ntrials));
-    BTRY(bmalloc(&b->ddone, (size_t)ntrials));
-    BTRY(bmalloc(&b->dqueue, 1));
-    struct Up {
-      void *d;
-      const void *h;
-      size_t n;
-    } ups[] = {{b->dT, tg.data(), tg.size() * sizeof(Rec3I)},
-               {b->dsrc, sr.data(), sr.size() * sizeof(double)},
-               {b->dsrc_trial, strial.data(), strial.size() * sizeof(int)},
-               {b->dttgt, ttgt.data(), ttgt.size() * sizeof(int)}};
-    for (const Up &u : ups)
-      if (u.n) {
-        e = hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice);
-        if (e != hipSuccess) BTRY(hip_fail(e, "hipMemcpy(batch)"));
-      }
-    e = hipMemset(b->dqueue, 0, sizeof(unsigned));
-    if (e != hipSuccess) BTRY(hip_fail(e, "hipMemset(queue)"));
-    BTRY(make_items(b));
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamCreateWithFlags");
   }
-#undef BTRY
+  if (!rc) rc = upload_level(b, b->lv[0], H[0]);
+  if (!rc) rc = upload_level(b, b->lv[1], H[1]);
+  if (!rc) rc = bmalloc(&b->dtrials, (size_t)ntrials);
+  if (!rc) rc = bmalloc(&b->ddone, (size_t)ntrials);
+  if (!rc) rc = bmalloc(&b->dqueue, 1);
+  if (!rc) {
+    hipError_t e = hipMemset(b->dqueue, 0, sizeof(unsigned));
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemset(queue)");
+  }
+  if (!rc) rc = make_all_items(b);
+  if (rc) {
+    free_batch(b);
+    return rc;
+  }
   *out = b;
   return DUGMA_OK;
-fail_:
-  free_batch(b);
-  return rc;
 }
 
 extern "C" int dugma_batch_create(int ntrials, int D, const int *N, const int *M,
@@ -697,51 +737,60 @@ extern "C" int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
                                 long long *pairs_per_iteration) {
   if (!b) return fail(DUGMA_ERR_ARG, "dugma_batch_info: null batch");
   if (ntrials) *ntrials = b->ntrials;
-  if (nitems) *nitems = b->nitems;
-  if (pairs_per_iteration) *pairs_per_iteration = b->pairs;
+  if (nitems) *nitems = b->lv[0].nitems;
+  if (pairs_per_iteration) *pairs_per_iteration = b->lv[0].pairs;
   return DUGMA_OK;
 }
 
 template <int W>
-static void launch_pairs(dugma_batch *b, int grid, const int *order, int ni, bool sigma) {
+static void launch_pairs(dugma_batch *b, const BLevel &L, int grid, const int *order, int ni, bool sigma) {
   if (sigma)
-    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, b->dT, b->dS,
-                       b->ditems, order, ni, b->dqueue, b->ddone, b->dtrials, b->dnn, b->dpart);
+    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dS, L.ditems,
+                       order, ni, b->dqueue, b->ddone, b->dtrials, L.dnn, L.dpart);
   else
-    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, b->dT, b->dS,
-                       b->ditems, order, ni, b->dqueue, b->ddone, b->dtrials, b->dnn, b->dpart);
+    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dS, L.ditems,
+                       order, ni, b->dqueue, b->ddone, b->dtrials, L.dnn, L.dpart);
 }
 
-// grid: workgroups of the persistent pair kernel (0: from the launch shape)
-static int launch_iteration(dugma_batch *b, int k0, int k1, const BParams &bp,
+// One iteration of trials [k0, k1) on level l.  grid: workgroups of the
+// persistent pair kernel (0: from the launch shape).
+static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams &bp,
                             bool one_trial, int grid) {
-  int s0 = 0, s1 = b->nsrc;
-  const int *order = b->dorder;
-  int ni = b->nitems;
+  const BLevel &L = b->lv[l];
+  int s0 = 0, s1 = L.nsrc;
+  const int *order = L.dorder;
+  int ni = L.nitems;
   if (one_trial) {
     // items of trial k0 are contiguous: [titem[k0], titem[k0+1])
-    const int i0 = b->titem[k0];
-    ni = b->titem[k0 + 1] - i0;
-    order = b->dident + i0;
-    s0 = b->sb[k0];
-    s1 = s0 + b->ns[k0];
+    const int i0 = L.titem[k0];
+    ni = L.titem[k0 + 1] - i0;
+    order = L.dident + i0;
+    s0 = L.sb[k0];
+    s1 = s0 + L.ns[k0];
   }
   if (s1 > s0)
-    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + 255) / 256), dim3(256), 0, b->stream, b->dsrc,
-                       b->dsrc_trial, s0, s1, b->dtrials, b->ddone, b->dS);
+    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + 255) / 256), dim3(256), 0, b->stream, L.dsrc,
+                       L.dsrc_trial, s0, s1, b->dtrials, b->ddone, L.dS);
   if (ni > 0) {
     if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
     grid = std::min(grid, ni);
     switch (b->waves) {
-      case 1: launch_pairs<1>(b, grid, order, ni, bp.sigma != 0); break;
-      case 2: launch_pairs<2>(b, grid, order, ni, bp.sigma != 0); break;
-      default: launch_pairs<4>(b, grid, order, ni, bp.sigma != 0); break;
+      case 1: launch_pairs<1>(b, L, grid, order, ni, bp.sigma != 0); break;
+      case 2: launch_pairs<2>(b, L, grid, order, ni, bp.sigma != 0); break;
+      default: launch_pairs<4>(b, L, grid, order, ni, bp.sigma != 0); break;
     }
   }
-  hipLaunchKernelGGL(k_batch_update, dim3(k1 - k0), dim3(kUpd), 0, b->stream, b->dpart, b->dtitem, b->dttgt, k0,
-                     k1, b->dtrials, b->ddone, b->dqueue, b->dnn, bp);
+  hipLaunchKernelGGL(k_batch_update, dim3(k1 - k0), dim3(kUpd), 0, b->stream, L.dpart, L.dtitem, L.dttgt, k0,
+                     k1, b->dtrials, b->ddone, b->dqueue, L.dnn, bp);
   DUGMA_HIP(hipGetLastError());
   return DUGMA_OK;
+}
+
+// Marks every running trial's sigma as unmeasured, so the next pass (on the
+// other level) measures it again before moving the pose.
+__global__ __launch_bounds__(256) void k_batch_resig(BTrial *__restrict__ trials, int K) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < K) trials[k].sig_ready = 0;
 }
 
 // Trial states at the start of a run, uploaded with every done flag clear
@@ -765,9 +814,10 @@ static int start_trials(dugma_batch *b, const double *R0, const double *t0, bool
   std::vector<int> done(K, 0);
   DUGMA_HIP(hipMemcpyAsync(b->dtrials, tr.data(), sizeof(BTrial) * K, hipMemcpyHostToDevice, b->stream));
   DUGMA_HIP(hipMemcpyAsync(b->ddone, done.data(), sizeof(int) * K, hipMemcpyHostToDevice, b->stream));
-  if (b->ntgt > 0)
-    hipLaunchKernelGGL(k_batch_fill, dim3((b->ntgt + 255) / 256), dim3(256), 0, b->stream, b->dnn, b->ntgt,
-                       kInfBits);
+  for (int l = 0; l < 2; ++l)
+    if (b->lv[l].ntgt > 0)
+      hipLaunchKernelGGL(k_batch_fill, dim3((b->lv[l].ntgt + 255) / 256), dim3(256), 0, b->stream, b->lv[l].dnn,
+                         b->lv[l].ntgt, kInfBits);
   DUGMA_HIP(hipGetLastError());
   DUGMA_HIP(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
   return DUGMA_OK;
@@ -779,6 +829,7 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
                                     int *status, double *E) {
   if (!b || !R0 || !t0 || !opts) return fail(DUGMA_ERR_ARG, "dugma_batch_register: bad argument");
   if (opts->max_iter < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_register: max_iter must be >= 1");
+  if (opts->coarse_iters < 0) return fail(DUGMA_ERR_ARG, "dugma_batch_register: coarse_iters must be >= 0");
   const int K = b->ntrials;
   const bool sigma = opts->sigma_update != 0;
   int rc = start_trials(b, R0, t0, opts->anneal != 0, sigma);
@@ -792,15 +843,30 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   bp.sigma = sigma ? 1 : 0;
   bp.relax = opts->relax > 1.0 ? opts->relax : 1.0;
   const int grid = opts->grid;
-  // with the sigma update the first pass only measures sig
-  const int passes = opts->max_iter + (sigma ? 1 : 0);
+  // pass schedule: with the sigma update a first pass only measures sig; the
+  // first coarse_iters pose steps run on the coarse level, and when the fine
+  // level takes over, its first pass measures sig again
+  const int coarse = std::min(opts->coarse_iters, opts->max_iter);
+  const int extra = sigma ? (coarse > 0 ? 2 : 1) : 0;
+  const int passes = opts->max_iter + extra;
+  const int fine_from = coarse > 0 ? coarse + (sigma ? 1 : 0) : 0;  // first fine pass
+  auto level_of = [&](int pass) { return pass < fine_from ? 1 : 0; };
+  // the coarse level never declares convergence
+  BParams bpc = bp;
+  bpc.tol_rot = -1.0;
+  bpc.tol_trans = -1.0;
+  auto params = [&](int l) -> const BParams & { return l == 1 ? bpc : bp; };
+  auto switch_level = [&](int k0, int n) {
+    if (sigma) hipLaunchKernelGGL(k_batch_resig, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->dtrials + k0, n);
+  };
   if (opts->one_trial_per_launch) {
     // baseline: each trial on its own, 3 launches and a host check per iteration
     int *hdone = nullptr;
     DUGMA_HIP(hipHostMalloc((void **)&hdone, sizeof(int), hipHostMallocDefault));
     for (int k = 0; k < K && !rc; ++k) {
       for (int it = 0; it < passes && !rc; ++it) {
-        rc = launch_iteration(b, k, k + 1, bp, true, grid);
+        if (coarse > 0 && it == fine_from) switch_level(k, 1);
+        rc = launch_iteration(b, level_of(it), k, k + 1, params(level_of(it)), true, grid);
         if (rc) break;
         hipError_t e = hipMemcpyAsync(hdone, b->ddone + k, sizeof(int), hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -810,19 +876,28 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     }
     (void)hipHostFree(hdone);
   } else if (opts->use_graph) {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    DUGMA_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_iteration(b, 0, K, bp, false, grid);
-    hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
-    if (!rc) {
-      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    // one captured iteration per level, replayed
+    hipGraph_t g[2] = {nullptr, nullptr};
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    for (int l = 0; l < 2 && !rc; ++l) {
+      if (l == 1 && coarse == 0) break;
+      hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "hipStreamBeginCapture");
+        break;
+      }
+      rc = launch_iteration(b, l, 0, K, params(l), false, grid);
+      e = hipStreamEndCapture(b->stream, &g[l]);
+      if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+      if (!rc) {
+        e = hipGraphInstantiate(&ge[l], g[l], nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+      }
     }
     const int check = opts->check_every > 0 ? opts->check_every : 10;
     for (int it = 0; it < passes && !rc; ++it) {
-      e = hipGraphLaunch(ge, b->stream);
+      if (coarse > 0 && it == fine_from) switch_level(0, K);
+      hipError_t e = hipGraphLaunch(ge[level_of(it)], b->stream);
       if (e != hipSuccess) {
         rc = hip_fail(e, "hipGraphLaunch");
         break;
@@ -841,10 +916,15 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
         if (nd == K) break;
       }
     }
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (g) (void)hipGraphDestroy(g);
+    for (int l = 0; l < 2; ++l) {
+      if (ge[l]) (void)hipGraphExecDestroy(ge[l]);
+      if (g[l]) (void)hipGraphDestroy(g[l]);
+    }
   } else {
-    for (int it = 0; it < passes && !rc; ++it) rc = launch_iteration(b, 0, K, bp, false, grid);
+    for (int it = 0; it < passes && !rc; ++it) {
+      if (coarse > 0 && it == fine_from) switch_level(0, K);
+      rc = launch_iteration(b, level_of(it), 0, K, params(level_of(it)), false, grid);
+    }
   }
   if (rc) return rc;
   std::vector<BTrial> tr(K);
@@ -863,8 +943,9 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   return DUGMA_OK;
 }
 
-// Average time of one all-trial iteration (ms) under the current launch
-// shape, from the given start poses: one untimed pass, then `reps` timed.
+// Average time of one all-trial iteration (ms) on the fine level under the
+// current launch shape, from the given start poses: one untimed pass, then
+// `reps` timed.
 static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bool sigma, int reps,
                           hipEvent_t ev0, hipEvent_t ev1, double *ms) {
   int rc = start_trials(b, R0, t0, false, sigma);
@@ -877,10 +958,10 @@ static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bo
   bp.max_iter = 1 << 30;
   bp.relax = 1.0;
   bp.sigma = sigma ? 1 : 0;
-  if ((rc = launch_iteration(b, 0, b->ntrials, bp, false, 0))) return rc;
+  if ((rc = launch_iteration(b, 0, 0, b->ntrials, bp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev0, b->stream));
   for (int r = 0; r < reps; ++r)
-    if ((rc = launch_iteration(b, 0, b->ntrials, bp, false, 0))) return rc;
+    if ((rc = launch_iteration(b, 0, 0, b->ntrials, bp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev1, b->stream));
   DUGMA_HIP(hipEventSynchronize(ev1));
   float f = 0.0f;
@@ -901,17 +982,18 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     (void)hipEventDestroy(ev0);
     return hip_fail(e, "hipEventCreate");
   }
+  const BLevel &L = b->lv[0];
   int rc = DUGMA_OK;
   double best = 0.0, cur = 0.0;
   bool present[kClasses] = {false, false, false, false};
   for (int k = 0; k < b->ntrials; ++k)
-    if (b->nt[k] > 0 && b->ns[k] > 0) present[size_class(b->ns[k])] = true;
+    if (L.nt[k] > 0 && L.ns[k] > 0) present[size_class(L.ns[k])] = true;
   // 1. waves per workgroup (the target tile)
   const int wave_opts[] = {1, 2, 4};
   int best_w = b->waves;
   for (int w : wave_opts) {
     b->waves = w;
-    if ((rc = make_items(b)) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
+    if ((rc = make_items(b, b->lv[0])) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
     if (best == 0.0 || cur < best) {
       best = cur;
       best_w = w;
@@ -927,7 +1009,7 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     for (int n : chunk_opts) {
       if (n == keep) continue;
       b->chunks[c] = n;
-      if ((rc = make_items(b)) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
+      if ((rc = make_items(b, b->lv[0])) || (rc = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur))) break;
       if (cur < best) {
         best = cur;
         best_n = n;
@@ -936,7 +1018,7 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     b->chunks[c] = best_n;
   }
   // 3. resident waves of the persistent kernel
-  if (!rc) rc = make_items(b);
+  if (!rc) rc = make_items(b, b->lv[0]);
   const int grid_opts[] = {4096, 8192, 16384, 32768};
   const int keep_g = b->grid_waves;
   int best_g = keep_g;
@@ -950,6 +1032,7 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     }
   }
   b->grid_waves = best_g;
+  if (!rc) rc = make_items(b, b->lv[1]);  // the coarse level follows the chosen shape
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   if (rc) return rc;
@@ -969,5 +1052,5 @@ extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunk
   b->waves = waves;
   for (int c = 0; c < kClasses; ++c) b->chunks[c] = chunks[c];
   b->grid_waves = grid_waves;
-  return make_items(b);
+  return make_all_items(b);
 }

@@ -199,6 +199,8 @@ typedef struct {
   int sigma_update;          /* 1: DUGMA sigma update (one extra measuring pass first) */
   double relax;              /* > 1: over-relax the pose step once annealing has ended
                                 (rotation angle and centroid move times relax) */
+  int coarse_iters;          /* first iterations on every 4th target and source point
+                                (no convergence test there); then the full clouds */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */

@@ -76,11 +76,15 @@ def test_one_step_matches_procrustes(sigma):
         assert out["iters"][k] == 1
 
 
-def test_modes_give_identical_poses():
+@pytest.mark.parametrize("coarse", [0, 9])
+def test_modes_give_identical_poses(coarse):
+    """Graph replay, plain launches and one trial per launch run the same
+    kernels in the same order per trial: bit-identical poses, also across
+    the switch from the every-4th-point level to the full clouds."""
     trials = [small_trial(10 + s, 400 + 97 * s, 350 + 55 * s, deg=30.0, outl=0.1) for s in range(6)]
     b = dg.Batch(trials)
     try:
-        kw = dict(max_iter=25, anneal=True)
+        kw = dict(max_iter=25, anneal=True, coarse_iters=coarse)
         g = b.register(use_graph=True, check_every=5, **kw)
         p = b.register(use_graph=False, **kw)
         o = b.register(one_trial_per_launch=True, **kw)
@@ -106,6 +110,32 @@ def test_easy_trials_converge():
     errs = [W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)]
     assert max(errs) < 5.0, (start, errs)
     assert np.median(errs) < 3.0, errs
+    for k, tr in enumerate(trials):
+        assert np.linalg.norm(out["t"][k] - tr["t_G"]) < 0.05
+
+
+def test_coarse_phase_then_full_clouds():
+    """The first coarse_iters steps run on every 4th point and never stop a
+    trial; the full clouds then reach the same accuracy as without them."""
+    trials = [small_trial(40 + s, 1000, 900, deg=25.0, stretch=(1.6, 1.0, 0.6)) for s in range(8)]
+    b = dg.Batch(trials)
+    try:
+        with pytest.raises(dg.DugmaError, match="coarse_iters"):
+            b.register(max_iter=10, coarse_iters=-1)
+        loose = dict(max_iter=60, anneal=False, tol_rot=1e-2, tol_trans=1e-2)
+        quick = b.register(coarse_iters=0, **loose)
+        held = b.register(coarse_iters=30, **loose)
+        out = b.register(max_iter=130, coarse_iters=30)
+        ref = b.register(max_iter=100)
+    finally:
+        b.free()
+    # loose tolerances stop every trial early on the full clouds, never
+    # during the coarse phase
+    assert (quick["status"] == 1).all() and (quick["iters"] < 30).all(), quick["iters"]
+    assert (held["iters"] > 30).all(), held["iters"]
+    errs = [W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)]
+    errs_ref = [W.rotation_error_deg(ref["R"][k], tr["R_G"]) for k, tr in enumerate(trials)]
+    assert max(errs) < 5.0, (errs, errs_ref)
     for k, tr in enumerate(trials):
         assert np.linalg.norm(out["t"][k] - tr["t_G"]) < 0.05
 
