@@ -54,6 +54,9 @@ constexpr int kUpd = 256;         // threads of the update kernel (one workgroup
 constexpr int kClasses = 4;       // size classes by source count: < 1000, < 2000, < 4000, more
 constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
 constexpr double kGMin = 1e-6;    // smallest variance scale g
+// variances are held times ln 4 = 2 ln 2, so exp2(-|d|^2 / s') = exp(-|d|^2 / 2s)
+constexpr double kLn4 = 1.3862943611198906;
+constexpr double kLn4Pow1p5 = 1.632236874939246;  // (ln 4)^(3/2): the factor e is short by
 
 static inline int size_class(int ns) { return ns < 1000 ? 0 : ns < 2000 ? 1 : ns < 4000 ? 2 : 3; }
 
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(256) void k_batch_transform(
   r.lx = (float)(v[0] - (double)r.hx);
   r.ly = (float)(v[1] - (double)r.hy);
   r.lz = (float)(v[2] - (double)r.hz);
-  r.s = (float)fma(tr.g, x[3], tr.a);
+  r.s = (float)(fma(tr.g, x[3], tr.a) * kLn4);
   r.pad0 = 0.0f;
   S[j] = r;
 }
@@ -125,7 +128,16 @@ __global__ __launch_bounds__(256) void k_batch_fill(unsigned *__restrict__ p, in
 // Statistics of one item, per target i (lane): W_i = sum_j w_ij,
 // B_i = sum_j w_ij x'_j, E_i = sum_j e_ij with w = e / s_ij,
 // s_ij = g s_i + (g s_j + a).  Item totals: [E, W, sum W_i y_i (3),
-// sum B_i (3), sum y_i B_i^T (9, row a = y_a)].  With SIG the item's
+// sum B_i (3), sum y_i B_i^T (9, row a = y_a)].
+// Every operand of the pair arithmetic that is not a streamed value sits in
+// a VGPR (VALU instructions reading an SGPR or a literal issue at half rate):
+//  - B_i comes from D_i = sum_j w_ij d_ij, d = y - x', as B_i = W_i y_i - D_i;
+//  - variances are held times ln 4 (records and g s_i), so that
+//    exp2(-|d|^2 / s') is the Gaussian exp(-|d|^2 / 2s) with no constant
+//    factor per pair; e and w then carry the constant factors (ln 4)^-3/2
+//    and (ln 4)^-5/2, which the pose step does not see (ratios of sums) and
+//    the update kernel takes out of E.
+// With SIG the item's
 // smallest |d_ij|^2 per target is folded into nn[i] (atomicMin on the bits of
 // a non-negative float, which order like the values).  The source stream is
 // software-pipelined: record j + 1 is requested before record j is used
@@ -156,12 +168,12 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const bool valid = (wave * 64 + lane) < it.tn;
     const Rec3I pt = T[valid ? ti : it.t0];
     const float gs = g * pt.s;
-    double dE = 0.0, dW = 0.0, dB[3] = {0.0, 0.0, 0.0};
+    double dE = 0.0, dW = 0.0, dD[3] = {0.0, 0.0, 0.0};
     float dmin = __builtin_huge_valf();
     const int s_end = it.s0 + it.sn;
     Rec3I qn = S[it.s0];  // it.sn >= 1
     for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
-      float fE = 0.0f, fW = 0.0f, fB0 = 0.0f, fB1 = 0.0f, fB2 = 0.0f;
+      float fE = 0.0f, fW = 0.0f, fD0 = 0.0f, fD1 = 0.0f, fD2 = 0.0f;
       const int j1 = min(j0 + kBSub, s_end);
 #pragma unroll 2
       for (int j = j0; j < j1; ++j) {
@@ -174,24 +186,25 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
         if (SIG) dmin = fminf(dmin, d2);
         const float rs = frsq(gs + q.s);
         const float inv = rs * rs;
-        const float e = fexp2((d2 * inv) * kNegHalfLog2e) * (rs * inv);
+        const float e = fexp2(-(d2 * inv)) * (rs * inv);
         const float w = e * inv;
         fE += e;
         fW += w;
-        fB0 = fmaf(w, q.hx, fB0);
-        fB1 = fmaf(w, q.hy, fB1);
-        fB2 = fmaf(w, q.hz, fB2);
+        fD0 = fmaf(w, dx, fD0);
+        fD1 = fmaf(w, dy, fD1);
+        fD2 = fmaf(w, dz, fD2);
       }
       dE += fE;
       dW += fW;
-      dB[0] += fB0;
-      dB[1] += fB1;
-      dB[2] += fB2;
+      dD[0] += fD0;
+      dD[1] += fD1;
+      dD[2] += fD2;
     }
     if (SIG && valid) atomicMin(nn + ti, __float_as_uint(dmin));
     // item totals, one statistic at a time (few registers live): [E, W,
     // W y (3), B (3), y B^T (9)]
     const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
+    const double dB[3] = {fma(dW, y[0], -dD[0]), fma(dW, y[1], -dD[1]), fma(dW, y[2], -dD[2])};
 #pragma unroll
     for (int a = 0; a < kItemStat; ++a) {
       double v;
@@ -306,7 +319,7 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   for (int i = titem[k]; i < titem[k + 1]; ++i)
     for (int a = 0; a < kItemStat; ++a) st[a] += part[(size_t)i * kStat + a];
   const double W = st[1];
-  tr.E = st[0];
+  tr.E = st[0] * kLn4Pow1p5;  // e was computed with variances times ln 4
   tr.iters += 1;
   if (!(W > 0.0) || !isfinite(W)) {
     tr.status = 3;  // no pair has any weight: nothing to align
@@ -651,7 +664,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
         r.hx = (float)(ty[3 * i] - c[0]);
         r.hy = (float)(ty[3 * i + 1] - c[1]);
         r.hz = (float)(ty[3 * i + 2] - c[2]);
-        r.s = (float)tv[i];
+        r.s = (float)(tv[i] * kLn4);
         if (l == 0) {
           r2 += (double)r.hx * r.hx + (double)r.hy * r.hy + (double)r.hz * r.hz;
           vt += tv[i];
