@@ -94,7 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
     return;
   }
 
-  constexpr int NR = GRAD ? kPosePart : 1;
   double r[kPosePart];
 #pragma unroll
   for (int a = 0; a < kPosePart; ++a) r[a] = 0.0;
@@ -108,21 +107,43 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
         r[0] += dacc[k][0];
     }
   }
+  double *o = out + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart;
+  if (GRAD) {
+    // Workgroup sums of the kPosePart values in a fixed order: every lane's
+    // values go to LDS; thread (a, k) = (tid / 32, tid % 32) adds entries
+    // k, k + 32, ..., k + 224 of value a (consecutive lanes read consecutive
+    // doubles: no bank conflict); each half wave then sums its 32 partials
+    // (DPP row rotations, then the two row sums), value 2 * wave in the low
+    // half and 2 * wave + 1 in the high half.
+    static_assert(kPosePart * 32 == kBlock, "one half wave per pose value");
+    __shared__ double stage[kPosePart][kBlock];
 #pragma unroll
-  for (int a = 0; a < NR; ++a) r[a] = wave_sum(r[a]);
-  __shared__ double red[kBlock / 64][kPosePart];
-  if (lane == 0)
-#pragma unroll
-    for (int a = 0; a < NR; ++a) red[wave][a] = r[a];
-  __syncthreads();
-  if (threadIdx.x < kPosePart) {
+    for (int a = 0; a < kPosePart; ++a) stage[a][threadIdx.x] = r[a];
+    __syncthreads();
+    const int a = threadIdx.x >> 5, k = threadIdx.x & 31;
     double s = 0.0;
-    if ((int)threadIdx.x < NR) {
-      s = red[0][threadIdx.x];
 #pragma unroll
-      for (int w = 1; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    for (int i = 0; i < kBlock / 32; ++i) s += stage[a][k + 32 * i];
+    double lo, hi;
+    half_wave_sums(s, lo, hi);
+    if (lane == 0) {
+      o[2 * wave] = lo;
+      o[2 * wave + 1] = hi;
     }
-    out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart + threadIdx.x] = s;
+  } else {
+    r[0] = wave_sum(r[0]);
+    __shared__ double red[kBlock / 64];
+    if (lane == 0) red[wave] = r[0];
+    __syncthreads();
+    if (threadIdx.x < kPosePart) {
+      double s = 0.0;
+      if (threadIdx.x == 0) {
+        s = red[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) s += red[w];
+      }
+      o[threadIdx.x] = s;
+    }
   }
 }
 
@@ -131,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(
 // order.  The result goes straight to the context's pinned host slot.
 __global__ __launch_bounds__(1024) void k_reduce_parts(
     const double *__restrict__ parts, int nparts, int width,
-    double *__restrict__ res) {
+    double *__restrict__ res, unsigned long long *__restrict__ done, unsigned long long seq) {
   __shared__ double sh[1024 / 64][kPosePart];
   double s[kPosePart];
 #pragma unroll
@@ -154,6 +175,7 @@ __global__ __launch_bounds__(1024) void k_reduce_parts(
     for (int w = 0; w < 1024 / 64; ++w) t += sh[w][threadIdx.x];
     res[threadIdx.x] = t;
   }
+  if (done) publish_done(done, seq);
 }
 
 // Per-point sums over streamed chunks, in chunk order.
@@ -418,10 +440,19 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  if (!host_sum)
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
+  unsigned long long seq = 0;
+  if (!host_sum) {
+    seq = next_seq(c);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres,
+                       done_slot(c), seq);
+  }
   DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  if (host_sum) {
+    DUGMA_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    rc = wait_done(c, seq);
+    if (rc) return rc;
+  }
   timer.finish();
   double sum[kPosePart] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (host_sum) {

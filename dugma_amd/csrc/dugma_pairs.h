@@ -45,6 +45,23 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Sums of v over the low and the high 32 lanes of a wave, in every lane:
+// DPP row rotations by 8, 4, 2, 1 sum each row of 16 lanes, then rows 0 + 1
+// and rows 2 + 3 are added from lanes 0, 16, 32, 48 (fixed order).
+__device__ __forceinline__ void half_wave_sums(double v, double &lo, double &hi) {
+  v += dpp_move<0x128>(v);  // row_ror:8
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x122>(v);  // row_ror:2
+  v += dpp_move<0x121>(v);  // row_ror:1
+  const int l = __double2loint(v), h = __double2hiint(v);
+  const double r0 = __hiloint2double(__builtin_amdgcn_readlane(h, 0), __builtin_amdgcn_readlane(l, 0));
+  const double r1 = __hiloint2double(__builtin_amdgcn_readlane(h, 16), __builtin_amdgcn_readlane(l, 16));
+  const double r2 = __hiloint2double(__builtin_amdgcn_readlane(h, 32), __builtin_amdgcn_readlane(l, 32));
+  const double r3 = __hiloint2double(__builtin_amdgcn_readlane(h, 48), __builtin_amdgcn_readlane(l, 48));
+  lo = r0 + r1;
+  hi = r2 + r3;
+}
+
 template <class Rec> struct RecTraits;
 template <> struct RecTraits<Rec3F> { static constexpr int D = 3, NACC = kAcc3F; static constexpr bool FULL = true; };
 template <> struct RecTraits<Rec3I> { static constexpr int D = 3, NACC = kAcc3I; static constexpr bool FULL = false; };
