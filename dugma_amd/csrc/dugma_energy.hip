@@ -33,8 +33,11 @@ struct PoseArg {
 // ---------------------------------------------------------------------------
 // Pair kernel
 // ---------------------------------------------------------------------------
+// With one stationary point per lane every variant fits in 64 VGPRs (no
+// scratch), which allows 8 waves per SIMD instead of 6; two per lane need
+// more registers and keep the compiler's default.
 template <class Rec, int MODE, int T, int COMP>
-__global__ __launch_bounds__(kBlock) void k_pairs(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ? 8 : 1, 8))) void k_pairs(
     const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
     int qchunk, double *__restrict__ out, double t0, double t1, double t2) {
   using Tr = RecTraits<Rec>;
