@@ -99,3 +99,35 @@ def test_batch_argument_errors_before_any_device_call():
     assert L.dugma_batch_free(None) == -1
     ch = (ctypes.c_int * 4)(1, 1, 1, 1)
     assert L.dugma_batch_set_shape(None, 2, ch, 8192) == -1
+
+
+def test_option_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of the option structs have the C layout: field
+    offsets and sizes from a C program compiled against include/dugma.h."""
+    import shutil
+    import subprocess
+    from dugma_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"dugma_batch_opts": _lib.BatchOpts, "dugma_register_opts": _lib.RegisterOpts}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "dugma.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    c = {}
+    for line in out:
+        if line:
+            name, field, value = line.split()
+            c[(name, field)] = int(value)
+    for cname, py in structs.items():
+        assert c[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert c[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
