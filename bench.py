@@ -26,12 +26,22 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector, AMD public spec
-# canonical algorithmic FLOP per pair (FMA = 2, exp/rsq/rcp = 1)
-FLOP_3D_FULL_FUSED = 107
-FLOP_3D_FULL_FWD = 59
-FLOP_3D_ISO_FUSED = 36
-FLOP_3D_ISO_FWD = 17
+
+
+def roofline_constants():
+    """The #define values of include/dugma_roofline.h: canonical algorithmic
+    FLOP per pair (FMA = 2, exp/rsq/rcp = 1) and the FP32 vector peak."""
+    import re
+    text = open(os.path.join(ROOT, "include", "dugma_roofline.h")).read()
+    return {k: float(v) for k, v in re.findall(r"^#define DUGMA_(\w+) ([0-9.]+)\s*$", text, re.M)}
+
+
+_RC = roofline_constants()
+PEAK_FP32_TFLOPS = _RC["PEAK_FP32_TFLOPS"]  # MI355X dense FP32 vector, AMD public spec
+FLOP_3D_FULL_FUSED = int(_RC["FLOP_3D_FULL_FUSED"])
+FLOP_3D_FULL_FWD = int(_RC["FLOP_3D_FULL_FWD"])
+FLOP_3D_ISO_FUSED = int(_RC["FLOP_3D_ISO_FUSED"])
+FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # HBM bytes per k_pairs launch (fused, MODE 1) on the 6,000 x 6,000
 # full-covariance workload, keyed by launch shape (lane_points, target_waves):
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by what the
