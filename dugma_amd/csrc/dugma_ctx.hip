@@ -1,8 +1,6 @@
 // dugma_ctx.hip -- context lifetime: init (init.cpp / init.cu), fr_m
 // (fr_m.cpp / fr_m.cu), error reporting and the energy-path point records.
 #include <algorithm>
-#include <atomic>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -61,32 +59,6 @@ void PairTimer::finish() {
     c->pair_ms += ms;
     c->pair_launches += 1;
   }
-}
-
-unsigned long long next_seq(dugma_ctx *c) { return ++c->seq; }
-
-unsigned long long *done_slot(dugma_ctx *c) {
-  return reinterpret_cast<unsigned long long *>(c->dres + dugma_ctx::kResDone);
-}
-
-int wait_done(dugma_ctx *c, unsigned long long seq) {
-  if (!c->timing) {
-    const volatile unsigned long long *f =
-        reinterpret_cast<const volatile unsigned long long *>(c->hres + dugma_ctx::kResDone);
-    const auto t0 = std::chrono::steady_clock::now();
-    for (unsigned it = 1;; ++it) {
-      if (*f == seq) {
-        std::atomic_thread_fence(std::memory_order_acquire);
-        return DUGMA_OK;
-      }
-      if ((it & 1023u) == 0 &&
-          std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kSpinUs))
-        break;
-      __builtin_ia32_pause();
-    }
-  }
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
-  return DUGMA_OK;
 }
 
 void DevBuf::release() {
@@ -307,15 +279,13 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     if (e != hipSuccess) TRY(hip_fail(e, "hipGetDevice"));
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) TRY(hip_fail(e, "hipStreamCreateWithFlags"));
-    // coherent: the host polls the completion slot while kernels run
     e = hipHostMalloc((void **)&c->hpinned, sizeof(double) * dugma_ctx::kHostStage,
-                      hipHostMallocCoherent);
+                      hipHostMallocDefault);
     if (e != hipSuccess) {
       c->hpinned = nullptr;
       TRY(hip_fail(e, "hipHostMalloc"));
     }
     c->hres = c->hpinned + dugma_ctx::kHostStage / 2;
-    *reinterpret_cast<volatile unsigned long long *>(c->hres + dugma_ctx::kResDone) = 0ull;
     e = hipHostGetDevicePointer((void **)&c->dres, c->hres, 0);
     if (e != hipSuccess) TRY(hip_fail(e, "hipHostGetDevicePointer"));
     e = hipHostMalloc((void **)&c->hparts, sizeof(double) * kPosePart * dugma_ctx::kHostParts,

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
 // order.  The result goes straight to the context's pinned host slot.
 __global__ __launch_bounds__(1024) void k_reduce_parts(
     const double *__restrict__ parts, int nparts, int width,
-    double *__restrict__ res, unsigned long long *__restrict__ done, unsigned long long seq) {
+    double *__restrict__ res) {
   __shared__ double sh[1024 / 64][kPosePart];
   double s[kPosePart];
 #pragma unroll
@@ -178,7 +178,6 @@ __global__ __launch_bounds__(1024) void k_reduce_parts(
     for (int w = 0; w < 1024 / 64; ++w) t += sh[w][threadIdx.x];
     res[threadIdx.x] = t;
   }
-  if (done) publish_done(done, seq);
 }
 
 // Per-point sums over streamed chunks, in chunk order.
@@ -443,19 +442,10 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  unsigned long long seq = 0;
-  if (!host_sum) {
-    seq = next_seq(c);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres,
-                       done_slot(c), seq);
-  }
+  if (!host_sum)
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
   DUGMA_HIP(hipGetLastError());
-  if (host_sum) {
-    DUGMA_HIP(hipStreamSynchronize(c->stream));
-  } else {
-    rc = wait_done(c, seq);
-    if (rc) return rc;
-  }
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
   timer.finish();
   double sum[kPosePart] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (host_sum) {

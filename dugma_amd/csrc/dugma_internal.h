@@ -71,20 +71,6 @@ struct PairTimer {
   void finish();
 };
 
-// Completion of a call whose results one workgroup writes to the context's
-// pinned result slots: after its result stores, every thread of that
-// workgroup calls publish_done, which stores the call's sequence number to
-// hres[kResDone] with system-scope release ordering.  The host side,
-// wait_done, spins on that slot for up to kSpinUs and then falls back to
-// hipStreamSynchronize, which also reports a failed launch.  The spin saves
-// the wake-up latency of a blocking synchronisation on short calls.
-__device__ __forceinline__ void publish_done(unsigned long long *flag, unsigned long long seq) {
-  __threadfence_system();
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-constexpr long kSpinUs = 2000;
-
 // Grow-only device scratch buffer owned by a context.
 struct DevBuf {
   void *ptr = nullptr;
@@ -133,10 +119,6 @@ struct dugma_ctx {
   double *hres = nullptr;
   double *dres = nullptr;        // device address of hres
   static constexpr int kResSig = 8, kResMoments = 64;
-  // hres[kResDone], read as a u64: sequence number of the last call whose
-  // results are complete (see wait_done); seq counts the calls that use it
-  static constexpr int kResDone = 32;
-  unsigned long long seq = 0;
 
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc
@@ -167,9 +149,3 @@ struct dugma_ctx {
   static constexpr int kHostParts = 256;
 };
 
-namespace dugma {
-// Next sequence number of c and the device address of its completion slot.
-unsigned long long next_seq(dugma_ctx *c);
-unsigned long long *done_slot(dugma_ctx *c);
-int wait_done(dugma_ctx *c, unsigned long long seq);
-}  // namespace dugma

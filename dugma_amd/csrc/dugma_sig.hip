@@ -103,9 +103,7 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
 // context's pinned result slot).
 __global__ __launch_bounds__(1024) void k_tree_sum_div(const unsigned *__restrict__ q,
                                                        float *__restrict__ a, int n,
-                                                       float *__restrict__ out,
-                                                       unsigned long long *__restrict__ done,
-                                                       unsigned long long seq) {
+                                                       float *__restrict__ out) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) a[k] = ((float)q[k]) / 1000000.0f;
   __syncthreads();
   long L = n, stride = 1;
@@ -119,7 +117,6 @@ __global__ __launch_bounds__(1024) void k_tree_sum_div(const unsigned *__restric
     stride *= 2;
   }
   if (threadIdx.x == 0) out[0] = a[0] / (float)n;
-  publish_done(done, seq);
 }
 
 // ---------------------------------------------------------------------------
@@ -280,12 +277,10 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
     hipLaunchKernelGGL(k_nn1_quant<3>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
   else
     hipLaunchKernelGGL(k_nn1_quant<2>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
-  const unsigned long long seq = next_seq(c);
   hipLaunchKernelGGL(k_tree_sum_div, dim3(1), dim3(1024), 0, c->stream, q, f, c->N,
-                     (float *)(c->dres + dugma_ctx::kResSig), done_slot(c), seq);
+                     (float *)(c->dres + dugma_ctx::kResSig));
   DUGMA_HIP(hipGetLastError());
-  rc = wait_done(c, seq);
-  if (rc) return rc;
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
   const volatile float *hs = (const volatile float *)(c->hres + dugma_ctx::kResSig);
   *sig = (double)hs[0];
   return DUGMA_OK;
