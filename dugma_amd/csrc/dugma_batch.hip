@@ -1045,7 +1045,11 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     }
   }
   b->grid_waves = best_g;
-  if (!rc) rc = make_items(b, b->lv[1]);  // the coarse level follows the chosen shape
+  {
+    // both levels follow the kept shape, also after a failed timing run
+    const int rc2 = make_all_items(b);
+    if (!rc) rc = rc2;
+  }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   if (rc) return rc;
