@@ -1,7 +1,10 @@
 // dugma_ctx.hip -- context lifetime: init (init.cpp / init.cu), fr_m
 // (fr_m.cpp / fr_m.cu), error reporting and the energy-path point records.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -60,6 +63,39 @@ void PairTimer::finish() {
     c->pair_launches += 1;
   }
 }
+
+template <class F>
+static void preset_nan(F *slots, int n) {
+  volatile F *v = slots;
+  for (int a = 0; a < n; ++a) v[a] = std::numeric_limits<F>::quiet_NaN();
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+}
+void preset_slots(double *slots, int n) { preset_nan(slots, n); }
+void preset_slots(float *slots, int n) { preset_nan(slots, n); }
+
+template <class F>
+static int wait_nan(dugma_ctx *c, const F *slots, int n) {
+  if (!c->timing) {
+    const volatile F *v = slots;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 1;; ++it) {
+      int set = 0;
+      for (int a = 0; a < n; ++a) set += v[a] == v[a] ? 1 : 0;  // not NaN
+      if (set == n) {
+        std::atomic_thread_fence(std::memory_order_acquire);
+        return DUGMA_OK;
+      }
+      if ((it & 1023u) == 0 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kSpinUs))
+        break;
+      __builtin_ia32_pause();
+    }
+  }
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  return DUGMA_OK;
+}
+int wait_slots(dugma_ctx *c, const double *slots, int n) { return wait_nan(c, slots, n); }
+int wait_slots(dugma_ctx *c, const float *slots, int n) { return wait_nan(c, slots, n); }
 
 void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
@@ -279,8 +315,9 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     if (e != hipSuccess) TRY(hip_fail(e, "hipGetDevice"));
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) TRY(hip_fail(e, "hipStreamCreateWithFlags"));
+    // coherent: result slots are polled by the host while kernels run
     e = hipHostMalloc((void **)&c->hpinned, sizeof(double) * dugma_ctx::kHostStage,
-                      hipHostMallocDefault);
+                      hipHostMallocCoherent);
     if (e != hipSuccess) {
       c->hpinned = nullptr;
       TRY(hip_fail(e, "hipHostMalloc"));

@@ -442,10 +442,17 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  if (!host_sum)
+  if (!host_sum) {
+    preset_slots(c->hres, width);
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
+  }
   DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  if (host_sum) {
+    DUGMA_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    rc = wait_slots(c, c->hres, width);
+    if (rc) return rc;
+  }
   timer.finish();
   double sum[kPosePart] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (host_sum) {

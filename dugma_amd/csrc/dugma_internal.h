@@ -71,6 +71,20 @@ struct PairTimer {
   void finish();
 };
 
+// Waiting for results that a call's last kernel stores to the context's
+// pinned result slots (coherent host memory, written without the GPU's L2):
+// the host presets the slots to a quiet NaN before the launch and spins until
+// none is NaN, for up to kSpinUs, then falls back to hipStreamSynchronize
+// (which also reports a failed launch, and covers results that are NaN).
+// Each slot is one aligned 8-byte (or 4-byte) store, so a slot that is no
+// longer NaN holds its final value.  The spin saves the wake-up latency of a
+// blocking synchronisation on short calls.
+constexpr long kSpinUs = 2000;
+void preset_slots(double *slots, int n);
+void preset_slots(float *slots, int n);
+int wait_slots(dugma_ctx *c, const double *slots, int n);
+int wait_slots(dugma_ctx *c, const float *slots, int n);
+
 // Grow-only device scratch buffer owned by a context.
 struct DevBuf {
   void *ptr = nullptr;

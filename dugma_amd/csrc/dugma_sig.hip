@@ -277,10 +277,13 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
     hipLaunchKernelGGL(k_nn1_quant<3>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
   else
     hipLaunchKernelGGL(k_nn1_quant<2>, grid, dim3(kSBlock), 0, c->stream, c->dX, c->N, m, c->M, chunk, q);
+  float *slot = (float *)(c->hres + dugma_ctx::kResSig);
+  preset_slots(slot, 1);
   hipLaunchKernelGGL(k_tree_sum_div, dim3(1), dim3(1024), 0, c->stream, q, f, c->N,
                      (float *)(c->dres + dugma_ctx::kResSig));
   DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  rc = wait_slots(c, slot, 1);
+  if (rc) return rc;
   const volatile float *hs = (const volatile float *)(c->hres + dugma_ctx::kResSig);
   *sig = (double)hs[0];
   return DUGMA_OK;
