@@ -713,11 +713,14 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
     hipLaunchKernelGGL(k_compi_mom<2>, dim3(mblocks), blk, 0, c->stream, wpart, m, nchunks, src, feat, nmom, tab, mpart);
   }
   // moment sums straight into the context's pinned result slots
+  double *slots = c->hres + dugma_ctx::kResMoments;
+  preset_slots(slots, nmom);
   hipLaunchKernelGGL(k_compi_msum, dim3((nmom + kCWaves - 1) / kCWaves), blk, 0, c->stream, mpart, mblocks, nmom,
                      c->dres + dugma_ctx::kResMoments);
   DUGMA_HIP(hipGetLastError());
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
-  const volatile double *hm = c->hres + dugma_ctx::kResMoments;
+  const int rcw = wait_slots(c, slots, nmom);
+  if (rcw) return rcw;
+  const volatile double *hm = slots;
 
   const int nm = Dd == 3 ? DUGMA_NMONO3 : DUGMA_NMONO2;
   for (int k = 0; k < nm; ++k) {
