@@ -45,12 +45,39 @@ def _cov_mode(mode):
     return _COV[mode] if isinstance(mode, str) else int(mode)
 
 
+class _PoseBuffers:
+    """Per-context argument and result buffers of the per-pose calls, with
+    their ctypes pointers made once (building a pointer from a numpy array
+    costs more per call than the copy of R and t).  A context is not
+    re-entrant, so one set serves every call on it."""
+
+    def __init__(self, D):
+        self.D = D
+        self.R = np.zeros((D, D), order="F")
+        self.t = np.zeros(D)
+        self.g = np.zeros(6 if D == 3 else 3)
+        self.E = ctypes.c_double()
+        self.Rp, self.tp, self.gp = _dp(self.R), _dp(self.t), _dp(self.g)
+        self.Ep = ctypes.byref(self.E)
+
+    def pose(self, R, t):
+        R = np.asarray(R, dtype=np.float64)
+        if R.shape != (self.D, self.D):
+            raise ValueError("expected shape %s, got %s" % ((self.D, self.D), R.shape))
+        t = np.ravel(np.asarray(t, dtype=np.float64))
+        if t.shape != (self.D,):
+            raise ValueError("expected %d translation entries, got %d" % (self.D, t.size))
+        self.R[...] = R
+        self.t[...] = t
+
+
 class Context:
     """The device state created by ``init`` (the reference's six handles)."""
 
     def __init__(self, ptr, N, M, D):
         self._ptr = ctypes.c_void_p(ptr)
         self.N, self.M, self.D = N, M, D
+        self._pb = _PoseBuffers(D)
 
     @property
     def ptr(self):
@@ -120,20 +147,19 @@ def init(X, X_C, Y, Y_C):
 
 def update_sig(h, R, t, N=None, M=None, D=None):
     """sig = update_sig(h, R, t, N, M, D) (FP32 value returned as a float)."""
-    R = _cm(R, (h.D, h.D))
-    t = _cm(np.ravel(t))
-    sig = ctypes.c_double()
-    _lib.check(_lib.load().dugma_update_sig(h.ptr, _dp(R), _dp(t), float(N or h.N),
-                                            float(M or h.M), float(D or h.D), ctypes.byref(sig)))
-    return sig.value
+    b = h._pb
+    b.pose(R, t)
+    _lib.check(_lib.load().dugma_update_sig(h.ptr, b.Rp, b.tp, float(N or h.N),
+                                            float(M or h.M), float(D or h.D), b.Ep))
+    return b.E.value
 
 
 def compi(h, R, t, N=None, M=None, D=None, sig=1.0):
     """coef = compi(h, R, t, N, M, D, sig): 637 (3-D) or 45 (2-D) coefficients."""
-    R = _cm(R, (h.D, h.D))
-    t = _cm(np.ravel(t))
+    b = h._pb
+    b.pose(R, t)
     coef = np.zeros(637 if h.D == 3 else 45)
-    _lib.check(_lib.load().dugma_compi(h.ptr, _dp(R), _dp(t), float(N or h.N), float(M or h.M),
+    _lib.check(_lib.load().dugma_compi(h.ptr, b.Rp, b.tp, float(N or h.N), float(M or h.M),
                                        float(D or h.D), float(sig), _dp(coef)))
     return coef
 
@@ -188,22 +214,18 @@ def DUGMA(X, Y, X_C, Y_C, R0, t0, Max_Iteration=300, accuracy_rotation=1e-4,
 
 
 def energy(h, R, t, cov="auto"):
-    R = _cm(R, (h.D, h.D))
-    t = _cm(np.ravel(t))
-    E = ctypes.c_double()
-    _lib.check(_lib.load().dugma_energy(h.ptr, _dp(R), _dp(t), _cov_mode(cov), ctypes.byref(E)))
-    return E.value
+    b = h._pb
+    b.pose(R, t)
+    _lib.check(_lib.load().dugma_energy(h.ptr, b.Rp, b.tp, _cov_mode(cov), b.Ep))
+    return b.E.value
 
 
 def energy_grad(h, R, t, cov="auto"):
     """(E, gpose): gpose = [dE/dw (3), dE/dt (3)] (3-D) or [dE/dtheta, dE/dt (2)]."""
-    R = _cm(R, (h.D, h.D))
-    t = _cm(np.ravel(t))
-    E = ctypes.c_double()
-    g = np.zeros(6 if h.D == 3 else 3)
-    _lib.check(_lib.load().dugma_energy_grad(h.ptr, _dp(R), _dp(t), _cov_mode(cov),
-                                             ctypes.byref(E), _dp(g)))
-    return E.value, g
+    b = h._pb
+    b.pose(R, t)
+    _lib.check(_lib.load().dugma_energy_grad(h.ptr, b.Rp, b.tp, _cov_mode(cov), b.Ep, b.gp))
+    return b.E.value, b.g.copy()
 
 
 def energy_grad_fd(h, R, t, step=1e-4, cov="auto"):
