@@ -150,33 +150,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
   }
 }
 
-// Fixed-order sum of the per-workgroup partials (one workgroup of 1024):
-// strided loads per thread, a DPP sum per wave, then the 16 wave sums in
-// order.  The result goes straight to the context's pinned host slot.
-__global__ __launch_bounds__(1024) void k_reduce_parts(
-    const double *__restrict__ parts, int nparts, int width,
-    double *__restrict__ res) {
-  __shared__ double sh[1024 / 64][kPosePart];
+// Fixed-order sum of the per-workgroup partials over kRedGroups workgroups
+// at most: workgroup g sums the contiguous range [g len, (g + 1) len) --
+// thread t adds partials t, t + 256, ... of it, then the 256 lanes' values
+// are summed through LDS and half-wave sums as in the pair kernel -- and
+// writes its kPosePart sums to res[kPosePart g, ...), the context's pinned
+// host slots; the host adds the workgroups' sums in order.
+constexpr int kRedGroups = 8;
+__global__ __launch_bounds__(kBlock) void k_reduce_parts(
+    const double *__restrict__ parts, int nparts, double *__restrict__ res) {
+  const int len = (nparts + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int q0 = blockIdx.x * len, q1 = min(q0 + len, nparts);
   double s[kPosePart];
 #pragma unroll
   for (int a = 0; a < kPosePart; ++a) s[a] = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += 1024) {
-    const double4 *p = reinterpret_cast<const double4 *>(parts + (size_t)k * kPosePart);
+  for (int q = q0 + (int)threadIdx.x; q < q1; q += kBlock) {
+    const double4 *p = reinterpret_cast<const double4 *>(parts + (size_t)q * kPosePart);
     const double4 u = p[0], v = p[1];
     s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
     s[4] += v.x; s[5] += v.y; s[6] += v.z; s[7] += v.w;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double stage[kPosePart][kBlock];
 #pragma unroll
-  for (int a = 0; a < kPosePart; ++a) {
-    const double w = wave_sum(s[a]);
-    if (lane == 0) sh[wave][a] = w;
-  }
+  for (int a = 0; a < kPosePart; ++a) stage[a][threadIdx.x] = s[a];
   __syncthreads();
-  if ((int)threadIdx.x < width) {
-    double t = 0.0;
-    for (int w = 0; w < 1024 / 64; ++w) t += sh[w][threadIdx.x];
-    res[threadIdx.x] = t;
+  const int a = threadIdx.x >> 5, k = threadIdx.x & 31;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 32; ++i) t += stage[a][k + 32 * i];
+  double lo, hi;
+  half_wave_sums(t, lo, hi);
+  if (lane == 0) {
+    res[kPosePart * blockIdx.x + 2 * wave] = lo;
+    res[kPosePart * blockIdx.x + 2 * wave + 1] = hi;
   }
 }
 
@@ -442,15 +449,16 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
+  const int groups = (int)std::min<size_t>(kRedGroups, (nparts + kBlock - 1) / kBlock);
   if (!host_sum) {
-    preset_slots(c->hres, width);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1024), 0, c->stream, parts, (int)nparts, width, c->dres);
+    preset_slots(c->hres, kPosePart * groups);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(groups), dim3(kBlock), 0, c->stream, parts, (int)nparts, c->dres);
   }
   DUGMA_HIP(hipGetLastError());
   if (host_sum) {
     DUGMA_HIP(hipStreamSynchronize(c->stream));
   } else {
-    rc = wait_slots(c, c->hres, width);
+    rc = wait_slots(c, c->hres, kPosePart * groups);
     if (rc) return rc;
   }
   timer.finish();
@@ -461,7 +469,8 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
       for (int a = 0; a < width; ++a) sum[a] += hp[q * kPosePart + a];
   } else {
     const volatile double *hr = c->hres;
-    for (int a = 0; a < width; ++a) sum[a] = hr[a];
+    for (int g = 0; g < groups; ++g)
+      for (int a = 0; a < width; ++a) sum[a] += hr[kPosePart * g + a];
   }
   // unscale: E ~ s^D, rotation gradient ~ s^D, translation gradient ~ s^(D+1)
   const double cn = gauss_norm(D);

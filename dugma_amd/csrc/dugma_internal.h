@@ -127,12 +127,12 @@ struct dugma_ctx {
   // so that covariances are O(1) in FP32 whatever the caller's units.
   int lscale = 0;
   double scale = 1.0;
-  // pinned host slots kernels write their final results to (read after the
-  // stream sync): [0, 8) pose-pass sums, [8] the update_sig value (as a
-  // float), [kResMoments, kResMoments + 400) the compi moments
+  // pinned host slots kernels write their final results to: [0, 64) the
+  // pose-pass sums (8 per reducing workgroup), [kResMoments, kResMoments +
+  // 400) the compi moments, [kResSig] the update_sig value (as a float)
   double *hres = nullptr;
   double *dres = nullptr;        // device address of hres
-  static constexpr int kResSig = 8, kResMoments = 64;
+  static constexpr int kResMoments = 64, kResSig = 512;
 
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc
