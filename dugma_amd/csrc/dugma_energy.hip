@@ -416,21 +416,26 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   const int width = mode == kModeEnergy ? 1 : (D == 3 ? 7 : 4);
   for (int a = 0; a < width; ++a) res[a] = 0.0;
   if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
+  // everything the launches need is settled first, so that the three
+  // launches leave the host back to back (the transform takes ~3 us)
   PoseArg ps = make_pose(c, R, t);
-  int rc = transform_sources(c, ps, iso);
-  if (rc) return rc;
   Plan pl = ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
   // few partials: the pair kernel writes them to pinned host memory and the
-  // host sums them after the sync; many: one device reduction launch
+  // host sums them after the sync; many: a device reduction launch
   const bool host_sum = nparts <= (size_t)dugma_ctx::kHostParts;
   double *parts = c->dparts;
+  int rc;
   if (!host_sum) {
     rc = c->part.ensure(nparts * kPosePart * sizeof(double));
     if (rc) return rc;
     parts = (double *)c->part.ptr;
   }
   const int comp = comp_enabled() ? 1 : 0;
+  const int groups = (int)std::min<size_t>(kRedGroups, (nparts + kBlock - 1) / kBlock);
+  if (!host_sum) preset_slots(c->hres, kPosePart * groups);
+  rc = transform_sources(c, ps, iso);
+  if (rc) return rc;
   PairTimer timer(c);
   if (D == 3) {
     if (iso)
@@ -449,9 +454,7 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
-  const int groups = (int)std::min<size_t>(kRedGroups, (nparts + kBlock - 1) / kBlock);
   if (!host_sum) {
-    preset_slots(c->hres, kPosePart * groups);
     hipLaunchKernelGGL(k_reduce_parts, dim3(groups), dim3(kBlock), 0, c->stream, parts, (int)nparts, c->dres);
   }
   DUGMA_HIP(hipGetLastError());
