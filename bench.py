@@ -46,9 +46,9 @@ FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # full-covariance workload, keyed by launch shape (lane_points, target_waves):
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by what the
 # counter reads for a 256 MiB copy (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
-MEASURED_TRAFFIC_6K_FULL = {(1, 4096): 6688000, (1, 8192): 6754000, (1, 16384): 6766000,
-                            (1, 32768): 6766000, (2, 4096): 7101000, (2, 8192): 7108000,
-                            (2, 16384): 7108000, (2, 32768): 7108000}
+MEASURED_TRAFFIC_6K_FULL = {(1, 4096): 6662000, (1, 8192): 6727000, (1, 16384): 6740000,
+                            (1, 32768): 6740000, (2, 4096): 7057000, (2, 8192): 7064000,
+                            (2, 16384): 7064000, (2, 32768): 7063000}
 
 
 class Hip:
