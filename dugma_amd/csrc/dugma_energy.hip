@@ -5,7 +5,10 @@
 // (targets, or sources in the per-point source pass) in VGPRs and walks a
 // chunk of the other cloud.  The streamed records are read with wave-uniform
 // addresses, so they come through the scalar cache into SGPRs and feed the
-// VALU directly: no LDS, no barriers in the pair loop.  Per-pair terms are
+// VALU directly: no LDS, no barriers in the pair loop.  (On gfx950 an
+// instruction with an SGPR operand issues at about half the rate of an
+// all-VGPR one, but staging the records in LDS and reading them into VGPRs
+// was slower: the wave-wide LDS reads become the bound.)  Per-pair terms are
 // FP32, summed in FP32 over sub-tiles of kSub streamed points and then in
 // FP64 per lane; workgroup partials are reduced in a fixed order, so results
 // are deterministic run to run.  Nothing of size N x M is stored.

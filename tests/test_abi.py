@@ -131,3 +131,24 @@ def test_option_structs_match_the_header(tmp_path):
         assert c[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert c[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_pose_buffers_check_shapes():
+    """The per-context pose buffers of the Python mirror take R (D x D) and t
+    (D entries) only, and copy them (column-major R as the C ABI expects)."""
+    import numpy as np
+    import dugma_amd as dg
+    b = dg._PoseBuffers(3)
+    R = np.arange(9.0).reshape(3, 3)
+    b.pose(R, [1.0, 2.0, 3.0])
+    assert b.R.flags.f_contiguous
+    np.testing.assert_array_equal(b.R, R)
+    np.testing.assert_array_equal(np.frombuffer(b.R.tobytes(order="A")), R.T.ravel())
+    np.testing.assert_array_equal(b.t, [1.0, 2.0, 3.0])
+    with pytest.raises(ValueError):
+        b.pose(np.eye(2), [0.0, 0.0, 0.0])
+    with pytest.raises(ValueError):
+        b.pose(np.eye(3), [0.0, 0.0])
+    b2 = dg._PoseBuffers(2)
+    b2.pose(np.eye(2), np.array([[1.0], [2.0]]))
+    np.testing.assert_array_equal(b2.t, [1.0, 2.0])
