@@ -208,6 +208,31 @@ def test_deterministic_repeats():
         dg.fr_m(h)
 
 
+def test_reduced_pass_result_paths_agree():
+    """Above 256 workgroup partials the pose pass reduces on the device over
+    several workgroups and the host polls the pinned result slots; with the
+    kernel timing on it synchronises the stream instead.  Both paths and
+    repeated calls give bit-identical results, and the energy-only entry
+    point gives the same energy."""
+    d = W.config2(n=3000, m=3000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = perturbed_pose(d, seed=2)
+        a = dg.energy_grad(h, R, t)
+        b = dg.energy_grad(h, R, t)
+        h.set_timing(True)
+        c = dg.energy_grad(h, R, t)
+        ms, launches = h.pair_time()
+        h.set_timing(False)
+        e = dg.energy(h, R, t)
+    finally:
+        dg.fr_m(h)
+    assert launches == 1 and ms > 0.0
+    for x in (b, c):
+        assert x[0] == a[0] and (x[1] == a[1]).all()
+    assert math.isclose(e, a[0], rel_tol=1e-6)
+
+
 def test_depth_views_full_size_adjoint_identities():
     """Configuration 5, 3-D: two 320 x 240 depth views (about 76,800 points
     each, full 3 x 3 covariances, invalid pixels left out) at full size,
