@@ -33,6 +33,12 @@
 // captured once into a hipGraph and replayed.  The same kernels run one
 // trial per launch for the baseline.
 //
+// Two resolutions: the first coarse_iters iterations run on every 4th target
+// and source point of each trial (1/16 of the pairs; same centroid and
+// variances), never declaring convergence; then the full clouds take over,
+// after one pass that measures sig again on them.  Each resolution has its
+// own work items and its own captured iteration.
+//
 // Launch shape: waves per workgroup (target tile = 64 x waves), the number
 // of source chunks per target tile for each size class of trials (by source
 // count), and the resident waves of the persistent kernel are chosen by
@@ -123,6 +129,13 @@ __global__ __launch_bounds__(256) void k_batch_transform(
 __global__ __launch_bounds__(256) void k_batch_fill(unsigned *__restrict__ p, int n, unsigned v) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Marks every running trial's sigma as unmeasured, so the next pass (on the
+// other level) measures it again before moving the pose.
+__global__ __launch_bounds__(256) void k_batch_resig(BTrial *__restrict__ trials, int K) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < K) trials[k].sig_ready = 0;
 }
 
 // Statistics of one item, per target i (lane): W_i = sum_j w_ij,
@@ -797,13 +810,6 @@ static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams
                      k1, b->dtrials, b->ddone, b->dqueue, L.dnn, bp);
   DUGMA_HIP(hipGetLastError());
   return DUGMA_OK;
-}
-
-// Marks every running trial's sigma as unmeasured, so the next pass (on the
-// other level) measures it again before moving the pose.
-__global__ __launch_bounds__(256) void k_batch_resig(BTrial *__restrict__ trials, int K) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k < K) trials[k].sig_ready = 0;
 }
 
 // Trial states at the start of a run, uploaded with every done flag clear
