@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "dugma_internal.h"
 
 namespace dugma {
@@ -215,6 +217,363 @@ __global__ __launch_bounds__(kSBlock) void k_knn_merge(
 
 // Candidate chunks so that about 8,192 waves run, each chunk at least
 // min_chunk candidates long.
+// ---------------------------------------------------------------------------
+// k-NN over a uniform grid (exact, same results as the all-pairs kernels):
+// the finite candidates are binned into cubic cells of side h (counting
+// sort: counts, exclusive scan, scatter); each query visits the cells of
+// Chebyshev rings r = 0, 1, 2, ... around its own cell.  A candidate not yet
+// visited after ring r lies in a cell at ring distance >= r + 1, so at least
+// r h away (also for a query outside the grid: its cell index is clamped
+// towards the candidates).  The search stops once the list holds K entries
+// and the squared distance to the unvisited cells (at least (r h)^2, more for
+// a query outside the grid), times 1 - 1e-4, exceeds the K-th distance (the
+// margin covers the FP32 rounding of the distances and the cell
+// arithmetic); after
+// kRingFallback rings it rescans all candidates from an empty list, which
+// bounds the cost of sparse regions.  Cells are visited out of index order,
+// so the list is kept in (distance, index) order, which is the all-pairs
+// kernels' lowest-index rule.  Distances use the same FP32 expression.
+// ---------------------------------------------------------------------------
+constexpr int kBoxGroups = 64;     // workgroups of the bounding-box pass
+constexpr int kRingFallback = 8;   // rings before the full rescan
+constexpr int kGridMinPairs = 1 << 24;  // below this many pairs the all-pairs kernels run
+
+struct GridParams {
+  double ox, oy, oz;  // origin (minimum corner of the finite candidates)
+  double h, inv_h;    // cell side
+  int nx, ny, nz, nvalid;
+};
+
+// Per-workgroup minimum / maximum of the finite candidates and their count:
+// part[g] = {min x, y, z, count (bits), max x, y, z, unused}.
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_grid_bbox(const float *__restrict__ P, int np,
+                                                       float *__restrict__ part) {
+  __shared__ float smin[D][kSBlock], smax[D][kSBlock];
+  __shared__ int scnt[kSBlock];
+  const int len = (np + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int j0 = blockIdx.x * len, j1 = min(j0 + len, np);
+  float lo[D], hi[D];
+  int cnt = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    lo[a] = __builtin_huge_valf();
+    hi[a] = -__builtin_huge_valf();
+  }
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += kSBlock) {
+    bool fin = true;
+    float x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      x[a] = P[(size_t)j * D + a];
+      fin = fin && isfinite(x[a]);
+    }
+    if (!fin) continue;
+    ++cnt;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      lo[a] = fminf(lo[a], x[a]);
+      hi[a] = fmaxf(hi[a], x[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    smin[a][threadIdx.x] = lo[a];
+    smax[a][threadIdx.x] = hi[a];
+  }
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int w = kSBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        smin[a][threadIdx.x] = fminf(smin[a][threadIdx.x], smin[a][threadIdx.x + w]);
+        smax[a][threadIdx.x] = fmaxf(smax[a][threadIdx.x], smax[a][threadIdx.x + w]);
+      }
+      scnt[threadIdx.x] += scnt[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float *o = part + (size_t)blockIdx.x * 8;
+    for (int a = 0; a < 3; ++a) {
+      o[a] = a < D ? smin[a][0] : 0.0f;
+      o[4 + a] = a < D ? smax[a][0] : 0.0f;
+    }
+    o[3] = __int_as_float(scnt[0]);
+    o[7] = 0.0f;
+  }
+}
+
+// Grid parameters from the partial boxes (one thread): about 2 cells per
+// finite candidate over the box, extents below 1/64 of the largest counted
+// as 1/64 of it (flat and thin clouds), at most 2048 cells per axis and
+// `cap` cells in all (the cell side grows until both hold).
+template <int D>
+__global__ void k_grid_params(const float *__restrict__ part, int groups, long long cap,
+                              GridParams *__restrict__ gp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int n = 0;
+  bool any = false;
+  for (int g = 0; g < groups; ++g) {
+    const float *o = part + (size_t)g * 8;
+    const int c = __float_as_int(o[3]);
+    if (c == 0) continue;
+    n += c;
+    for (int a = 0; a < D; ++a) {
+      lo[a] = any ? fmin(lo[a], (double)o[a]) : (double)o[a];
+      hi[a] = any ? fmax(hi[a], (double)o[4 + a]) : (double)o[4 + a];
+    }
+    any = true;
+  }
+  GridParams p;
+  p.ox = lo[0];
+  p.oy = lo[1];
+  p.oz = lo[2];
+  p.nvalid = n;
+  double e[3] = {0, 0, 0}, emax = 0.0;
+  for (int a = 0; a < D; ++a) {
+    e[a] = hi[a] - lo[a];
+    emax = fmax(emax, e[a]);
+  }
+  double h = 1.0;
+  if (n > 0 && emax > 0.0) {
+    double vol = 1.0;
+    for (int a = 0; a < D; ++a) vol *= fmax(e[a], emax / 64.0);
+    const double cells = 2.0 * n;
+    h = D == 3 ? cbrt(vol / cells) : sqrt(vol / cells);
+  }
+  // h grows until both limits hold, so that every candidate lies inside its
+  // own cell (no candidate cell index is clamped)
+  int nn[3] = {1, 1, 1};
+  for (int it = 0; it < 400; ++it) {
+    double tot = 1.0, mx = 1.0;
+    for (int a = 0; a < D; ++a) {
+      const double na = floor(e[a] / h) + 1.0;
+      tot *= na;
+      mx = fmax(mx, na);
+    }
+    if (mx <= 2048.0 && tot <= (double)cap) break;
+    h *= 1.26;
+  }
+  for (int a = 0; a < D; ++a) nn[a] = (int)(floor(e[a] / h) + 1.0);
+  p.h = h;
+  p.inv_h = 1.0 / h;
+  p.nx = nn[0];
+  p.ny = nn[1];
+  p.nz = D == 3 ? nn[2] : 1;
+  *gp = p;
+}
+
+__device__ __forceinline__ int grid_coord(float x, double o, double inv_h, int n) {
+  const double c = floor(((double)x - o) * inv_h);
+  return (int)fmin(fmax(c, 0.0), (double)(n - 1));
+}
+
+template <int D>
+__device__ __forceinline__ int grid_cell(const float *x, const GridParams &g) {
+  const int cx = grid_coord(x[0], g.ox, g.inv_h, g.nx);
+  const int cy = grid_coord(x[1], g.oy, g.inv_h, g.ny);
+  const int cz = D == 3 ? grid_coord(x[2], g.oz, g.inv_h, g.nz) : 0;
+  return (cz * g.ny + cy) * g.nx + cx;
+}
+
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_grid_count(const float *__restrict__ P, int np,
+                                                        const GridParams *__restrict__ gp,
+                                                        int *__restrict__ cnt) {
+  const int j = blockIdx.x * kSBlock + threadIdx.x;
+  if (j >= np) return;
+  float x[D];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    x[a] = P[(size_t)j * D + a];
+    fin = fin && isfinite(x[a]);
+  }
+  if (!fin) return;
+  atomicAdd(cnt + grid_cell<D>(x, *gp), 1);
+}
+
+// Candidates into cell order (order inside a cell is arbitrary: the lists
+// are kept in (distance, index) order).  cnt counts down to 0.
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_grid_scatter(const float *__restrict__ P, int np,
+                                                          const GridParams *__restrict__ gp,
+                                                          const int *__restrict__ start,
+                                                          int *__restrict__ cnt, float *__restrict__ spts,
+                                                          int *__restrict__ sidx) {
+  const int j = blockIdx.x * kSBlock + threadIdx.x;
+  if (j >= np) return;
+  float x[D];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    x[a] = P[(size_t)j * D + a];
+    fin = fin && isfinite(x[a]);
+  }
+  if (!fin) return;
+  const int cell = grid_cell<D>(x, *gp);
+  const int pos = start[cell] + atomicSub(cnt + cell, 1) - 1;
+#pragma unroll
+  for (int a = 0; a < D; ++a) spts[(size_t)pos * D + a] = x[a];
+  sidx[pos] = j;
+}
+
+template <int K>
+__device__ __forceinline__ void topk_insert_lex(float *bd, int *bi, float dd, int j) {
+  if (!(dd < bd[K - 1] || (dd == bd[K - 1] && j < bi[K - 1]))) return;
+  bool after[K];  // entry s comes after (dd, j)
+#pragma unroll
+  for (int s = 0; s < K; ++s) after[s] = bd[s] > dd || (bd[s] == dd && bi[s] > j);
+#pragma unroll
+  for (int s = K - 1; s >= 0; --s) {
+    if (after[s]) {
+      if (s > 0 && after[s - 1]) {
+        bd[s] = bd[s - 1];
+        bi[s] = bi[s - 1];
+      } else {
+        bd[s] = dd;
+        bi[s] = j;
+      }
+    }
+  }
+}
+
+template <int D, int K>
+__device__ __forceinline__ void grid_visit(const float *q, int i, int self_exclude, int p0, int p1,
+                                           const float *__restrict__ spts, const int *__restrict__ sidx,
+                                           float *bd, int *bi) {
+  for (int pos = p0; pos < p1; ++pos) {
+    const int j = sidx[pos];
+    const float *p = spts + (size_t)pos * D;
+    const float dx = q[0] - p[0], dy = q[1] - p[1];
+    float dd;
+    if (D == 3) {
+      const float dz = q[2] - p[2];
+      dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    } else {
+      dd = fmaf(dy, dy, dx * dx);
+    }
+    if (self_exclude && j == i) continue;
+    topk_insert_lex<K>(bd, bi, dd, j);
+  }
+}
+
+// Lower bound of the squared distance from q to every cell outside the
+// block of rings 0..r around its cell (cx, cy, cz): per axis and side with
+// cells left beyond the block, the squared distance to that face of the
+// block plus the squared gaps of q to the grid's extent along the other axes
+// (zero for a query inside the grid).  +inf when no cell is left.
+template <int D>
+__device__ __forceinline__ double unvisited_dist2(const float *q, int cx, int cy, int cz, int r,
+                                                  const GridParams &g) {
+  const double o[3] = {g.ox, g.oy, g.oz};
+  const int c[3] = {cx, cy, cz}, n[3] = {g.nx, g.ny, g.nz};
+  double gap2[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double x = q[a], lo = o[a], hi = o[a] + n[a] * g.h;
+    const double gp = x < lo ? lo - x : (x > hi ? x - hi : 0.0);
+    gap2[a] = gp * gp;
+  }
+  double lb = __builtin_huge_val();
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double others = 0.0;
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+      if (b != a) others += gap2[b];
+    const double x = q[a];
+    if (c[a] - r > 0) {  // cells below the block
+      const double d = fmax(x - (o[a] + (c[a] - r) * g.h), 0.0);
+      lb = fmin(lb, d * d + others);
+    }
+    if (c[a] + r < n[a] - 1) {  // cells above the block
+      const double d = fmax(o[a] + (c[a] + r + 1) * g.h - x, 0.0);
+      lb = fmin(lb, d * d + others);
+    }
+  }
+  return lb;
+}
+
+template <int D, int K>
+__global__ __launch_bounds__(kSBlock) void k_knn_grid(
+    const float *__restrict__ Q, int nq, const float *__restrict__ spts, const int *__restrict__ sidx,
+    const int *__restrict__ start, const GridParams *__restrict__ gp, int self_exclude, int k,
+    float *__restrict__ od, int *__restrict__ oi) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i >= nq) return;
+  const GridParams g = *gp;
+  float q[D];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    q[a] = Q[(size_t)i * D + a];
+    fin = fin && isfinite(q[a]);
+  }
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    bd[s] = __builtin_huge_valf();
+    bi[s] = -1;
+  }
+  if (fin && g.nvalid > 0) {
+    const int cx = grid_coord(q[0], g.ox, g.inv_h, g.nx);
+    const int cy = grid_coord(q[1], g.oy, g.inv_h, g.ny);
+    const int cz = D == 3 ? grid_coord(q[2], g.oz, g.inv_h, g.nz) : 0;
+    // ring beyond which no cell is left
+    int rall = max(max(cx, g.nx - 1 - cx), max(cy, g.ny - 1 - cy));
+    if (D == 3) rall = max(rall, max(cz, g.nz - 1 - cz));
+    for (int r = 0;; ++r) {
+      if (r > kRingFallback) {
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+          bd[s] = __builtin_huge_valf();
+          bi[s] = -1;
+        }
+        grid_visit<D, K>(q, i, self_exclude, 0, g.nvalid, spts, sidx, bd, bi);
+        break;
+      }
+      const int zr = D == 3 ? r : 0;
+      for (int a = -r; a <= r; ++a) {
+        const int x = cx + a;
+        if (x < 0 || x >= g.nx) continue;
+        for (int b = -r; b <= r; ++b) {
+          const int y = cy + b;
+          if (y < 0 || y >= g.ny) continue;
+          const bool edge = a == -r || a == r || b == -r || b == r;
+          if (D == 2) {
+            if (!edge) continue;
+            const int cell = y * g.nx + x;
+            grid_visit<D, K>(q, i, self_exclude, start[cell], start[cell + 1], spts, sidx, bd, bi);
+            continue;
+          }
+          // 3-D: the whole z column on the rim of the (a, b) square, else its two ends
+          const int step = (edge || zr == 0) ? 1 : 2 * zr;
+          for (int c = -zr; c <= zr; c += step) {
+            const int z = cz + c;
+            if (z < 0 || z >= g.nz) continue;
+            const int cell = (z * g.ny + y) * g.nx + x;
+            grid_visit<D, K>(q, i, self_exclude, start[cell], start[cell + 1], spts, sidx, bd, bi);
+          }
+        }
+      }
+      if (r >= rall) break;
+      if (bi[K - 1] >= 0 && unvisited_dist2<D>(q, cx, cy, cz, r, g) * (1.0 - 1e-4) > (double)bd[K - 1])
+        break;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (s < k) {
+      od[(size_t)i * k + s] = bd[s];
+      oi[(size_t)i * k + s] = bi[s];
+    }
+}
+
 static int chunks_for(int nq, int np, int min_chunk = 256) {
   const int qblocks = (nq + kSBlock - 1) / kSBlock;
   int c = (8192 + qblocks * 4 - 1) / (qblocks * 4);
@@ -289,6 +648,80 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
   return DUGMA_OK;
 }
 
+// The grid k-NN: candidates Pp (transformed from the source cloud first when
+// R is given), queries Qp.
+static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const double *t, const float *Qp,
+                    int nq, const float *Pp, int np, int self, int *idx, float *d2) {
+  const long long cap = std::min<long long>(8LL * np + 64, 1LL << 24);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t temp_bytes = 0;
+  DUGMA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, (const int *)nullptr, (int *)nullptr,
+                                             (int)(cap + 1), c->stream));
+  const size_t need_m = R ? al((size_t)np * D * 4) : 0;
+  const size_t b_part = al((size_t)kBoxGroups * 8 * 4), b_gp = al(sizeof(GridParams));
+  const size_t b_cnt = al((size_t)(cap + 1) * 4), b_pts = al((size_t)np * D * 4), b_idx = al((size_t)np * 4);
+  const size_t b_tmp = al(temp_bytes), b_out = al((size_t)nq * k * 4);
+  int rc = c->scratch2.ensure(need_m + b_part + b_gp + 2 * b_cnt + b_pts + b_idx + b_tmp + 2 * b_out);
+  if (rc) return rc;
+  char *p = (char *)c->scratch2.ptr;
+  if (R) {
+    rc = transform_compat(c, R, t, (float *)p);
+    if (rc) return rc;
+    Pp = (const float *)p;
+  }
+  p += need_m;
+  float *part = (float *)p;
+  p += b_part;
+  GridParams *gp = (GridParams *)p;
+  p += b_gp;
+  int *cnt = (int *)p;
+  p += b_cnt;
+  int *start = (int *)p;
+  p += b_cnt;
+  float *spts = (float *)p;
+  p += b_pts;
+  int *sidx = (int *)p;
+  p += b_idx;
+  void *tmp = p;
+  p += b_tmp;
+  float *od = (float *)p;
+  p += b_out;
+  int *oi = (int *)p;
+  const dim3 blk(kSBlock), pgrid((np + kSBlock - 1) / kSBlock), qgrid((nq + kSBlock - 1) / kSBlock);
+  if (D == 3) {
+    hipLaunchKernelGGL(k_grid_bbox<3>, dim3(kBoxGroups), blk, 0, c->stream, Pp, np, part);
+    hipLaunchKernelGGL(k_grid_params<3>, dim3(1), dim3(64), 0, c->stream, part, kBoxGroups, cap, gp);
+  } else {
+    hipLaunchKernelGGL(k_grid_bbox<2>, dim3(kBoxGroups), blk, 0, c->stream, Pp, np, part);
+    hipLaunchKernelGGL(k_grid_params<2>, dim3(1), dim3(64), 0, c->stream, part, kBoxGroups, cap, gp);
+  }
+  DUGMA_HIP(hipMemsetAsync(cnt, 0, (size_t)(cap + 1) * 4, c->stream));
+  if (D == 3)
+    hipLaunchKernelGGL(k_grid_count<3>, pgrid, blk, 0, c->stream, Pp, np, gp, cnt);
+  else
+    hipLaunchKernelGGL(k_grid_count<2>, pgrid, blk, 0, c->stream, Pp, np, gp, cnt);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, temp_bytes, cnt, start, (int)(cap + 1), c->stream));
+  if (D == 3)
+    hipLaunchKernelGGL(k_grid_scatter<3>, pgrid, blk, 0, c->stream, Pp, np, gp, start, cnt, spts, sidx);
+  else
+    hipLaunchKernelGGL(k_grid_scatter<2>, pgrid, blk, 0, c->stream, Pp, np, gp, start, cnt, spts, sidx);
+#define KNN_GRID(DD, KK)                                                                         \
+  hipLaunchKernelGGL((k_knn_grid<DD, KK>), qgrid, blk, 0, c->stream, Qp, nq, spts, sidx, start, gp, \
+                     self, k, od, oi)
+  if (D == 3) {
+    if (K == 8) KNN_GRID(3, 8); else if (K == 16) KNN_GRID(3, 16); else KNN_GRID(3, 32);
+  } else {
+    if (K == 8) KNN_GRID(2, 8); else if (K == 16) KNN_GRID(2, 16); else KNN_GRID(2, 32);
+  }
+#undef KNN_GRID
+  DUGMA_HIP(hipGetLastError());
+  if (d2) DUGMA_HIP(hipMemcpyAsync(d2, od, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
+  if (idx) DUGMA_HIP(hipMemcpyAsync(idx, oi, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  return DUGMA_OK;
+}
+
 extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
                          const double *t, int *idx, float *d2) {
   if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_knn: null context");
@@ -313,6 +746,9 @@ extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
     return fail(DUGMA_ERR_ARG, "dugma_knn: unknown query set");
   }
   const int K = k <= 8 ? 8 : (k <= 16 ? 16 : 32);
+  if ((long long)nq * np >= kGridMinPairs)
+    return knn_grid(c, D, K, k, which == DUGMA_KNN_TARGET_TO_SOURCE ? R : nullptr, t, Qp, nq, Pp, np, self, idx,
+                    d2);
   const int nch = chunks_for(nq, np);
   const int chunk = (np + nch - 1) / nch;
   const int nchunks = (np + chunk - 1) / chunk;

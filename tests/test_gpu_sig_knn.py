@@ -2,6 +2,8 @@
 bit-exact against the oracle."""
 import os
 
+import math
+
 import numpy as np
 import pytest
 
@@ -109,6 +111,88 @@ def test_knn_ties_lowest_index(oracle, k):
     try:
         idx, d2 = dg.knn(h, dg.KNN_TARGET_SELF, k)
         ri, rd = oracle.knn(X.T.astype(np.float32), X.T.astype(np.float32), k, True)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+    finally:
+        dg.fr_m(h)
+
+
+# The cases below hold at least 2^24 query x candidate pairs, where the
+# library searches a uniform grid instead of all pairs; the results must be
+# the all-pairs ones, bit for bit.
+
+@pytest.mark.parametrize("k", [1, 8, 13, 32])
+def test_knn_grid_ties_lowest_index(oracle, k):
+    rng = np.random.default_rng(100 + k)
+    X = rng.integers(0, 10, size=(3, 5000)).astype(float)  # ~5 points per lattice site
+    C = W.iso_blocks(np.full(5000, 0.01))
+    ok, h = dg.init(X, C, X, C)
+    try:
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_SELF, k)
+        Xf = X.T.astype(np.float32)
+        ri, rd = oracle.knn(Xf, Xf, k, True)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+    finally:
+        dg.fr_m(h)
+
+
+def test_knn_grid_disjoint_clouds(oracle):
+    """Every query far outside the candidates' box."""
+    d = W.config3(n=6000, m=5000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R = d["R_G"]
+        t = np.ravel(d["t_G"]) + np.array([4.0, -1.5, 0.5])
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_TO_SOURCE, 8, R, t)
+        Xf, Yf = d["X"].T.astype(np.float32), d["Y"].T.astype(np.float32)
+        ri, rd = oracle.knn(Xf, oracle.transform_points(Yf, R, t), 8, False)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+    finally:
+        dg.fr_m(h)
+
+
+def test_knn_grid_non_finite_and_flat(oracle):
+    """Non-finite points (never neighbours; a non-finite query gets -1 /
+    +inf) and a flat cloud (zero extent along z)."""
+    rng = np.random.default_rng(7)
+    X = rng.random((3, 6000))
+    X[2] = 0.25
+    X[:, 10] = np.nan
+    X[1, 20] = np.inf
+    X[:, 30] = 1e6
+    C = W.iso_blocks(np.full(6000, 0.01))
+    ok, h = dg.init(X, C, X, C)
+    try:
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_SELF, 8)
+        Xf = X.T.astype(np.float32)
+        ri, rd = oracle.knn(Xf, Xf, 8, True)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+        assert (idx[10] == -1).all() and np.isinf(d2[10]).all()
+    finally:
+        dg.fr_m(h)
+
+
+def test_knn_grid_2d(oracle):
+    rng = np.random.default_rng(8)
+    X = rng.random((2, 5000))
+    Y = rng.random((2, 4500)) * 0.8 + 0.1
+    CX = W.blocks(np.tile(0.01 * np.eye(2), (5000, 1, 1)))
+    CY = W.blocks(np.tile(0.01 * np.eye(2), (4500, 1, 1)))
+    ok, h = dg.init(X, CX, Y, CY)
+    try:
+        th = 0.3
+        R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+        t = np.array([0.05, -0.02])
+        idx, d2 = dg.knn(h, dg.KNN_TARGET_TO_SOURCE, 16, R, t)
+        ri, rd = oracle.knn(X.T.astype(np.float32), oracle.transform_points(Y.T.astype(np.float32), R, t), 16,
+                            False)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(d2, rd)
+        idx, d2 = dg.knn(h, dg.KNN_SOURCE_SELF, 8)
+        ri, rd = oracle.knn(Y.T.astype(np.float32), Y.T.astype(np.float32), 8, True)
         np.testing.assert_array_equal(idx, ri)
         np.testing.assert_array_equal(d2, rd)
     finally:
