@@ -240,14 +240,17 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
             "speedup": round(rps / rps1, 2)}
 
 
-def bench_knn(dg, d, k=8, reps=3):
+def bench_knn(dg, d, k=8, reps=10):
     """k nearest transformed sources of every target (the sigma update of the
     200K x 200K configuration), wall time per call including the copy of the
-    N x k indices and distances to the host."""
+    N x k indices and distances to the host.  Above 2^24 pairs the library
+    searches a uniform grid (exact: the all-pairs results, bit for bit), so
+    the rate is effective: N x M pairs per second of the call."""
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
         R, t = d["R_G"], d["t_G"]
-        dg.knn(h, dg.KNN_TARGET_TO_SOURCE, k, R, t)  # warm-up
+        for _ in range(2):
+            dg.knn(h, dg.KNN_TARGET_TO_SOURCE, k, R, t)  # warm-up
         t0 = time.perf_counter()
         for _ in range(reps):
             dg.knn(h, dg.KNN_TARGET_TO_SOURCE, k, R, t)
@@ -257,7 +260,7 @@ def bench_knn(dg, d, k=8, reps=3):
         dg.fr_m(h)
     log("knn k=%d: %.3f ms per call" % (k, ms))
     return {"k": k, "ms_per_call": round(ms, 3), "value": round(pairs / (ms / 1e3) / 1e9, 3),
-            "unit": "Gpair-distance-evals/s"}
+            "unit": "effective Gpair-distance-evals/s", "search": "uniform grid, exact (all-pairs results)"}
 
 
 def bench_reference_path(dg, W, d2):

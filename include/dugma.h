@@ -168,8 +168,12 @@ enum {
 /* k <= 32 nearest neighbours with FP32 squared distances computed as
  * fmaf(dz,dz,fmaf(dy,dy,dx*dx)) on the FP32 points of init, ranked by
  * (distance, index) so that ties go to the lowest index.  idx: nq x k int,
- * d2: nq x k float (either may be NULL); missing neighbours are -1 / +inf.
- * R, t are used by DUGMA_KNN_TARGET_TO_SOURCE only. */
+ * d2: nq x k float (either may be NULL); missing neighbours are -1 / +inf;
+ * points with a non-finite coordinate are never neighbours.  R, t are used
+ * by DUGMA_KNN_TARGET_TO_SOURCE only.  Small problems compare all pairs;
+ * from 2^24 query x candidate pairs on, the candidates are binned into a
+ * uniform grid and searched ring by ring with an exact stopping bound: the
+ * results are the same, bit for bit. */
 int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
               const double *t, int *idx, float *d2);
 
