@@ -162,6 +162,17 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         dg.fr_m(h)
 
 
+def bench_plain_large(hip, dg, d, big, gradients=2):
+    """The finite-difference gradient on the 200K x 200K clouds, beside the fused step measured there."""
+    plain = bench_plain(hip, dg, d, gradients)
+    speedup = plain["ms_per_gradient"] / big["ms_per_step"]
+    log("plain FD 200Kx200K: %.2f ms/gradient, fused speed-up %.2fx" % (plain["ms_per_gradient"], speedup))
+    return {"value": round(plain["pairs"] / (plain["ms_per_gradient"] / 1e3) / 1e9, 3), "unit": "Gpair-evals/s",
+            "ms_per_gradient": round(plain["ms_per_gradient"], 3),
+            "scheme": "6-DoF central differences, h=1e-4, 13 forward launches",
+            "fused_speedup": round(speedup, 2)}
+
+
 # Batched iteration: DUGMA's sigma update (the kernel variance tracks the mean
 # nearest distance) on top of an annealed extra variance, pose steps
 # over-relaxed by 1.9 once annealing has ended; the first 160 iterations on
@@ -371,7 +382,7 @@ def main():
                 d3 = W.config3()
                 big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
                 print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"],
-                                  "knn": bench_knn(dg, d3)}))
+                                  "plain_fd": bench_plain_large(hip, dg, d3, big), "knn": bench_knn(dg, d3)}))
             else:
                 print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(128, args.trials))}))
             return
@@ -426,7 +437,7 @@ def main():
                         "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
                         "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
-                        "knn": bench_knn(dg, d3)}
+                        "plain_fd": bench_plain_large(hip, dg, d3, big), "knn": bench_knn(dg, d3)}
     if not args.no_batch:
         out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_loop:
