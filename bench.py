@@ -8,7 +8,14 @@ The value is pair evaluations per second of that whole call.  Beside it the
 line reports:
   plain      the same gradient by central finite differences (13 forward
              launches per gradient), same GPU;
-  large      the fused call at 200,000 x 200,000 isotropic;
+  large      the fused call at 200,000 x 200,000 isotropic, with its own
+             finite-difference gradient and the k = 8 nearest-neighbour
+             search of the sigma update;
+  points     energy with gradients wrt every point's position and
+             covariance on two 320 x 240 depth views;
+  batched    4,096 independent registrations in one persistent launch per
+             iteration, against one trial per launch;
+  reference_path  update_sig, compi and a whole DUGMA.m loop;
   roofline   achieved FP32 FLOP/s of the all-pairs kernel (HIP events on the
              context's stream) against MI355X's 157.3 TFLOP/s;
   cpu_baseline  the FP64 oracle (OpenMP, all host cores) on the same
@@ -171,6 +178,35 @@ def bench_plain_large(hip, dg, d, big, gradients=2):
             "ms_per_gradient": round(plain["ms_per_gradient"], 3),
             "scheme": "6-DoF central differences, h=1e-4, 13 forward launches",
             "fused_speedup": round(speedup, 2)}
+
+
+def bench_points(hip, dg, W, calls=3):
+    """Energy with the gradient wrt every point's position and covariance, on
+    two 320 x 240 depth views: two pair passes per call (targets stationary,
+    then sources), with the per-point sums copied to the host."""
+    d = W.config5_3d()
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = start_pose(d)
+        dg.energy_grad_points(h, R, t)
+        ms = hip.timed(h.stream(), lambda: dg.energy_grad_points(h, R, t), calls) / calls
+        h.set_timing(True)
+        dg.energy_grad_points(h, R, t)
+        kms, launches = h.pair_time()
+        h.set_timing(False)
+        nT, nS, _ = h.energy_info()
+    finally:
+        dg.fr_m(h)
+    evals = 2 * nT * nS
+    ach = evals * FLOP_3D_FULL_FUSED / (kms / 1e3) / 1e12
+    log("per-point gradients %dx%d: %.2f ms/call, pair passes %.2f ms (%d launches), %.1f%% of peak"
+        % (nT, nS, ms, kms, launches, 100 * ach / PEAK_FP32_TFLOPS))
+    return {"workload": "3-D two 320 x 240 depth views, %d x %d points, full covariance; energy + gradient "
+                        "wrt every point's position and covariance" % (nT, nS),
+            "ms_per_call": round(ms, 3), "kernel_ms": round(kms, 3), "launches": int(launches),
+            "value": round(evals / (kms / 1e3) / 1e9, 3), "unit": "Gpair-evals/s (two passes per call)",
+            "achieved_tflops": round(ach, 3), "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+            "flop_per_pair": FLOP_3D_FULL_FUSED}
 
 
 # Batched iteration: DUGMA's sigma update (the kernel variance tracks the mean
@@ -346,9 +382,10 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 200K x 200K measurement")
     ap.add_argument("--no-plain", action="store_true", help="skip the finite-difference measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched registrations")
+    ap.add_argument("--no-points", action="store_true", help="skip the per-point gradients of the depth views")
     ap.add_argument("--no-loop", action="store_true", help="skip update_sig / compi / DUGMA-loop timings")
     ap.add_argument("--trials", type=int, default=4096, help="batched trials")
-    ap.add_argument("--only", choices=["fused", "plain", "large", "batch", "loop"], default=None,
+    ap.add_argument("--only", choices=["fused", "plain", "large", "points", "batch", "loop"], default=None,
                     help="run one measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the launch shape from DUGMA_LANE_POINTS / DUGMA_TARGET_WAVES")
@@ -374,9 +411,12 @@ def main():
         args.no_large = only != "large"
         args.no_batch = only != "batch"
         args.no_loop = only != "loop"
+        args.no_points = only != "points"
         args.no_cpu = True
-        if only in ("large", "batch", "loop"):
-            if only == "loop":
+        if only in ("large", "points", "batch", "loop"):
+            if only == "points":
+                print(json.dumps({"only": only, **bench_points(hip, dg, W)}))
+            elif only == "loop":
                 print(json.dumps({"only": only, **bench_reference_path(dg, W, W.config2(n=6000, m=6000))}))
             elif only == "large":
                 d3 = W.config3()
@@ -438,6 +478,8 @@ def main():
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
                         "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
                         "plain_fd": bench_plain_large(hip, dg, d3, big), "knn": bench_knn(dg, d3)}
+    if not args.no_points:
+        out["points"] = bench_points(hip, dg, W)
     if not args.no_batch:
         out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_loop:

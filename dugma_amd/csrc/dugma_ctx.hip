@@ -57,6 +57,7 @@ void PairTimer::stop() {
 
 void PairTimer::finish() {
   if (!c->timing) return;
+  (void)hipEventSynchronize(c->ev1);
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
     c->pair_ms += ms;
@@ -99,6 +100,30 @@ int wait_slots(dugma_ctx *c, const float *slots, int n) { return wait_nan(c, slo
 
 void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int PinBuf::ensure(size_t need) {
+  if (need <= bytes) return DUGMA_OK;
+  if (ptr) {
+    hipError_t e = hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (e != hipSuccess) return hip_fail(e, "hipHostFree(staging)");
+  }
+  size_t want = need + need / 4 + 4096;
+  hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return hip_fail(e, "hipHostMalloc(staging)");
+  }
+  bytes = want;
+  return DUGMA_OK;
+}
+
+void PinBuf::release() {
+  if (ptr) (void)hipHostFree(ptr);
   ptr = nullptr;
   bytes = 0;
 }
@@ -264,6 +289,7 @@ static void free_ctx(dugma_ctx *c) {
     if (p) (void)hipFree(p);
   c->part.release();
   c->scratch2.release();
+  c->hstage.release();
   if (c->hpinned) (void)hipHostFree(c->hpinned);
   if (c->hparts) (void)hipHostFree(c->hparts);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -63,7 +63,7 @@ int hip_fail(hipError_t e, const char *what);
   } while (0)
 
 // Timing bracket around a pair-kernel launch (no-op unless timing is on);
-// finish() must run after the stream has been synchronised.
+// finish() waits for the stop event, then adds the elapsed time.
 struct PairTimer {
   dugma_ctx *c;
   explicit PairTimer(dugma_ctx *ctx);
@@ -87,6 +87,15 @@ int wait_slots(dugma_ctx *c, const float *slots, int n);
 
 // Grow-only device scratch buffer owned by a context.
 struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+};
+
+// Grow-only pinned host buffer owned by a context (device-to-host copies of
+// bulk results at DMA speed).
+struct PinBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
   int ensure(size_t need);
@@ -152,6 +161,7 @@ struct dugma_ctx {
 
   dugma::DevBuf part;        // pair-kernel partials
   dugma::DevBuf scratch2;    // reductions / k-NN merge
+  dugma::PinBuf hstage;      // per-point sums on their way to the host
   double *hpinned = nullptr; // pinned host staging, kHostStage doubles
   static constexpr int kHostStage = 4096;
   // pinned host buffer the pair kernel writes its per-workgroup partials to

@@ -257,8 +257,10 @@ int dugma_ctx_energy_info(const dugma_ctx *ctx, int *n_target, int *n_source,
 /* The HIP stream (hipStream_t) the context launches on, for event timing. */
 void *dugma_ctx_stream(const dugma_ctx *ctx);
 /* Kernel timing: when on, HIP events are recorded on the context's stream
- * around every launch of the all-pairs kernels; dugma_ctx_pair_time returns
- * the summed milliseconds and launch count since timing was switched on. */
+ * around every launch of the all-pairs kernels (energy, pose-gradient and
+ * per-point passes; a per-point call counts two launches, and each launch
+ * then waits for its kernel); dugma_ctx_pair_time returns the summed
+ * milliseconds and launch count since timing was switched on. */
 int dugma_ctx_set_timing(dugma_ctx *ctx, int on);
 int dugma_ctx_pair_time(const dugma_ctx *ctx, double *ms, long long *launches);
 const char *dugma_last_error(void);
