@@ -422,7 +422,8 @@ def main():
                 d3 = W.config3()
                 big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
                 print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"],
-                                  "plain_fd": bench_plain_large(hip, dg, d3, big), "knn": bench_knn(dg, d3)}))
+                                  "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
+                                  "knn": bench_knn(dg, d3)}))
             else:
                 print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(128, args.trials))}))
             return
@@ -477,7 +478,8 @@ def main():
                         "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
                         "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
-                        "plain_fd": bench_plain_large(hip, dg, d3, big), "knn": bench_knn(dg, d3)}
+                        "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
+                        "knn": bench_knn(dg, d3)}
     if not args.no_points:
         out["points"] = bench_points(hip, dg, W)
     if not args.no_batch:

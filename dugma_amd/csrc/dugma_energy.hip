@@ -889,9 +889,11 @@ extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
   DUGMA_HIP(hipEventCreate(&e0));
   DUGMA_HIP(hipEventCreate(&e1));
   static const int kT[2] = {1, 2};
-  // The largest targets serve large problems: at 200K x 200K, 65,536 waves
-  // (21 source chunks) run 3% faster than 32,768 (11 chunks).
-  static const int kW[7] = {2048, 4096, 8192, 16384, 32768, 65536, 131072};
+  // The largest targets serve large problems: at 200K x 200K the kernel
+  // takes 17.75 ms at 32,768 waves (11 source chunks), 16.89 ms at 131,072
+  // (42) and 16.70-16.76 ms at 524,288 (168).
+  static const int kW[] = {2048, 4096, 8192, 16384, 32768, 65536, 131072, 262144, 524288};
+  constexpr int kNW = (int)(sizeof(kW) / sizeof(kW[0]));
   const double pairs = (double)ctx->nT * ctx->nS;
   const int reps = pairs > 4e9 ? 1 : (pairs > 1e8 ? 3 : 10);
   for (int m = 0; m < 2; ++m) {
@@ -899,7 +901,7 @@ extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
     dugma_ctx::Tuned best;
     float best_ms = 1e30f;
     for (int ti = 0; ti < 2; ++ti)
-      for (int wi = 0; wi < 7; ++wi) {
+      for (int wi = 0; wi < kNW; ++wi) {
         dugma_ctx::Tuned &tu = ctx->tuned[m][iso ? 1 : 0];
         tu.T = kT[ti];
         tu.target_waves = kW[wi];
