@@ -137,7 +137,9 @@ int dugma_energy_grad(dugma_ctx *ctx, const double *R, const double *t,
  *        pose is updated as R <- dR R, t <- t + dt (DUGMA.m:67-78).
  * These passes always run the full-covariance kernels (cov_mode is only
  * checked): E and gdT describe source covariances R CY R^T also when R is
- * not a rotation, as for dugma_energy with DUGMA_COV_FULL. */
+ * not a rotation, as for dugma_energy with DUGMA_COV_FULL.  The per-point
+ * sums come back through a pinned buffer that the context keeps, and up to
+ * 8 host threads turn them into the outputs. */
 int dugma_energy_grad_points(dugma_ctx *ctx, const double *R, const double *t,
                              int cov_mode, double *E, double *gpose,
                              double *gX, double *gXC, double *gY, double *gYC,
