@@ -19,6 +19,19 @@
  *  dor_compi          golden vectors evaluated from compi.cu's own per-pair
  *                     coefficient expressions, and F(dT) == brute force
  *                     surrogate at random dT
+ *  dor_transform_points  closed forms through dor_update_sig (Y = X, R = I
+ *                     gives 0) and the brute-force nearest-distance check
+ *  dor_tree_sum_f32   a hand-worked 5-element case on which the pairwise
+ *                     order and a left-to-right sum differ
+ *  dor_energy_fd_pose central differences against the analytic pose
+ *                     gradient, which float64 torch autograd also checks
+ *  dor_pair_weight    summed over all pairs at the identity pose, equal to
+ *                     the surrogate's value there
+ *  dor_surrogate_eval equal to the 637 / 45 coefficient polynomial at
+ *                     random dT (the polynomial identity)
+ *  dor_poly_eval      closed-form monomial evaluation (tests/test_poly.py)
+ *  dor_constraints    zero on rotations, non-zero off them, the 2-D form
+ *                     on a unit vector
  */
 #include "dugma_oracle.h"
 

@@ -2,6 +2,16 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the dugma_amd package.
+
+Pinning of the functions written here (those of the C library are listed in
+dugma_oracle.c):
+  dugma_loop   parity unpinned vs fmincon: scipy's BFGS stands in for it as
+               an independent minimiser of the same surrogate, so the GPU
+               loop is compared at iterate level; end to end, the Kinect
+               fixture scenes are checked against their ground-truth poses
+  the ZYX Euler angles of its convergence test (DUGMA.m:80, rotm2eul)
+               parity unpinned: textbook ZYX, the Robotics Toolbox being
+               absent
 """
 import ctypes
 import os
