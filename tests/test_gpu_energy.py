@@ -292,3 +292,31 @@ def test_depth_views_subsample_matches_oracle(oracle):
     assert rel(out["gXC"], ref["gSY"]) <= G_TOL
     assert rel(out["gY"], ref["gX"]) <= G_TOL
     assert rel(out["gYC"], ref["gSX"]) <= G_TOL
+
+
+def test_points_repeatable_and_shape_independent():
+    """Per-point gradients come from GPU sums that the host turns into
+    gradients on several threads over fixed chunks: repeated calls give the
+    same bits.  Another launch shape (fewer streamed chunks) regroups the
+    FP32 sub-tile sums, and agrees within the parity bars."""
+    d = W.config2(n=9000, m=8500)  # more than two host chunks on each side
+    R, t = perturbed_pose(d, deg=2.0, dt=0.005, seed=21)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    old = os.environ.get("DUGMA_POINT_WAVES")
+    try:
+        a = dg.energy_grad_points(h, R, t)
+        b = dg.energy_grad_points(h, R, t)
+        for key in ("gX", "gXC", "gY", "gYC", "gpose", "gdT"):
+            assert np.array_equal(a[key], b[key]), key
+        assert a["E"] == b["E"]
+        os.environ["DUGMA_POINT_WAVES"] = "2048"
+        c = dg.energy_grad_points(h, R, t)
+    finally:
+        if old is None:
+            os.environ.pop("DUGMA_POINT_WAVES", None)
+        else:
+            os.environ["DUGMA_POINT_WAVES"] = old
+        dg.fr_m(h)
+    assert abs(c["E"] / a["E"] - 1) <= E_TOL
+    for key in ("gX", "gXC", "gY", "gYC", "gpose", "gdT"):
+        assert rel(c[key], a[key]) <= G_TOL, key
