@@ -9,8 +9,9 @@ line reports:
   plain      the same gradient by central finite differences (13 forward
              launches per gradient), same GPU;
   large      the fused call at 200,000 x 200,000 isotropic, with its own
-             finite-difference gradient and the k = 8 nearest-neighbour
-             search of the sigma update;
+             finite-difference gradient, the k = 8 nearest-neighbour
+             search of the sigma update and a registration on the
+             convolved energy;
   points     energy with gradients wrt every point's position and
              covariance on two 320 x 240 depth views;
   batched    4,096 independent registrations in one persistent launch per
@@ -167,6 +168,28 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         return dict(ms_per_gradient=ms / steps, pairs=nT * nS)
     finally:
         dg.fr_m(h)
+
+
+def bench_register_energy(dg, d):
+    """Registration on the convolved energy (BFGS, one fused pass per trial
+    pose) on the 200K x 200K clouds, from 5 degrees about a fixed axis and
+    1 cm per axis off the ground truth."""
+    from dugma_amd import workloads as W
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        if TUNE:
+            h.autotune()
+        R0 = W.rot_axis_angle([1.0, -2.0, 0.5], np.radians(5.0)) @ d["R_G"]
+        t0 = np.ravel(d["t_G"]) + np.array([0.01, -0.01, 0.01])
+        t1 = time.perf_counter()
+        R, t, it, E = dg.register_energy(h, R0, t0)
+        sec = time.perf_counter() - t1
+    finally:
+        dg.fr_m(h)
+    log("register_energy 200Kx200K: %d steps in %.3f s" % (it, sec))
+    return {"seconds": round(sec, 4), "steps": it, "start_rotation_error_deg": round(W.rotation_error_deg(R0, d["R_G"]), 4),
+            "rotation_error_deg": round(W.rotation_error_deg(R, d["R_G"]), 4),
+            "translation_error": float(np.linalg.norm(t - np.ravel(d["t_G"])))}
 
 
 def bench_plain_large(hip, dg, d, big, gradients=2):
@@ -423,7 +446,7 @@ def main():
                 big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
                 print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"],
                                   "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
-                                  "knn": bench_knn(dg, d3)}))
+                                  "knn": bench_knn(dg, d3), "register_energy": bench_register_energy(dg, d3)}))
             else:
                 print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(128, args.trials))}))
             return
@@ -479,7 +502,7 @@ def main():
                         "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
                         "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
                         "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
-                        "knn": bench_knn(dg, d3)}
+                        "knn": bench_knn(dg, d3), "register_energy": bench_register_energy(dg, d3)}
     if not args.no_points:
         out["points"] = bench_points(hip, dg, W)
     if not args.no_batch:

@@ -238,6 +238,23 @@ def energy_grad_fd(h, R, t, step=1e-4, cov="auto"):
     return E.value, g
 
 
+def register_energy(h, R0, t0, max_iter=200, tol_rot=1e-7, tol_trans=1e-14, cov="auto"):
+    """Registration on the convolved energy (dugma_register_energy): BFGS
+    descent of -log E over left pose increments from (R0, t0).  Returns
+    (R, t, iterations, E)."""
+    D = h.D
+    R0 = _cm(R0, (D, D))
+    t0 = _cm(np.ravel(t0))
+    o = _lib.RegisterEnergyOpts(int(max_iter), float(tol_rot), float(tol_trans), _cov_mode(cov))
+    R = np.zeros((D, D), order="F")
+    t = np.zeros(D)
+    it = ctypes.c_int()
+    E = ctypes.c_double()
+    _lib.check(_lib.load().dugma_register_energy(h.ptr, _dp(R0), _dp(t0), ctypes.byref(o), _dp(R), _dp(t),
+                                                 ctypes.byref(it), ctypes.byref(E)))
+    return np.ascontiguousarray(R), t, it.value, E.value
+
+
 def energy_grad_points(h, R, t, cov="auto"):
     """Energy with per-point gradients.  Returns a dict of numpy arrays:
     gX (N,D), gXC (N,D,D), gY (M,D), gYC (M,D,D), gpose, gdT (12 or 4)."""

@@ -33,6 +33,12 @@ class RegisterOpts(ctypes.Structure):
                 ("verbose", ctypes.c_int)]
 
 
+class RegisterEnergyOpts(ctypes.Structure):
+    """dugma_register_energy_opts of include/dugma.h."""
+    _fields_ = [("max_iter", ctypes.c_int), ("tol_rot", ctypes.c_double),
+                ("tol_trans", ctypes.c_double), ("cov_mode", ctypes.c_int)]
+
+
 _pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
 
 # name: (restype, argtypes) -- one entry per declaration of include/dugma.h
@@ -46,6 +52,7 @@ SIGNATURES = {
     "dugma_register": (_ci, [_d, _d, _d, _d, _ci, _ci, _ci, _d, _d, ctypes.POINTER(RegisterOpts), _d, _d, _i, _d]),
     "dugma_energy": (_ci, [_vp, _d, _d, _ci, _d]),
     "dugma_energy_grad": (_ci, [_vp, _d, _d, _ci, _d, _d]),
+    "dugma_register_energy": (_ci, [_vp, _d, _d, ctypes.POINTER(RegisterEnergyOpts), _d, _d, _i, _d]),
     "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),
     "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),
     "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),

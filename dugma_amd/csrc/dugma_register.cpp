@@ -439,3 +439,160 @@ extern "C" int dugma_register(const double *X, const double *X_C, const double *
   if (sig_out) *sig_out = sig;
   return DUGMA_OK;
 }
+
+// ---- registration on the convolved energy ----------------------------------
+
+namespace dugma {
+
+// Pose after a step s in scaled parameters (rotation part times L):
+// R <- exp([s_w / L]x) R (2-D: Rot(s_0 / L) R), t <- t + s_t.
+static void energy_step(int D, const double *R, const double *t, const double *s, double L,
+                        double *Rn, double *tn) {
+  double p[6];
+  const int nr = D == 3 ? 3 : 1, np = D == 3 ? 6 : 3;
+  for (int a = 0; a < np; ++a) p[a] = a < nr ? s[a] / L : s[a];
+  const double zero[3] = {0, 0, 0};
+  double dR[9], dt[3];
+  apply(D, R, zero, p, dR, dt);  // dR = exp(..) R, dt = the translation part
+  memcpy(Rn, dR, sizeof(double) * D * D);
+  for (int a = 0; a < D; ++a) tn[a] = t[a] + dt[a];
+}
+
+}  // namespace dugma
+
+extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const double *t0,
+                                     const dugma_register_energy_opts *opts, double *R, double *t,
+                                     int *iterations, double *E_out) {
+  if (!ctx || !R0 || !t0 || !R || !t) return fail(DUGMA_ERR_ARG, "dugma_register_energy: null argument");
+  dugma_register_energy_opts op = {200, 1e-7, 1e-14, DUGMA_COV_AUTO};
+  if (opts) op = *opts;
+  if (op.max_iter < 0 || !(op.tol_rot >= 0) || !(op.tol_trans >= 0))
+    return fail(DUGMA_ERR_ARG, "dugma_register_energy: max_iter and tolerances must be >= 0");
+  const int D = ctx->D, DD = D * D, np = D == 3 ? 6 : 3, nr = D == 3 ? 3 : 1;
+  if (ctx->nS == 0 || ctx->nT == 0)
+    return fail(DUGMA_ERR_STATE, "dugma_register_energy: no valid target or source point");
+  // Length scales from the kept source points (host copy, scaled units):
+  // L, the RMS radius about their centroid; w0, the per-axis kernel width
+  // (source variance counted twice for the target's).
+  double L = 0.0, w0 = 0.0;
+  {
+    const double *src = ctx->hSsrc.data();
+    const size_t n = ctx->nS, st = (size_t)(D + DD);
+    double c[3] = {0, 0, 0}, var = 0.0;
+    for (size_t k = 0; k < n; ++k)
+      for (int a = 0; a < D; ++a) c[a] += src[k * st + a] / (double)n;
+    for (size_t k = 0; k < n; ++k) {
+      for (int a = 0; a < D; ++a) {
+        const double d = src[k * st + a] - c[a];
+        L += d * d / (double)n;
+        var += src[k * st + D + a * (D + 1)] / (double)(n * D);
+      }
+    }
+    L = std::sqrt(L) / ctx->scale;
+    w0 = std::sqrt(2.0 * std::max(var, 0.0)) / ctx->scale;
+  }
+  if (!(L > 0) || !std::isfinite(L)) L = 1.0;
+  if (!(w0 > 0) || !std::isfinite(w0)) w0 = 1e-3 * L;
+  const double step_max = 0.25 * L;
+
+  double cR[9], ct[3], E, g[6];
+  memcpy(cR, R0, sizeof(double) * DD);
+  memcpy(ct, t0, sizeof(double) * D);
+  int rc = dugma_energy_grad(ctx, cR, ct, op.cov_mode, &E, g);
+  if (rc) return rc;
+  if (!(E > 0) || !std::isfinite(E))
+    return fail(DUGMA_ERR_STATE, "dugma_register_energy: the energy is zero at the start pose");
+  // gradient of f = -log E in the scaled parameters
+  auto grad_q = [&](double e, const double *gp, double *gq) {
+    for (int a = 0; a < np; ++a) gq[a] = a < nr ? -gp[a] / (e * L) : -gp[a] / e;
+  };
+  double gq[6], H[36];
+  grad_q(E, g, gq);
+  auto reset = [&]() {
+    for (int a = 0; a < np * np; ++a) H[a] = (a % (np + 1) == 0) ? 1.0 : 0.0;
+  };
+  reset();
+  bool scaled = false;
+  int it = 0;
+  while (it < op.max_iter) {
+    double d[6], slope = 0.0, dn = 0.0;
+    for (int a = 0; a < np; ++a) {
+      double v = 0.0;
+      for (int b = 0; b < np; ++b) v -= H[a * np + b] * gq[b];
+      d[a] = v;
+      slope += gq[a] * v;
+    }
+    if (!(slope < 0.0)) {  // not a descent direction: restart from the gradient
+      reset();
+      scaled = false;
+      slope = 0.0;
+      for (int a = 0; a < np; ++a) {
+        d[a] = -gq[a];
+        slope -= gq[a] * gq[a];
+      }
+      if (!(slope < 0.0)) break;  // zero gradient
+    }
+    for (int a = 0; a < np; ++a) dn += d[a] * d[a];
+    dn = std::sqrt(dn);
+    double alpha = scaled ? 1.0 : w0 / dn;
+    if (alpha * dn > step_max) alpha = step_max / dn;
+    const double f = -std::log(E);
+    double s[6], Rn[9], tn[3], En = 0.0, gn[6];
+    bool ok = false;
+    for (int k = 0; k < 40 && !ok; ++k) {
+      for (int a = 0; a < np; ++a) s[a] = alpha * d[a];
+      energy_step(D, cR, ct, s, L, Rn, tn);
+      rc = dugma_energy_grad(ctx, Rn, tn, op.cov_mode, &En, gn);
+      if (rc) return rc;
+      if (En > 0 && std::isfinite(En) && -std::log(En) <= f + 1e-4 * alpha * slope)
+        ok = true;
+      else
+        alpha *= 0.5;
+    }
+    if (!ok) break;  // no decrease along the direction: stationary to working precision
+    ++it;
+    double gqn[6], y[6], sy = 0.0, yy = 0.0, ss = 0.0;
+    grad_q(En, gn, gqn);
+    for (int a = 0; a < np; ++a) {
+      y[a] = gqn[a] - gq[a];
+      sy += s[a] * y[a];
+      yy += y[a] * y[a];
+      ss += s[a] * s[a];
+    }
+    if (sy > 1e-12 * std::sqrt(ss * yy)) {
+      if (!scaled) {  // first curvature pair: scale the initial inverse Hessian
+        for (int a = 0; a < np * np; ++a) H[a] *= sy / yy;
+        scaled = true;
+      }
+      // H <- (I - r s y^T) H (I - r y s^T) + r s s^T, r = 1 / s^T y
+      const double r = 1.0 / sy;
+      double Hy[6], yHy = 0.0;
+      for (int a = 0; a < np; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < np; ++b) v += H[a * np + b] * y[b];
+        Hy[a] = v;
+        yHy += y[a] * v;
+      }
+      for (int a = 0; a < np; ++a)
+        for (int b = 0; b < np; ++b)
+          H[a * np + b] += -r * (Hy[a] * s[b] + s[a] * Hy[b]) + (r * r * yHy + r) * s[a] * s[b];
+    }
+    memcpy(cR, Rn, sizeof(double) * DD);
+    memcpy(ct, tn, sizeof(double) * D);
+    E = En;
+    memcpy(gq, gqn, sizeof(gq));
+    double ang = 0.0, tt = 0.0;
+    for (int a = 0; a < np; ++a) {
+      if (a < nr)
+        ang += (s[a] / L) * (s[a] / L);
+      else
+        tt += s[a] * s[a];
+    }
+    if (std::sqrt(ang) < op.tol_rot && tt < op.tol_trans) break;
+  }
+  memcpy(R, cR, sizeof(double) * DD);
+  memcpy(t, ct, sizeof(double) * D);
+  if (iterations) *iterations = it;
+  if (E_out) *E_out = E;
+  return DUGMA_OK;
+}

@@ -126,6 +126,30 @@ int dugma_energy(dugma_ctx *ctx, const double *R, const double *t, int cov_mode,
 int dugma_energy_grad(dugma_ctx *ctx, const double *R, const double *t,
                       int cov_mode, double *E, double *gpose);
 
+/* Registration on the convolved energy itself: quasi-Newton (BFGS) descent
+ * of -log E(R, t) over left pose increments R <- exp([w]x) R (2-D: a
+ * rotation by theta on the left), t <- t + dt, re-centred on every accepted
+ * pose, each trial pose evaluated by one fused energy + pose-gradient pass
+ * (dugma_energy_grad); backtracking (Armijo) line search.  This is the
+ * "SE(3) step on the convolved energy" beside the surrogate minimiser of
+ * dugma_register; it has no reference counterpart (DUGMA.m minimises the
+ * compi surrogate, DUGMA.m:57-123).  Rotation parameters are scaled by the
+ * RMS radius of the source cloud so that both blocks are lengths; the first
+ * step moves by about one kernel width.  Stops when a step rotates by less
+ * than tol_rot (rad) and translates by less than sqrt(tol_trans), when no
+ * decrease is found along the search direction, or after max_iter accepted
+ * steps.  iterations = accepted steps; E = energy at the returned pose.
+ * opts may be NULL (max_iter 200, tol_rot 1e-7, tol_trans 1e-14, AUTO). */
+typedef struct {
+  int max_iter;
+  double tol_rot;
+  double tol_trans;
+  int cov_mode;
+} dugma_register_energy_opts;
+int dugma_register_energy(dugma_ctx *ctx, const double *R0, const double *t0,
+                          const dugma_register_energy_opts *opts, double *R, double *t,
+                          int *iterations, double *E);
+
 /* Energy with per-point gradients (no N x M storage: one target-major and
  * one source-major pass).  Any output may be NULL.
  *  gX  : N x D      dE/dX_i            gY  : M x D      dE/dY_j

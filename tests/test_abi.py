@@ -110,7 +110,8 @@ def test_option_structs_match_the_header(tmp_path):
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
-    structs = {"dugma_batch_opts": _lib.BatchOpts, "dugma_register_opts": _lib.RegisterOpts}
+    structs = {"dugma_batch_opts": _lib.BatchOpts, "dugma_register_opts": _lib.RegisterOpts,
+               "dugma_register_energy_opts": _lib.RegisterEnergyOpts}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "dugma.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
