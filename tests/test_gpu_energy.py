@@ -320,3 +320,49 @@ def test_points_repeatable_and_shape_independent():
     assert abs(c["E"] / a["E"] - 1) <= E_TOL
     for key in ("gX", "gXC", "gY", "gYC", "gpose", "gdT"):
         assert rel(c[key], a[key]) <= G_TOL, key
+
+
+def test_energy_full_size_config3(oracle):
+    """The fused pass at the large configuration's full size (200,000 x
+    200,000, isotropic), through properties that do not need an all-pairs
+    FP64 reference:
+      * 512 sampled targets against all 200,000 sources match the FP64
+        oracle (energy and pose gradient);
+      * the energy and pose gradient are additive over a split of the
+        targets (the two halves' contexts against the whole one's);
+      * the fused pose gradient agrees with central differences of the
+        GPU energy itself."""
+    d = W.config3()
+    R, t = perturbed_pose(d, deg=1.0, dt=0.003, seed=31)
+    XC = W.unblocks(d["X_C"])
+    N = d["X"].shape[1]
+
+    def sub(idx):
+        return dict(X=d["X"][:, idx].copy(), X_C=W.blocks(XC[idx]), Y=d["Y"], Y_C=d["Y_C"])
+
+    def run(dd, fn):
+        ok, h = dg.init(dd["X"], dd["X_C"], dd["Y"], dd["Y_C"])
+        try:
+            return fn(h)
+        finally:
+            dg.fr_m(h)
+
+    q = np.sort(np.random.default_rng(4).choice(N, 512, replace=False))
+    s = sub(q)
+    E_s, g_s = run(s, lambda h: dg.energy_grad(h, R, t))
+    Y, SY, X, SX = as_oracle(s)
+    ref = oracle.energy(Y, SY, X, SX, R, t, grads=True)
+    assert abs(E_s / ref["E"] - 1) <= E_TOL, (E_s, ref["E"])
+    assert rel(g_s, ref["gpose"]) <= G_TOL
+
+    def full(h):
+        E, g = dg.energy_grad(h, R, t)
+        fd = dg.energy_grad_fd(h, R, t, step=1e-4)[1]
+        return E, g, fd
+    E, g, gfd = run(d, full)
+    half = np.arange(N) < N // 2
+    E1, g1 = run(sub(np.flatnonzero(half)), lambda h: dg.energy_grad(h, R, t))
+    E2, g2 = run(sub(np.flatnonzero(~half)), lambda h: dg.energy_grad(h, R, t))
+    assert abs((E1 + E2) / E - 1) <= E_TOL
+    assert rel(np.asarray(g1) + np.asarray(g2), g) <= G_TOL
+    assert rel(gfd, g) < 1e-3  # FD truncation at h = 1e-4 and FP32 rounding
