@@ -312,21 +312,36 @@ __global__ __launch_bounds__(kSBlock) void k_grid_bbox(const float *__restrict__
 template <int D>
 __global__ void k_grid_params(const float *__restrict__ part, int groups, long long cap,
                               GridParams *__restrict__ gp) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  // one wave: each lane reads its share of the group partials (the loads
+  // overlap), then the wave reduces them; lane 0 sizes the grid
+  const int lane = threadIdx.x;
+  float flo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+  float fhi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
   int n = 0;
-  bool any = false;
-  for (int g = 0; g < groups; ++g) {
+  for (int g = lane; g < groups; g += 64) {
     const float *o = part + (size_t)g * 8;
     const int c = __float_as_int(o[3]);
     if (c == 0) continue;
     n += c;
     for (int a = 0; a < D; ++a) {
-      lo[a] = any ? fmin(lo[a], (double)o[a]) : (double)o[a];
-      hi[a] = any ? fmax(hi[a], (double)o[4 + a]) : (double)o[4 + a];
+      flo[a] = fminf(flo[a], o[a]);
+      fhi[a] = fmaxf(fhi[a], o[4 + a]);
     }
-    any = true;
   }
+  for (int m = 32; m >= 1; m >>= 1) {
+    n += __shfl_xor(n, m, 64);
+    for (int a = 0; a < D; ++a) {
+      flo[a] = fminf(flo[a], __shfl_xor(flo[a], m, 64));
+      fhi[a] = fmaxf(fhi[a], __shfl_xor(fhi[a], m, 64));
+    }
+  }
+  if (lane != 0 || blockIdx.x != 0) return;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  if (n > 0)
+    for (int a = 0; a < D; ++a) {
+      lo[a] = (double)flo[a];
+      hi[a] = (double)fhi[a];
+    }
   GridParams p;
   p.ox = lo[0];
   p.oy = lo[1];
@@ -421,6 +436,43 @@ __global__ __launch_bounds__(kSBlock) void k_grid_scatter(const float *__restric
   sidx[pos] = j;
 }
 
+// Queries into cell order too, so that the lanes of a wave search
+// neighbouring cells (shared ring extents, nearby candidate loads).  The key
+// is the query's cell, or the number of cells for a query with a
+// non-finite coordinate (answered without a search).  Each query's result
+// does not depend on the order.
+template <int D>
+__device__ __forceinline__ int query_key(const float *__restrict__ Q, int i, const GridParams &g) {
+  float x[D];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    x[a] = Q[(size_t)i * D + a];
+    fin = fin && isfinite(x[a]);
+  }
+  return fin ? grid_cell<D>(x, g) : g.nx * g.ny * g.nz;
+}
+
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_query_count(const float *__restrict__ Q, int nq,
+                                                         const GridParams *__restrict__ gp,
+                                                         int *__restrict__ cnt) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i >= nq) return;
+  atomicAdd(cnt + query_key<D>(Q, i, *gp), 1);
+}
+
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_query_scatter(const float *__restrict__ Q, int nq,
+                                                           const GridParams *__restrict__ gp,
+                                                           const int *__restrict__ start,
+                                                           int *__restrict__ cnt, int *__restrict__ qord) {
+  const int i = blockIdx.x * kSBlock + threadIdx.x;
+  if (i >= nq) return;
+  const int key = query_key<D>(Q, i, *gp);
+  qord[start[key] + atomicSub(cnt + key, 1) - 1] = i;
+}
+
 template <int K>
 __device__ __forceinline__ void topk_insert_lex(float *bd, int *bi, float dd, int j) {
   if (!(dd < bd[K - 1] || (dd == bd[K - 1] && j < bi[K - 1]))) return;
@@ -500,11 +552,12 @@ __device__ __forceinline__ double unvisited_dist2(const float *q, int cx, int cy
 
 template <int D, int K>
 __global__ __launch_bounds__(kSBlock) void k_knn_grid(
-    const float *__restrict__ Q, int nq, const float *__restrict__ spts, const int *__restrict__ sidx,
-    const int *__restrict__ start, const GridParams *__restrict__ gp, int self_exclude, int k,
-    float *__restrict__ od, int *__restrict__ oi) {
-  const int i = blockIdx.x * kSBlock + threadIdx.x;
-  if (i >= nq) return;
+    const float *__restrict__ Q, int nq, const int *__restrict__ qord, const float *__restrict__ spts,
+    const int *__restrict__ sidx, const int *__restrict__ start, const GridParams *__restrict__ gp,
+    int self_exclude, int k, float *__restrict__ od, int *__restrict__ oi) {
+  const int lane = blockIdx.x * kSBlock + threadIdx.x;
+  if (lane >= nq) return;
+  const int i = qord[lane];
   const GridParams g = *gp;
   float q[D];
   bool fin = true;
@@ -660,8 +713,8 @@ static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const do
   const size_t need_m = R ? al((size_t)np * D * 4) : 0;
   const size_t b_part = al((size_t)kBoxGroups * 8 * 4), b_gp = al(sizeof(GridParams));
   const size_t b_cnt = al((size_t)(cap + 1) * 4), b_pts = al((size_t)np * D * 4), b_idx = al((size_t)np * 4);
-  const size_t b_tmp = al(temp_bytes), b_out = al((size_t)nq * k * 4);
-  int rc = c->scratch2.ensure(need_m + b_part + b_gp + 2 * b_cnt + b_pts + b_idx + b_tmp + 2 * b_out);
+  const size_t b_tmp = al(temp_bytes), b_out = al((size_t)nq * k * 4), b_ord = al((size_t)nq * 4);
+  int rc = c->scratch2.ensure(need_m + b_part + b_gp + 3 * b_cnt + b_pts + b_idx + b_tmp + 2 * b_out + b_ord);
   if (rc) return rc;
   char *p = (char *)c->scratch2.ptr;
   if (R) {
@@ -678,6 +731,10 @@ static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const do
   p += b_cnt;
   int *start = (int *)p;
   p += b_cnt;
+  int *qstart = (int *)p;
+  p += b_cnt;
+  int *qord = (int *)p;
+  p += b_ord;
   float *spts = (float *)p;
   p += b_pts;
   int *sidx = (int *)p;
@@ -706,8 +763,20 @@ static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const do
     hipLaunchKernelGGL(k_grid_scatter<3>, pgrid, blk, 0, c->stream, Pp, np, gp, start, cnt, spts, sidx);
   else
     hipLaunchKernelGGL(k_grid_scatter<2>, pgrid, blk, 0, c->stream, Pp, np, gp, start, cnt, spts, sidx);
-#define KNN_GRID(DD, KK)                                                                         \
-  hipLaunchKernelGGL((k_knn_grid<DD, KK>), qgrid, blk, 0, c->stream, Qp, nq, spts, sidx, start, gp, \
+  // the candidate scatter has counted cnt back down to zero: reuse it
+  if (D == 3)
+    hipLaunchKernelGGL(k_query_count<3>, qgrid, blk, 0, c->stream, Qp, nq, gp, cnt);
+  else
+    hipLaunchKernelGGL(k_query_count<2>, qgrid, blk, 0, c->stream, Qp, nq, gp, cnt);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, temp_bytes, cnt, qstart, (int)(cap + 1), c->stream));
+  if (D == 3)
+    hipLaunchKernelGGL(k_query_scatter<3>, qgrid, blk, 0, c->stream, Qp, nq, gp, qstart, cnt, qord);
+  else
+    hipLaunchKernelGGL(k_query_scatter<2>, qgrid, blk, 0, c->stream, Qp, nq, gp, qstart, cnt, qord);
+  DUGMA_HIP(hipGetLastError());
+#define KNN_GRID(DD, KK)                                                                               \
+  hipLaunchKernelGGL((k_knn_grid<DD, KK>), qgrid, blk, 0, c->stream, Qp, nq, qord, spts, sidx, start, gp, \
                      self, k, od, oi)
   if (D == 3) {
     if (K == 8) KNN_GRID(3, 8); else if (K == 16) KNN_GRID(3, 16); else KNN_GRID(3, 32);
