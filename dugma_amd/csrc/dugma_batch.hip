@@ -61,8 +61,8 @@ constexpr int kClasses = 4;       // size classes by source count: < 1000, < 200
 constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
 constexpr double kGMin = 1e-6;    // smallest variance scale g
 // variances are held times ln 4 = 2 ln 2, so exp2(-|d|^2 / s') = exp(-|d|^2 / 2s)
-constexpr double kLn4 = 1.3862943611198906;
-constexpr double kLn4Pow1p5 = 1.632236874939246;  // (ln 4)^(3/2): the factor e is short by
+// (kLn4 and kLn4Pow1p5, the factor e is then short by, are shared with the
+// isotropic pair records)
 
 static inline int size_class(int ns) { return ns < 1000 ? 0 : ns < 2000 ? 1 : ns < 4000 ? 2 : 3; }
 

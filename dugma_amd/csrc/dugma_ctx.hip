@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "dugma_internal.h"
+#include "dugma_pairs.h"
 
 namespace dugma {
 
@@ -249,7 +250,7 @@ static void make_target_records(int D, const std::vector<int> &keep,
       r[k].s23 = (float)(sym32(S, 3, 1, 2) * sc2);
       if (ri) {
         ri[k].hx = r[k].hx; ri[k].hy = r[k].hy; ri[k].hz = r[k].hz;
-        ri[k].s = r[k].s11;
+        ri[k].s = (float)((double)r[k].s11 * kLn4);  // isotropic records: ln 4 times the variance
       }
     }
   } else {
@@ -270,7 +271,7 @@ static void make_target_records(int D, const std::vector<int> &keep,
       r[k].s12 = (float)(sym32(S, 2, 0, 1) * sc2);
       if (ri) {
         ri[k].hx = r[k].hx; ri[k].hy = r[k].hy;
-        ri[k].s = r[k].s11;
+        ri[k].s = (float)((double)r[k].s11 * kLn4);  // isotropic records: ln 4 times the variance
       }
     }
   }

@@ -89,6 +89,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
       for (int a = 0; a < NA; ++a) dacc[k][a] += (double)acc[k][a];
   }
 
+  if (!RecTraits<Rec>::FULL) {  // isotropic records: variances were times ln 4
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      dacc[k][0] *= RecTraits<Rec>::ESCALE;
+#pragma unroll
+      for (int a = 1; a < NA; ++a) dacc[k][a] *= RecTraits<Rec>::GSCALE;
+    }
+  }
+
   if (MODE == kModePoint) {
 #pragma unroll
     for (int k = 0; k < T; ++k) {
@@ -234,13 +243,13 @@ __global__ __launch_bounds__(kBlock) void k_transform_src(
       Rec3I r;
       r.hx = hi[0]; r.hy = hi[1]; r.hz = hi[2];
       r.lx = lo[0]; r.ly = lo[1]; r.lz = lo[2];
-      r.s = (float)S[0];
+      r.s = (float)(S[0] * kLn4);
       r.pad0 = 0.0f;
       reinterpret_cast<Rec3I *>(outv)[k] = r;
     } else {
       Rec2I r;
       r.hx = hi[0]; r.hy = hi[1]; r.lx = lo[0]; r.ly = lo[1];
-      r.s = (float)S[0];
+      r.s = (float)(S[0] * kLn4);
       r.pad0 = r.pad1 = r.pad2 = 0.0f;
       reinterpret_cast<Rec2I *>(outv)[k] = r;
     }

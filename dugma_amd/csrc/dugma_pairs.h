@@ -17,6 +17,18 @@ namespace dugma {
 
 constexpr float kNegHalfLog2e = -0.72134752044448170368f;  // -log2(e)/2
 
+// Isotropic records hold their variance times ln 4 = 2 ln 2, so that
+// exp2(-|d|^2 / s') = exp(-|d|^2 / 2s) takes no constant multiply per pair.
+// Per pair, e then comes out short by (ln 4)^(D/2) and e/s by
+// (ln 4)^(D/2 + 1); the kernel epilogue restores both in FP64 (ESCALE,
+// GSCALE below).  The full-covariance records keep plain covariances: their
+// adjoint term S^-1 d d^T S^-1 - S^-1 would not scale uniformly.
+constexpr double kLn4 = 1.3862943611198906;
+constexpr float kLn4f = 1.38629436f;
+constexpr double kLn4Pow1p5 = 1.632236874939246;
+constexpr double kLn4Pow2 = 1.9218120556728056;
+constexpr double kLn4Pow2p5 = 2.2627607757402286;
+
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -62,11 +74,29 @@ __device__ __forceinline__ void half_wave_sums(double v, double &lo, double &hi)
   hi = r2 + r3;
 }
 
+// ESCALE / GSCALE: factors that restore the energy / the gradient
+// accumulators of a record kind (1 for full covariance).
 template <class Rec> struct RecTraits;
-template <> struct RecTraits<Rec3F> { static constexpr int D = 3, NACC = kAcc3F; static constexpr bool FULL = true; };
-template <> struct RecTraits<Rec3I> { static constexpr int D = 3, NACC = kAcc3I; static constexpr bool FULL = false; };
-template <> struct RecTraits<Rec2F> { static constexpr int D = 2, NACC = kAcc2F; static constexpr bool FULL = true; };
-template <> struct RecTraits<Rec2I> { static constexpr int D = 2, NACC = kAcc2I; static constexpr bool FULL = false; };
+template <> struct RecTraits<Rec3F> {
+  static constexpr int D = 3, NACC = kAcc3F;
+  static constexpr bool FULL = true;
+  static constexpr double ESCALE = 1.0, GSCALE = 1.0;
+};
+template <> struct RecTraits<Rec3I> {
+  static constexpr int D = 3, NACC = kAcc3I;
+  static constexpr bool FULL = false;
+  static constexpr double ESCALE = kLn4Pow1p5, GSCALE = kLn4Pow2p5;
+};
+template <> struct RecTraits<Rec2F> {
+  static constexpr int D = 2, NACC = kAcc2F;
+  static constexpr bool FULL = true;
+  static constexpr double ESCALE = 1.0, GSCALE = 1.0;
+};
+template <> struct RecTraits<Rec2I> {
+  static constexpr int D = 2, NACC = kAcc2I;
+  static constexpr bool FULL = false;
+  static constexpr double ESCALE = kLn4, GSCALE = kLn4Pow2;
+};
 
 // GRAD: 0 energy only; 1 pose adjoint; 2 per-point adjoint (isotropic
 // records then also accumulate G).  COMP: how the FP32 hi/lo position split
@@ -127,17 +157,17 @@ __device__ __forceinline__ void pair_eval(const Rec3I &p, const Rec3I &q, float 
   DUGMA_DIFF(y)
   DUGMA_DIFF(z)
   const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  const float s = p.s + q.s;
+  const float s = p.s + q.s;  // ln 4 times the variance
   const float rs = frsq(s);
   const float inv = rs * rs;  // 1/s
-  const float e = fexp2((d2 * inv) * kNegHalfLog2e) * (rs * inv);  // s^-3/2
+  const float e = fexp2(-(d2 * inv)) * (rs * inv);  // s^-3/2
   acc[0] += e;
   if (GRAD) {
     const float einv = e * inv;
     acc[1] = fmaf(einv, dx, acc[1]);
     acc[2] = fmaf(einv, dy, acc[2]);
     acc[3] = fmaf(einv, dz, acc[3]);
-    if (GRAD == 2) acc[4] = fmaf(einv, fmaf(d2, inv, -3.0f), acc[4]);
+    if (GRAD == 2) acc[4] = fmaf(einv, fmaf(d2 * inv, kLn4f, -3.0f), acc[4]);
   }
 }
 
@@ -173,14 +203,14 @@ __device__ __forceinline__ void pair_eval(const Rec2I &p, const Rec2I &q, float 
   DUGMA_DIFF(x)
   DUGMA_DIFF(y)
   const float d2 = fmaf(dx, dx, dy * dy);
-  const float inv = frcp(p.s + q.s);
-  const float e = fexp2((d2 * inv) * kNegHalfLog2e) * inv;  // s^-1
+  const float inv = frcp(p.s + q.s);  // 1 / (ln 4 times the variance)
+  const float e = fexp2(-(d2 * inv)) * inv;  // s^-1
   acc[0] += e;
   if (GRAD) {
     const float einv = e * inv;
     acc[1] = fmaf(einv, dx, acc[1]);
     acc[2] = fmaf(einv, dy, acc[2]);
-    if (GRAD == 2) acc[3] = fmaf(einv, fmaf(d2, inv, -2.0f), acc[3]);
+    if (GRAD == 2) acc[3] = fmaf(einv, fmaf(d2 * inv, kLn4f, -2.0f), acc[3]);
   }
 }
 
