@@ -89,12 +89,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
       for (int a = 0; a < NA; ++a) dacc[k][a] += (double)acc[k][a];
   }
 
-  if (!RecTraits<Rec>::FULL) {  // isotropic records: variances were times ln 4
+  if (!RecTraits<Rec>::FULL) {
+    // isotropic records: variances were times ln 4.  A pose pass with COMP 2
+    // has the transformed sources stationary: d = x' - y, so the gradient
+    // sums change sign, and the torque is taken about the sources.
+    constexpr double gs = (MODE == kModePose && COMP == 2) ? -RecTraits<Rec>::GSCALE : RecTraits<Rec>::GSCALE;
 #pragma unroll
     for (int k = 0; k < T; ++k) {
       dacc[k][0] *= RecTraits<Rec>::ESCALE;
 #pragma unroll
-      for (int a = 1; a < NA; ++a) dacc[k][a] *= RecTraits<Rec>::GSCALE;
+      for (int a = 1; a < NA; ++a) dacc[k][a] *= gs;
     }
   }
 
@@ -360,10 +364,20 @@ static void launch_pairs(int mode, int comp, const Plan &pl, const Rec *P,
                          int nP, const Rec *Q, int nQ, double *out,
                          const double *t, hipStream_t s) {
 #define DUGMA_L(M, C) launch_pairs_c<Rec, M, C>(pl, P, nP, Q, nQ, out, t, s)
-  if (mode == kModeEnergy) {
-    if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
-  } else if (mode == kModePose) {
-    if (comp == 1) DUGMA_L(kModePose, 1); else DUGMA_L(kModePose, 0);
+  if (mode == kModeEnergy || mode == kModePose) {
+    // comp 2 in these passes: isotropic records only, with the transformed
+    // sources stationary (see pose_pass)
+    if constexpr (!RecTraits<Rec>::FULL) {
+      if (comp == 2) {
+        if (mode == kModeEnergy) DUGMA_L(kModeEnergy, 2); else DUGMA_L(kModePose, 2);
+        return;
+      }
+    }
+    if (mode == kModeEnergy) {
+      if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
+    } else {
+      if (comp == 1) DUGMA_L(kModePose, 1); else DUGMA_L(kModePose, 0);
+    }
   } else {
     if (comp == 1) DUGMA_L(kModePoint, 1);
     else if (comp == 2) DUGMA_L(kModePoint, 2);
@@ -433,7 +447,15 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   // everything the launches need is settled first, so that the three
   // launches leave the host back to back (the transform takes ~3 us)
   PoseArg ps = make_pose(c, R, t);
-  Plan pl = ctx_plan(c, c->nT, c->nS, mode, iso);
+  // Isotropic passes keep the transformed sources (FP32 hi + lo) stationary
+  // and stream the targets (exact in FP32): the lo correction is then an
+  // operation on VGPRs only (COMP 2) instead of a second one on the streamed
+  // SGPRs (COMP 1).  E is symmetric in the two roles; the gradient sums
+  // change sign (kernel epilogue).  Full-covariance passes keep the targets
+  // stationary: their torque term uses the stationary covariance.
+  const bool swap = iso && comp_enabled();
+  const int comp = comp_enabled() ? (swap ? 2 : 1) : 0;
+  Plan pl = swap ? ctx_plan(c, c->nS, c->nT, mode, iso) : ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
   // few partials: the pair kernel writes them to pinned host memory and the
   // host sums them after the sync; many: a device reduction launch
@@ -445,23 +467,24 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
     if (rc) return rc;
     parts = (double *)c->part.ptr;
   }
-  const int comp = comp_enabled() ? 1 : 0;
   const int groups = (int)std::min<size_t>(kRedGroups, (nparts + kBlock - 1) / kBlock);
   if (!host_sum) preset_slots(c->hres, kPosePart * groups);
   rc = transform_sources(c, ps, iso);
   if (rc) return rc;
   PairTimer timer(c);
+  const void *isoP = swap ? c->dSrecI : c->dTrecI, *isoQ = swap ? c->dTrecI : c->dSrecI;
+  const int isoNP = swap ? c->nS : c->nT, isoNQ = swap ? c->nT : c->nS;
   if (D == 3) {
     if (iso)
-      launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)c->dTrecI, c->nT, (const Rec3I *)c->dSrecI, c->nS,
-                          parts, ps.t, c->stream);
+      launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)isoP, isoNP, (const Rec3I *)isoQ, isoNQ, parts, ps.t,
+                          c->stream);
     else
       launch_pairs<Rec3F>(mode, comp, pl, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS,
                           parts, ps.t, c->stream);
   } else {
     if (iso)
-      launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)c->dTrecI, c->nT, (const Rec2I *)c->dSrecI, c->nS,
-                          parts, ps.t, c->stream);
+      launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)isoP, isoNP, (const Rec2I *)isoQ, isoNQ, parts, ps.t,
+                          c->stream);
     else
       launch_pairs<Rec2F>(mode, comp, pl, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS,
                           parts, ps.t, c->stream);
