@@ -89,11 +89,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
       for (int a = 0; a < NA; ++a) dacc[k][a] += (double)acc[k][a];
   }
 
-  if (!RecTraits<Rec>::FULL) {
-    // isotropic records: variances were times ln 4.  A pose pass with COMP 2
-    // has the transformed sources stationary: d = x' - y, so the gradient
-    // sums change sign, and the torque is taken about the sources.
-    constexpr double gs = (MODE == kModePose && COMP == 2) ? -RecTraits<Rec>::GSCALE : RecTraits<Rec>::GSCALE;
+  // Isotropic records: variances were times ln 4 (ESCALE, GSCALE).  A pose
+  // pass with COMP 2 has the transformed sources stationary: d = x' - y
+  // flips the sign of EW (and of nothing else: B holds d d^T), and the
+  // gradient is then taken about the sources, whose rotation also turns
+  // their covariances.  With EW and B both negated, pose_contrib's
+  // (p - t) x EW + vee(S_p B - B S_p) is exactly that gradient.
+  constexpr bool kSwapped = MODE == kModePose && COMP == 2;
+  if (!RecTraits<Rec>::FULL || kSwapped) {
+    constexpr double gs = kSwapped ? -RecTraits<Rec>::GSCALE : RecTraits<Rec>::GSCALE;
 #pragma unroll
     for (int k = 0; k < T; ++k) {
       dacc[k][0] *= RecTraits<Rec>::ESCALE;
@@ -364,20 +368,11 @@ static void launch_pairs(int mode, int comp, const Plan &pl, const Rec *P,
                          int nP, const Rec *Q, int nQ, double *out,
                          const double *t, hipStream_t s) {
 #define DUGMA_L(M, C) launch_pairs_c<Rec, M, C>(pl, P, nP, Q, nQ, out, t, s)
-  if (mode == kModeEnergy || mode == kModePose) {
-    // comp 2 in these passes: isotropic records only, with the transformed
-    // sources stationary (see pose_pass)
-    if constexpr (!RecTraits<Rec>::FULL) {
-      if (comp == 2) {
-        if (mode == kModeEnergy) DUGMA_L(kModeEnergy, 2); else DUGMA_L(kModePose, 2);
-        return;
-      }
-    }
-    if (mode == kModeEnergy) {
-      if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
-    } else {
-      if (comp == 1) DUGMA_L(kModePose, 1); else DUGMA_L(kModePose, 0);
-    }
+  if (mode == kModeEnergy) {
+    // comp 2 in these passes: the transformed sources are stationary (see pose_pass)
+    if (comp == 2) DUGMA_L(kModeEnergy, 2); else if (comp == 1) DUGMA_L(kModeEnergy, 1); else DUGMA_L(kModeEnergy, 0);
+  } else if (mode == kModePose) {
+    if (comp == 2) DUGMA_L(kModePose, 2); else if (comp == 1) DUGMA_L(kModePose, 1); else DUGMA_L(kModePose, 0);
   } else {
     if (comp == 1) DUGMA_L(kModePoint, 1);
     else if (comp == 2) DUGMA_L(kModePoint, 2);
@@ -447,14 +442,14 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   // everything the launches need is settled first, so that the three
   // launches leave the host back to back (the transform takes ~3 us)
   PoseArg ps = make_pose(c, R, t);
-  // Isotropic passes keep the transformed sources (FP32 hi + lo) stationary
-  // and stream the targets (exact in FP32): the lo correction is then an
+  // The passes keep the transformed sources (FP32 hi + lo) stationary and
+  // stream the targets (exact in FP32): the lo correction is then an
   // operation on VGPRs only (COMP 2) instead of a second one on the streamed
   // SGPRs (COMP 1).  E is symmetric in the two roles; the gradient sums
-  // change sign (kernel epilogue).  Full-covariance passes keep the targets
-  // stationary: their torque term uses the stationary covariance.
-  const bool swap = iso && comp_enabled();
-  const int comp = comp_enabled() ? (swap ? 2 : 1) : 0;
+  // change sign (kernel epilogue).  Without compensation the targets stay
+  // stationary (COMP 0).
+  const bool swap = comp_enabled();
+  const int comp = swap ? 2 : 0;
   Plan pl = swap ? ctx_plan(c, c->nS, c->nT, mode, iso) : ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
   // few partials: the pair kernel writes them to pinned host memory and the
@@ -472,22 +467,19 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   rc = transform_sources(c, ps, iso);
   if (rc) return rc;
   PairTimer timer(c);
-  const void *isoP = swap ? c->dSrecI : c->dTrecI, *isoQ = swap ? c->dTrecI : c->dSrecI;
-  const int isoNP = swap ? c->nS : c->nT, isoNQ = swap ? c->nT : c->nS;
+  const void *tgt = iso ? c->dTrecI : c->dTrecF, *src = iso ? c->dSrecI : c->dSrecF;
+  const void *P = swap ? src : tgt, *Q = swap ? tgt : src;
+  const int nP = swap ? c->nS : c->nT, nQ = swap ? c->nT : c->nS;
   if (D == 3) {
     if (iso)
-      launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)isoP, isoNP, (const Rec3I *)isoQ, isoNQ, parts, ps.t,
-                          c->stream);
+      launch_pairs<Rec3I>(mode, comp, pl, (const Rec3I *)P, nP, (const Rec3I *)Q, nQ, parts, ps.t, c->stream);
     else
-      launch_pairs<Rec3F>(mode, comp, pl, (const Rec3F *)c->dTrecF, c->nT, (const Rec3F *)c->dSrecF, c->nS,
-                          parts, ps.t, c->stream);
+      launch_pairs<Rec3F>(mode, comp, pl, (const Rec3F *)P, nP, (const Rec3F *)Q, nQ, parts, ps.t, c->stream);
   } else {
     if (iso)
-      launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)isoP, isoNP, (const Rec2I *)isoQ, isoNQ, parts, ps.t,
-                          c->stream);
+      launch_pairs<Rec2I>(mode, comp, pl, (const Rec2I *)P, nP, (const Rec2I *)Q, nQ, parts, ps.t, c->stream);
     else
-      launch_pairs<Rec2F>(mode, comp, pl, (const Rec2F *)c->dTrecF, c->nT, (const Rec2F *)c->dSrecF, c->nS,
-                          parts, ps.t, c->stream);
+      launch_pairs<Rec2F>(mode, comp, pl, (const Rec2F *)P, nP, (const Rec2F *)Q, nQ, parts, ps.t, c->stream);
   }
   timer.stop();
   DUGMA_HIP(hipGetLastError());
@@ -932,8 +924,15 @@ extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
     const int mode = m == 0 ? kModeEnergy : kModePose;
     dugma_ctx::Tuned best;
     float best_ms = 1e30f;
-    for (int ti = 0; ti < 2; ++ti)
+    // the stationary and streamed sets of pose_pass
+    const int nP = comp_enabled() ? ctx->nS : ctx->nT, nQ = comp_enabled() ? ctx->nT : ctx->nS;
+    for (int ti = 0; ti < 2; ++ti) {
+      int last_chunks = -1;
       for (int wi = 0; wi < kNW; ++wi) {
+        // targets that give the launch of the previous one are not timed again
+        const int chunks = make_plan(nP, nQ, mode, kT[ti], kW[wi]).qchunks;
+        if (chunks == last_chunks) continue;
+        last_chunks = chunks;
         dugma_ctx::Tuned &tu = ctx->tuned[m][iso ? 1 : 0];
         tu.T = kT[ti];
         tu.target_waves = kW[wi];
@@ -951,6 +950,7 @@ extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
           best = tu;
         }
       }
+    }
     ctx->tuned[m][iso ? 1 : 0] = best;
     if (rc) break;
   }
