@@ -51,12 +51,22 @@ FLOP_3D_FULL_FWD = int(_RC["FLOP_3D_FULL_FWD"])
 FLOP_3D_ISO_FUSED = int(_RC["FLOP_3D_ISO_FUSED"])
 FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # HBM bytes per k_pairs launch (fused, MODE 1) on the 6,000 x 6,000
-# full-covariance workload, keyed by launch shape (lane_points, target_waves):
-# rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by what the
-# counter reads for a 256 MiB copy (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
-MEASURED_TRAFFIC_6K_FULL = {(1, 4096): 6662000, (1, 8192): 6727000, (1, 16384): 6740000,
-                            (1, 32768): 6740000, (2, 4096): 7057000, (2, 8192): 7064000,
-                            (2, 16384): 7064000, (2, 32768): 7063000}
+# full-covariance workload, keyed by the launch it runs (lane_points, source
+# chunks): rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by
+# what the counter reads for a 256 MiB copy
+# (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
+MEASURED_TRAFFIC_6K_FULL = {(1, 22): 6629000, (1, 43): 6661000, (1, 86): 6727000, (1, 94): 6740000,
+                            (2, 43): 7042000, (2, 86): 7075000, (2, 94): 7081000}
+
+
+def launch_chunks(n_stationary, n_streamed, lane_points, target_waves):
+    """Streamed chunks of a pair launch, as the library sizes them: 256-lane
+    workgroups of lane_points stationary points per lane, chunks of at least
+    64 streamed points, wave count near target_waves."""
+    blocks = -(-n_stationary // (256 * lane_points))
+    qc = max(1, min(-(-target_waves // (blocks * 4)), -(-n_streamed // 64)))
+    chunk = -(-n_streamed // qc)
+    return -(-n_streamed // chunk)
 
 
 class Hip:
@@ -479,7 +489,8 @@ def main():
         "init_ms": round(fused["init_ms"], 3),
     }
     lp = fused["plan"]
-    out["roofline"]["traffic"] = MEASURED_TRAFFIC_6K_FULL.get((lp["lane_points"], lp["target_waves"]))
+    out["roofline"]["traffic"] = MEASURED_TRAFFIC_6K_FULL.get(
+        (lp["lane_points"], launch_chunks(6000, 6000, lp["lane_points"], lp["target_waves"])))
     if args.only == "fused":
         print(json.dumps(out), flush=True)
         return

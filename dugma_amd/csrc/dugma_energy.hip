@@ -430,6 +430,10 @@ static double gauss_norm(int D) {
 }
 
 static bool comp_enabled() { return env_int("DUGMA_COMPENSATED", 1) != 0; }
+// the pose and energy passes keep the transformed sources stationary unless
+// DUGMA_STATIONARY_SOURCES=0 (then the targets, with the lo correction on
+// the streamed side); both give the same sums to FP32 rounding
+static bool sources_stationary() { return comp_enabled() && env_int("DUGMA_STATIONARY_SOURCES", 1) != 0; }
 
 // Energy-only or fused pose pass; res receives [E, rotation, translation]
 // in caller units.
@@ -448,8 +452,8 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   // SGPRs (COMP 1).  E is symmetric in the two roles; the gradient sums
   // change sign (kernel epilogue).  Without compensation the targets stay
   // stationary (COMP 0).
-  const bool swap = comp_enabled();
-  const int comp = swap ? 2 : 0;
+  const bool swap = sources_stationary();
+  const int comp = swap ? 2 : (comp_enabled() ? 1 : 0);
   Plan pl = swap ? ctx_plan(c, c->nS, c->nT, mode, iso) : ctx_plan(c, c->nT, c->nS, mode, iso);
   const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
   // few partials: the pair kernel writes them to pinned host memory and the
@@ -925,7 +929,7 @@ extern "C" int dugma_autotune(dugma_ctx *ctx, int cov_mode) {
     dugma_ctx::Tuned best;
     float best_ms = 1e30f;
     // the stationary and streamed sets of pose_pass
-    const int nP = comp_enabled() ? ctx->nS : ctx->nT, nQ = comp_enabled() ? ctx->nT : ctx->nS;
+    const int nP = sources_stationary() ? ctx->nS : ctx->nT, nQ = sources_stationary() ? ctx->nT : ctx->nS;
     for (int ti = 0; ti < 2; ++ti) {
       int last_chunks = -1;
       for (int wi = 0; wi < kNW; ++wi) {
