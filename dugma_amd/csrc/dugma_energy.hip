@@ -2,8 +2,9 @@
 // per-point adjoint and the finite-difference ("plain") gradient.
 //
 // Kernel shape: a workgroup is 4 waves; each lane owns T stationary points
-// (targets, or sources in the per-point source pass) in VGPRs and walks a
-// chunk of the other cloud.  The streamed records are read with wave-uniform
+// in VGPRs (the transformed sources in the pose and energy passes; targets,
+// then sources, in the two per-point passes) and walks a chunk of the other
+// cloud.  The streamed records are read with wave-uniform
 // addresses, so they come through the scalar cache into SGPRs and feed the
 // VALU directly: no LDS, no barriers in the pair loop.  (On gfx950 an
 // instruction with an SGPR operand issues at about half the rate of an
