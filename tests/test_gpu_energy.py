@@ -366,3 +366,34 @@ def test_energy_full_size_config3(oracle):
     assert abs((E1 + E2) / E - 1) <= E_TOL
     assert rel(np.asarray(g1) + np.asarray(g2), g) <= G_TOL
     assert rel(gfd, g) < 1e-3  # FD truncation at h = 1e-4 and FP32 rounding
+
+
+@pytest.mark.parametrize("case", ["full3", "iso3", "iso2"])
+def test_stationary_set_does_not_change_results(case):
+    """The pose and energy passes keep the transformed sources stationary
+    (the gradient sums change sign, the torque is taken about the sources);
+    with DUGMA_STATIONARY_SOURCES=0 the targets are stationary.  Both give
+    the same energy and pose gradient within the parity bars."""
+    if case == "full3":
+        d = W.config2(n=1500, m=1300)
+    elif case == "iso3":
+        d = W.config3(n=3000, m=2600)
+    else:
+        d = W.config5_2d(n=900, m=800)
+    R, t = perturbed_pose(d, deg=2.0, dt=0.005, seed=41)
+    old = os.environ.get("DUGMA_STATIONARY_SOURCES")
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        out = {}
+        for flag in ("1", "0"):
+            os.environ["DUGMA_STATIONARY_SOURCES"] = flag
+            out[flag] = (dg.energy(h, R, t),) + tuple(dg.energy_grad(h, R, t))
+    finally:
+        if old is None:
+            os.environ.pop("DUGMA_STATIONARY_SOURCES", None)
+        else:
+            os.environ["DUGMA_STATIONARY_SOURCES"] = old
+        dg.fr_m(h)
+    (Ea, E2a, ga), (Eb, E2b, gb) = out["1"], out["0"]
+    assert abs(Ea / Eb - 1) <= E_TOL and abs(E2a / E2b - 1) <= E_TOL
+    assert rel(ga, gb) <= G_TOL, (ga, gb)
