@@ -55,16 +55,17 @@ FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # chunks): rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by
 # what the counter reads for a 256 MiB copy
 # (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
-MEASURED_TRAFFIC_6K_FULL = {(1, 22): 6629000, (1, 43): 6661000, (1, 86): 6727000, (1, 94): 6740000,
-                            (2, 43): 7042000, (2, 86): 7075000, (2, 94): 7081000}
+MEASURED_TRAFFIC_6K_FULL = {(1, 22): 6629000, (1, 43): 6661000, (1, 86): 6727000, (1, 167): 6852000,
+                            (1, 188): 6884000, (2, 43): 7042000, (2, 86): 7075000, (2, 167): 7137000,
+                            (2, 188): 7154000}
 
 
 def launch_chunks(n_stationary, n_streamed, lane_points, target_waves):
     """Streamed chunks of a pair launch, as the library sizes them: 256-lane
     workgroups of lane_points stationary points per lane, chunks of at least
-    64 streamed points, wave count near target_waves."""
+    32 streamed points, wave count near target_waves."""
     blocks = -(-n_stationary // (256 * lane_points))
-    qc = max(1, min(-(-target_waves // (blocks * 4)), -(-n_streamed // 64)))
+    qc = max(1, min(-(-target_waves // (blocks * 4)), -(-n_streamed // 32)))
     chunk = -(-n_streamed // qc)
     return -(-n_streamed // chunk)
 

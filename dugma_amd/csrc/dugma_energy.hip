@@ -28,6 +28,7 @@ namespace dugma {
 
 constexpr int kBlock = 256;  // 4 waves of 64
 constexpr int kSub = 64;     // FP32 sub-tile length before the FP64 flush
+constexpr int kMinChunk = 32;  // fewest streamed points per workgroup
 
 enum { kModeEnergy = 0, kModePose = 1, kModePoint = 2 };
 
@@ -330,7 +331,7 @@ static Plan make_plan(int nP, int nQ, int mode, int T, int target_waves, int max
   pl.pblocks = (nP + kBlock * T - 1) / (kBlock * T);
   const int waves_p = pl.pblocks * (kBlock / 64);
   int qc = (target_waves + waves_p - 1) / waves_p;
-  int maxc = (nQ + kSub - 1) / kSub;
+  int maxc = (nQ + kMinChunk - 1) / kMinChunk;
   if (mode == kModePoint) maxc = std::min(maxc, max_chunks);
   qc = std::max(1, std::min(qc, maxc));
   pl.qchunk = (nQ + qc - 1) / qc;
