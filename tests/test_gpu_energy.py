@@ -397,3 +397,25 @@ def test_stationary_set_does_not_change_results(case):
     (Ea, E2a, ga), (Eb, E2b, gb) = out["1"], out["0"]
     assert abs(Ea / Eb - 1) <= E_TOL and abs(E2a / E2b - 1) <= E_TOL
     assert rel(ga, gb) <= G_TOL, (ga, gb)
+
+
+@pytest.mark.parametrize("case", ["full3", "iso3"])
+def test_one_point_per_lane_matches_oracle(oracle, case):
+    """The autotuner's usual pick at 6K and 200K (one stationary point per
+    lane, many streamed chunks) against the FP64 oracle; the other tests run
+    the default plan (two per lane)."""
+    d = W.config2(n=1400, m=1200) if case == "full3" else W.config3(n=3000, m=2600)
+    R, t = perturbed_pose(d, deg=2.0, dt=0.005, seed=43)
+    saved = {k: os.environ.get(k) for k in ("DUGMA_LANE_POINTS", "DUGMA_TARGET_WAVES")}
+    os.environ["DUGMA_LANE_POINTS"] = "1"
+    os.environ["DUGMA_TARGET_WAVES"] = "16384"
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        check_energy(h, d, R, t, oracle)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        dg.fr_m(h)
