@@ -564,29 +564,7 @@ static void perturb(int D, const double *R, const double *t, const double *p,
   }
 }
 
-// Host work over n items on up to kHostThreads threads (the caller's thread
-// included); items are claimed in increasing order from a shared counter.
-constexpr size_t kHostChunk = 4096;
-constexpr unsigned kHostThreads = 8;
-template <class F>
-static void parallel_for(size_t n, F &&f) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const unsigned nt = (unsigned)std::min<size_t>(std::min(hw, kHostThreads), n);
-  std::atomic<size_t> next{0};
-  auto work = [&]() {
-    for (size_t q; (q = next.fetch_add(1)) < n;) f(q);
-  };
-  std::vector<std::thread> pool;
-  for (unsigned k = 1; k < nt; ++k) {
-    try {
-      pool.emplace_back(work);
-    } catch (...) {
-      break;  // no more threads: the ones running take the rest
-    }
-  }
-  work();
-  for (auto &th : pool) th.join();
-}
+constexpr size_t kHostChunk = 4096;  // points per host task of the per-point conversion
 
 // One per-point pass (stationary set P, streamed set Q), full covariance
 // records; writes per-point FP64 sums [nP][NACC] to out (device).

@@ -3,9 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dugma.h"
@@ -84,6 +87,29 @@ void preset_slots(double *slots, int n);
 void preset_slots(float *slots, int n);
 int wait_slots(dugma_ctx *c, const double *slots, int n);
 int wait_slots(dugma_ctx *c, const float *slots, int n);
+
+// Host work over n items on up to kHostThreads threads (the caller's thread
+// included); items are claimed in increasing order from a shared counter.
+constexpr unsigned kHostThreads = 8;
+template <class F>
+inline void parallel_for(size_t n, F &&f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nt = (unsigned)std::min<size_t>(std::min(hw, kHostThreads), n);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t q; (q = next.fetch_add(1)) < n;) f(q);
+  };
+  std::vector<std::thread> pool;
+  for (unsigned k = 1; k < nt; ++k) {
+    try {
+      pool.emplace_back(work);
+    } catch (...) {
+      break;  // no more threads: the ones running take the rest
+    }
+  }
+  work();
+  for (auto &th : pool) th.join();
+}
 
 // Grow-only device scratch buffer owned by a context.
 struct DevBuf {

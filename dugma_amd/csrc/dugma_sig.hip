@@ -701,6 +701,30 @@ extern "C" int dugma_update_sig(dugma_ctx *ctx, const double *R, const double *t
   return DUGMA_OK;
 }
 
+// The lists to the caller's arrays (pageable memory): one device-to-host
+// copy of each into the context's pinned staging buffer at DMA speed, then
+// host copies of 1 MiB pieces on a few threads.
+static int download_lists(dugma_ctx *c, const float *od, const int *oi, size_t n, float *d2, int *idx) {
+  if (!d2 && !idx) return DUGMA_OK;
+  const size_t bytes = n * 4;
+  int rc = c->hstage.ensure(2 * bytes);
+  if (rc) return rc;
+  char *h = (char *)c->hstage.ptr;
+  if (d2) DUGMA_HIP(hipMemcpyAsync(h, od, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (idx) DUGMA_HIP(hipMemcpyAsync(h + bytes, oi, bytes, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  constexpr size_t kPiece = size_t(1) << 20;
+  const size_t np = (bytes + kPiece - 1) / kPiece;
+  parallel_for(2 * np, [&](size_t q) {
+    const bool isd = q < np;
+    char *dst = isd ? (char *)d2 : (char *)idx;
+    if (!dst) return;
+    const size_t off = (isd ? q : q - np) * kPiece;
+    memcpy(dst + off, h + (isd ? 0 : bytes) + off, std::min(kPiece, bytes - off));
+  });
+  return DUGMA_OK;
+}
+
 // The grid k-NN: candidates Pp (transformed from the source cloud first when
 // R is given), queries Qp.
 static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const double *t, const float *Qp,
@@ -785,10 +809,7 @@ static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const do
   }
 #undef KNN_GRID
   DUGMA_HIP(hipGetLastError());
-  if (d2) DUGMA_HIP(hipMemcpyAsync(d2, od, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
-  if (idx) DUGMA_HIP(hipMemcpyAsync(idx, oi, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
-  return DUGMA_OK;
+  return download_lists(c, od, oi, (size_t)nq * k, d2, idx);
 }
 
 extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
@@ -855,8 +876,5 @@ extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
   else
     hipLaunchKernelGGL(k_knn_merge<32>, mgrid, block, 0, c->stream, pd, pi, nq, nchunks, k, od, oi);
   DUGMA_HIP(hipGetLastError());
-  if (d2) DUGMA_HIP(hipMemcpyAsync(d2, od, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
-  if (idx) DUGMA_HIP(hipMemcpyAsync(idx, oi, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c->stream));
-  DUGMA_HIP(hipStreamSynchronize(c->stream));
-  return DUGMA_OK;
+  return download_lists(c, od, oi, (size_t)nq * k, d2, idx);
 }

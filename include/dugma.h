@@ -199,7 +199,8 @@ enum {
  * by DUGMA_KNN_TARGET_TO_SOURCE only.  Small problems compare all pairs;
  * from 2^24 query x candidate pairs on, the candidates are binned into a
  * uniform grid and searched ring by ring with an exact stopping bound: the
- * results are the same, bit for bit. */
+ * results are the same, bit for bit.  The lists reach idx / d2 through a
+ * pinned buffer that the context keeps. */
 int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
               const double *t, int *idx, float *d2);
 
