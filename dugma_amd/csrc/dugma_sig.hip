@@ -101,24 +101,43 @@ __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
 // update_sig.cu:79-91 then 291-307 with n_c = 1: a[i] = fl32(q[i]) / 1e6
 // (IEEE division), then the pairwise tree, level by level; at each level
 // element 2k*stride absorbs element (2k+1)*stride, an odd last element is
-// carried.  One workgroup; writes sum / N (update_sig.cu:385) to out (the
-// context's pinned result slot).
+// carried.  The same tree, 6 levels at a time: each wave sums 64 aligned
+// elements with lane shuffles (lane i adds lane i + s for s = 1, 2, ..., 32;
+// lane 0 ends with the group's tree sum), the group sums form the next
+// level's array, and so on.  The values are >= 0, so padding with +0 is
+// exact and stands in for the carry of an odd last element.  One workgroup;
+// a holds at least n floats (ping-pong of the group sums); writes sum / N
+// (update_sig.cu:385) to out (the context's pinned result slot).
 __global__ __launch_bounds__(1024) void k_tree_sum_div(const unsigned *__restrict__ q,
                                                        float *__restrict__ a, int n,
                                                        float *__restrict__ out) {
-  for (int k = threadIdx.x; k < n; k += blockDim.x) a[k] = ((float)q[k]) / 1000000.0f;
-  __syncthreads();
-  long L = n, stride = 1;
-  while (L > 1) {
-    const long half = (L + 1) / 2;
-    const long npairs = (L & 1) ? half - 1 : half;
-    for (long k = threadIdx.x; k < npairs; k += blockDim.x)
-      a[k * 2 * stride] = a[k * 2 * stride] + a[k * 2 * stride + stride];
-    __syncthreads();
-    L = half;
-    stride *= 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int g0 = (n + 63) / 64;
+  float *cur = a, *next = a + g0;
+  for (int g = wave; g < g0; g += nwaves) {
+    const int i = g * 64 + lane;
+    float v = i < n ? ((float)q[i]) / 1000000.0f : 0.0f;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) v = v + __shfl_down(v, s, 64);
+    if (lane == 0) cur[g] = v;
   }
-  if (threadIdx.x == 0) out[0] = a[0] / (float)n;
+  __syncthreads();
+  for (int m = g0; m > 1;) {
+    const int groups = (m + 63) / 64;
+    for (int g = wave; g < groups; g += nwaves) {
+      const int i = g * 64 + lane;
+      float v = i < m ? cur[i] : 0.0f;
+#pragma unroll
+      for (int s = 1; s < 64; s <<= 1) v = v + __shfl_down(v, s, 64);
+      if (lane == 0) next[g] = v;
+    }
+    __syncthreads();
+    float *t = cur;
+    cur = next;
+    next = t;
+    m = groups;
+  }
+  if (threadIdx.x == 0) out[0] = cur[0] / (float)n;
 }
 
 // ---------------------------------------------------------------------------
