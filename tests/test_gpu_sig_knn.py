@@ -230,3 +230,22 @@ def test_knn_full_size_config3(oracle):
         assert (~keep).sum() == 1
         np.testing.assert_array_equal(idx_s[qi], ri[r][keep][:8])
         np.testing.assert_array_equal(d2_s[qi], rd[r][keep][:8])
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4097, 70000])
+def test_update_sig_tree_sizes(oracle, n):
+    """The pairwise tree of update_sig at sizes around its 64-element groups
+    (one, a partial group, whole groups, a carried element, three levels of
+    groups), bit-exact against the oracle."""
+    d = W.config3(n=max(n, 2), m=300)
+    X = d["X"][:, :n].copy()
+    XC = W.blocks(W.unblocks(d["X_C"])[:n])
+    rng = np.random.default_rng(n)
+    R = W.random_rotation(rng, 3.0) @ d["R_G"]
+    t = np.ravel(d["t_G"]) + rng.uniform(-0.01, 0.01, 3)
+    ok, h = dg.init(X, XC, d["Y"], d["Y_C"])
+    try:
+        s = dg.update_sig(h, R, t, h.N, h.M, 3)
+    finally:
+        dg.fr_m(h)
+    assert s == oracle.update_sig(X.T, d["Y"].T, R, t)
