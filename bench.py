@@ -14,6 +14,8 @@ line reports:
              convolved energy;
   points     energy with gradients wrt every point's position and
              covariance on two 320 x 240 depth views;
+  contours_2d  the 2-D path: fused step and finite differences on
+             3,000 x 3,000 contours;
   batched    4,096 independent registrations in one persistent launch per
              iteration, against one trial per launch;
   reference_path  update_sig, compi and a whole DUGMA.m loop;
@@ -243,6 +245,31 @@ def bench_points(hip, dg, W, calls=3):
             "flop_per_pair": FLOP_3D_FULL_FUSED}
 
 
+def bench_contours_2d(hip, dg, W, steps=200):
+    """The 2-D path: fused energy + pose adjoint (theta, t) on the largest
+    contours of the mixed 2-D / 3-D configuration (3,000 x 3,000 points,
+    isotropic sigma from the 8-NN spacing), and the same gradient by central
+    differences."""
+    d = W.config5_2d(n=3000, m=3000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        if TUNE:
+            h.autotune()
+        R, t = d["R_G"], np.ravel(d["t_G"])
+        for _ in range(5):
+            dg.energy_grad(h, R, t)
+        ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps) / steps
+        fd = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, 1e-4), steps // 10) / (steps // 10)
+        nT, nS, iso = h.energy_info()
+    finally:
+        dg.fr_m(h)
+    log("2-D contours %dx%d: %.4f ms/step, FD %.4f ms/gradient" % (nT, nS, ms, fd))
+    return {"workload": "2-D contours, %d x %d points, isotropic; fused energy + (theta, t) adjoint" % (nT, nS),
+            "ms_per_step": round(ms, 4), "value": round(nT * nS / (ms / 1e3) / 1e9, 3), "unit": "Gpair-evals/s",
+            "plain_fd_ms_per_gradient": round(fd, 4), "fused_speedup": round(fd / ms, 2),
+            "fd_scheme": "central differences in (theta, t), h=1e-4, 7 forward launches"}
+
+
 # Batched iteration: DUGMA's sigma update (the kernel variance tracks the mean
 # nearest distance) on top of an annealed extra variance, pose steps
 # over-relaxed by 1.9 once annealing has ended; the first 160 iterations on
@@ -449,7 +476,8 @@ def main():
         args.no_cpu = True
         if only in ("large", "points", "batch", "loop"):
             if only == "points":
-                print(json.dumps({"only": only, **bench_points(hip, dg, W)}))
+                print(json.dumps({"only": only, **bench_points(hip, dg, W),
+                                  "contours_2d": bench_contours_2d(hip, dg, W)}))
             elif only == "loop":
                 print(json.dumps({"only": only, **bench_reference_path(dg, W, W.config2(n=6000, m=6000))}))
             elif only == "large":
@@ -517,6 +545,7 @@ def main():
                         "knn": bench_knn(dg, d3), "register_energy": bench_register_energy(dg, d3)}
     if not args.no_points:
         out["points"] = bench_points(hip, dg, W)
+        out["contours_2d"] = bench_contours_2d(hip, dg, W)
     if not args.no_batch:
         out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_loop:
