@@ -139,6 +139,30 @@ def start_pose(d, deg=2.0, dt=0.003, seed=0):
 TUNE = True
 
 
+def c_abi_step(dg, h, R, t, fd_h=None):
+    """One step as a C caller makes it: dugma_energy_grad (or, with fd_h,
+    dugma_energy_grad_fd) through ctypes with the arguments bound once, so
+    the timed loop holds the library call and not the Python mirror's
+    argument handling.  E and the gradient land in the context's result
+    buffers on every call."""
+    from dugma_amd import _lib
+    b = h._pb
+    b.pose(R, t)
+    if fd_h is None:
+        fn, args = _lib.load().dugma_energy_grad, (h.ptr, b.Rp, b.tp, 0, b.Ep, b.gp)
+    else:
+        fn, args = _lib.load().dugma_energy_grad_fd, (h.ptr, b.Rp, b.tp, 0, ctypes.c_double(fd_h), b.Ep, b.gp)
+    return lambda: fn(*args)
+
+
+def check_step(h):
+    """The timed calls ignore the status they return: check that the last
+    one left a finite energy and gradient."""
+    b = h._pb
+    if not (np.isfinite(b.E.value) and np.all(np.isfinite(b.g))):
+        raise RuntimeError("the timed steps did not produce a finite energy and gradient")
+
+
 def bench_fused(hip, dg, d, steps, warmup, label):
     # init (host arrays -> device state, the one PCIe transfer): the second of
     # two contexts, so that code objects are already loaded
@@ -153,7 +177,9 @@ def bench_fused(hip, dg, d, steps, warmup, label):
                                           int(os.environ.get("DUGMA_TARGET_WAVES", 0)))
         for _ in range(warmup):
             dg.energy_grad(h, R, t)
-        ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps)
+        step = c_abi_step(dg, h, R, t)
+        ms = hip.timed(h.stream(), step, steps)
+        check_step(h)
         # kernel-only duration: HIP events around each pair-kernel launch
         h.set_timing(True)
         for _ in range(max(3, min(steps, 20))):
@@ -176,7 +202,8 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         if TUNE:
             h.autotune()
         dg.energy_grad_fd(h, R, t, step_h)
-        ms = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, step_h), steps)
+        ms = hip.timed(h.stream(), c_abi_step(dg, h, R, t, fd_h=step_h), steps)
+        check_step(h)
         nT, nS, _ = h.energy_info()
         return dict(ms_per_gradient=ms / steps, pairs=nT * nS)
     finally:
@@ -258,8 +285,10 @@ def bench_contours_2d(hip, dg, W, steps=200):
         R, t = d["R_G"], np.ravel(d["t_G"])
         for _ in range(5):
             dg.energy_grad(h, R, t)
-        ms = hip.timed(h.stream(), lambda: dg.energy_grad(h, R, t), steps) / steps
-        fd = hip.timed(h.stream(), lambda: dg.energy_grad_fd(h, R, t, 1e-4), steps // 10) / (steps // 10)
+        ms = hip.timed(h.stream(), c_abi_step(dg, h, R, t), steps) / steps
+        check_step(h)
+        fd = hip.timed(h.stream(), c_abi_step(dg, h, R, t, fd_h=1e-4), steps // 10) / (steps // 10)
+        check_step(h)
         nT, nS, iso = h.energy_info()
     finally:
         dg.fr_m(h)
