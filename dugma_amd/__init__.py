@@ -26,6 +26,7 @@ from ._lib import DugmaError
 
 COV_AUTO, COV_FULL, COV_ISO = 0, 1, 2
 KNN_TARGET_SELF, KNN_SOURCE_SELF, KNN_TARGET_TO_SOURCE = 0, 1, 2
+SIGMA_TARGET, SIGMA_SOURCE = 1, 2
 _COV = {"auto": COV_AUTO, "full": COV_FULL, "iso": COV_ISO}
 
 
@@ -290,6 +291,22 @@ def knn(h, which, k, R=None, t=None):
                                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                      d2.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     return idx, d2
+
+
+def update_sigma_knn(h, which=SIGMA_TARGET | SIGMA_SOURCE, k=8, want_sigma=True):
+    """Per-point isotropic covariances sigma_i^2 I from the mean distance to
+    the k nearest neighbours within each selected cloud, on the device (see
+    include/dugma.h).  Returns (sigma_target, sigma_source, unchanged); a
+    sigma array is None for a cloud not selected or when want_sigma is
+    False."""
+    fp = ctypes.POINTER(ctypes.c_float)
+    sT = np.empty(h.N, np.float32) if (want_sigma and which & SIGMA_TARGET) else None
+    sS = np.empty(h.M, np.float32) if (want_sigma and which & SIGMA_SOURCE) else None
+    un = ctypes.c_int(0)
+    _lib.check(_lib.load().dugma_update_sigma_knn(
+        h.ptr, int(which), int(k), None if sT is None else sT.ctypes.data_as(fp),
+        None if sS is None else sS.ctypes.data_as(fp), ctypes.byref(un)))
+    return sT, sS, un.value
 
 
 def version():

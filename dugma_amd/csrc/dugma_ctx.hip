@@ -145,42 +145,9 @@ __global__ void k_prep_compat(const double *__restrict__ P,
   if (k >= n) return;
   for (int a = 0; a < D; ++a) Pf[(size_t)k * D + a] = (float)P[(size_t)k * D + a];
   const double *c = C + (size_t)k * D * D;
-  float *o = Cinv + (size_t)k * D * D;
-  if (D == 2) {
-    float a1 = (float)c[0], a2 = (float)c[1], a3 = (float)c[2], a4 = (float)c[3];
-    float d = a1 * a4 - a2 * a3;
-    if (d > 0) {
-      o[0] = a4 / d;
-      o[1] = (-a2) / d;
-      o[2] = (-a3) / d;
-      o[3] = a1 / d;
-      g[k] = sqrtf(1.0f / d);
-    } else {
-      for (int q = 0; q < 4; ++q) o[q] = 100000.0f;
-      g[k] = 0.0f;
-    }
-  } else {
-    float a11 = (float)c[0], a21 = (float)c[1], a31 = (float)c[2];
-    float a12 = (float)c[3], a22 = (float)c[4], a32 = (float)c[5];
-    float a13 = (float)c[6], a23 = (float)c[7], a33 = (float)c[8];
-    float d = a11 * a22 * a33 - a11 * a23 * a32 - a12 * a21 * a33 +
-              a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31;
-    if (d > 0) {
-      o[0] = (a22 * a33 - a23 * a32) / d;
-      o[1] = (-(a21 * a33 - a23 * a31)) / d;
-      o[2] = (a21 * a32 - a22 * a31) / d;
-      o[3] = (-(a12 * a33 - a13 * a32)) / d;
-      o[4] = (a11 * a33 - a13 * a31) / d;
-      o[5] = (-(a11 * a32 - a12 * a31)) / d;
-      o[6] = (a12 * a23 - a13 * a22) / d;
-      o[7] = (-(a11 * a23 - a13 * a21)) / d;
-      o[8] = (a11 * a22 - a12 * a21) / d;
-      g[k] = sqrtf(1.0f / d);
-    } else {
-      for (int q = 0; q < 9; ++q) o[q] = 100000.0f;
-      g[k] = 0.0f;
-    }
-  }
+  float a32[D * D];
+  for (int q = 0; q < D * D; ++q) a32[q] = (float)c[q];
+  compat_inverse<D>(a32, Cinv + (size_t)k * D * D, g + k);
 }
 
 // Validity rule of the convolved energy: finite coordinates, finite
@@ -277,6 +244,16 @@ static void make_target_records(int D, const std::vector<int> &keep,
   }
 }
 
+int refresh_host_sources(dugma_ctx *c) {
+  if (!c->hSsrc_stale) return DUGMA_OK;
+  if (!c->hSsrc.empty())
+    DUGMA_HIP(hipMemcpyAsync(c->hSsrc.data(), c->dSsrc, sizeof(double) * c->hSsrc.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  c->hSsrc_stale = false;
+  return DUGMA_OK;
+}
+
 }  // namespace dugma
 
 using namespace dugma;
@@ -290,6 +267,7 @@ static void free_ctx(dugma_ctx *c) {
     if (p) (void)hipFree(p);
   c->part.release();
   c->scratch2.release();
+  c->sigbuf.release();
   c->hstage.release();
   if (c->hpinned) (void)hipHostFree(c->hpinned);
   if (c->hparts) (void)hipHostFree(c->hparts);
@@ -400,10 +378,10 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
     // The energy path works on the FP32 points and covariances (as the
     // reference's device state does), so validity is judged on those.
     std::vector<int> keepT, keepS;
-    bool iso = true;
+    bool isoT = true, isoS = true;
     double logv = 0.0;
     long nlog = 0;
-    auto consider = [&](const double *P, const double *C, int n, std::vector<int> &keep) {
+    auto consider = [&](const double *P, const double *C, int n, std::vector<int> &keep, bool &iso) {
       double p32[3], c32[9];
       for (int i = 0; i < n; ++i) {
         for (int a = 0; a < D; ++a) p32[a] = f32(P[(size_t)i * D + a]);
@@ -417,11 +395,13 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
         ++nlog;
       }
     };
-    consider(X, X_C, N, keepT);
-    consider(Y, Y_C, M, keepS);
+    consider(X, X_C, N, keepT, isoT);
+    consider(Y, Y_C, M, keepS, isoS);
     c->nT = (int)keepT.size();
     c->nS = (int)keepS.size();
-    c->iso = iso && c->nT > 0 && c->nS > 0;
+    c->isoT = isoT;
+    c->isoS = isoS;
+    c->iso = isoT && isoS && c->nT > 0 && c->nS > 0;
     // s = 2^k with s^2 * (geometric-mean variance) ~ 1
     int k = nlog ? (int)std::lround(-0.5 * logv / (double)nlog) : 0;
     k = std::max(-60, std::min(60, k));

@@ -716,6 +716,8 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
   if (rc) return rc;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(e, "dugma_energy_grad_points: download");
+  rc = refresh_host_sources(c);
+  if (rc) return rc;
 
   // Everything below is in scaled units until the final unscaling:
   // positions x s, covariances x s^2, energy / s^D.

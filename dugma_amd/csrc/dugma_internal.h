@@ -52,6 +52,50 @@ enum { kAcc3F = 10, kAcc3I = 5, kAcc2F = 6, kAcc2I = 4 };
 // Pose-gradient partial per workgroup: [E, dofs...], padded to 8 doubles.
 enum { kPosePart = 8 };
 
+// init.cu:22-190 for one point (detX_gpu / detY_gpu): the FP32 column-major
+// covariance a -> its inverse o (adjugate / det, in the reference's
+// operation order; callers are compiled without FP contraction, and HIP's
+// FP32 division and sqrtf are correctly rounded) and g = sqrtf(1/det);
+// det <= 0 or NaN: inverse 100000 in every entry, g = 0 (init.cu:48-57,86-100).
+template <int D>
+__device__ __forceinline__ void compat_inverse(const float *a, float *o, float *g) {
+  if (D == 2) {
+    const float a1 = a[0], a2 = a[1], a3 = a[2], a4 = a[3];
+    const float d = a1 * a4 - a2 * a3;
+    if (d > 0) {
+      o[0] = a4 / d;
+      o[1] = (-a2) / d;
+      o[2] = (-a3) / d;
+      o[3] = a1 / d;
+      *g = sqrtf(1.0f / d);
+    } else {
+      for (int q = 0; q < 4; ++q) o[q] = 100000.0f;
+      *g = 0.0f;
+    }
+  } else {
+    const float a11 = a[0], a21 = a[1], a31 = a[2];
+    const float a12 = a[3], a22 = a[4], a32 = a[5];
+    const float a13 = a[6], a23 = a[7], a33 = a[8];
+    const float d = a11 * a22 * a33 - a11 * a23 * a32 - a12 * a21 * a33 +
+                    a12 * a23 * a31 + a13 * a21 * a32 - a13 * a22 * a31;
+    if (d > 0) {
+      o[0] = (a22 * a33 - a23 * a32) / d;
+      o[1] = (-(a21 * a33 - a23 * a31)) / d;
+      o[2] = (a21 * a32 - a22 * a31) / d;
+      o[3] = (-(a12 * a33 - a13 * a32)) / d;
+      o[4] = (a11 * a33 - a13 * a31) / d;
+      o[5] = (-(a11 * a32 - a12 * a31)) / d;
+      o[6] = (a12 * a23 - a13 * a22) / d;
+      o[7] = (-(a11 * a23 - a13 * a21)) / d;
+      o[8] = (a11 * a22 - a12 * a21) / d;
+      *g = sqrtf(1.0f / d);
+    } else {
+      for (int q = 0; q < 9; ++q) o[q] = 100000.0f;
+      *g = 0.0f;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Error plumbing
 // ---------------------------------------------------------------------------
@@ -130,6 +174,13 @@ struct PinBuf {
 
 }  // namespace dugma
 
+struct dugma_ctx;
+namespace dugma {
+// Bring the host copy of the source data (hSsrc) up to date after a sigma
+// update rewrote it on the device.
+int refresh_host_sources(dugma_ctx *c);
+}  // namespace dugma
+
 // ---------------------------------------------------------------------------
 // Context (opaque to callers).  The first six device arrays are the state the
 // reference keeps behind its six MEX handles (init.cu:326-331).
@@ -149,7 +200,8 @@ struct dugma_ctx {
 
   // convolved-energy state: valid points only
   int nT = 0, nS = 0;        // kept targets / sources
-  bool iso = false;          // every kept covariance is s^2 I
+  bool iso = false;          // every kept covariance is s^2 I (isoT && isoS)
+  bool isoT = false, isoS = false;  // the same, per cloud
   int *dTidx = nullptr;      // nT: original target index of each kept point
   int *dSidx = nullptr;      // nS
   void *dTrecF = nullptr;    // nT full-covariance target records
@@ -171,6 +223,7 @@ struct dugma_ctx {
 
   std::vector<int> hTidx, hSidx;  // host copies of the kept-point lists
   std::vector<double> hSsrc;      // host copy of dSsrc
+  bool hSsrc_stale = false;       // dSsrc changed on the device (sigma update)
 
   // launch shapes chosen by dugma_autotune: [energy / pose][full / iso]
   struct Tuned {
@@ -187,6 +240,7 @@ struct dugma_ctx {
 
   dugma::DevBuf part;        // pair-kernel partials
   dugma::DevBuf scratch2;    // reductions / k-NN merge
+  dugma::DevBuf sigbuf;      // sigma update: gathered points, sigma, counts
   dugma::PinBuf hstage;      // per-point sums on their way to the host
   double *hpinned = nullptr; // pinned host staging, kHostStage doubles
   static constexpr int kHostStage = 4096;

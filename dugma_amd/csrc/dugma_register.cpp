@@ -476,6 +476,8 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
   // (source variance counted twice for the target's).
   double L = 0.0, w0 = 0.0;
   {
+    int rc0 = refresh_host_sources(ctx);
+    if (rc0) return rc0;
     const double *src = ctx->hSsrc.data();
     const size_t n = ctx->nS, st = (size_t)(D + DD);
     double c[3] = {0, 0, 0}, var = 0.0;

@@ -9,10 +9,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include <hipcub/hipcub.hpp>
 
 #include "dugma_internal.h"
+#include "dugma_pairs.h"
 
 namespace dugma {
 
@@ -747,7 +749,7 @@ static int download_lists(dugma_ctx *c, const float *od, const int *oi, size_t n
 // The grid k-NN: candidates Pp (transformed from the source cloud first when
 // R is given), queries Qp.
 static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const double *t, const float *Qp,
-                    int nq, const float *Pp, int np, int self, int *idx, float *d2) {
+                    int nq, const float *Pp, int np, int self, const float **lists_d, const int **lists_i) {
   const long long cap = std::min<long long>(8LL * np + 64, 1LL << 24);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t temp_bytes = 0;
@@ -828,36 +830,22 @@ static int knn_grid(dugma_ctx *c, int D, int K, int k, const double *R, const do
   }
 #undef KNN_GRID
   DUGMA_HIP(hipGetLastError());
-  return download_lists(c, od, oi, (size_t)nq * k, d2, idx);
+  *lists_d = od;
+  *lists_i = oi;
+  return DUGMA_OK;
 }
 
-extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
-                         const double *t, int *idx, float *d2) {
-  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_knn: null context");
-  if (k < 1 || k > 32) return fail(DUGMA_ERR_ARG, "dugma_knn: k must be in 1..32");
-  dugma_ctx *c = ctx;
+// k-NN lists on the device: nq x k distances and indices in the context's
+// scratch (valid until the next call that uses it).  Candidates Pp, or the
+// transformed source cloud when R is given; self: skip candidate j == i.
+static int knn_device(dugma_ctx *c, int k, const double *R, const double *t, const float *Qp, int nq,
+                      const float *Pp, int np, int self, const float **lists_d, const int **lists_i) {
   const int D = c->D;
-  const float *Qp;
-  const float *Pp;
-  int nq, np, self = 0;
-  size_t off = 0;
-  // scratch layout: [transformed candidates] | partial d | partial idx | out d | out idx
-  size_t need_m = 0;
-  if (which == DUGMA_KNN_TARGET_SELF) {
-    Qp = c->dX; nq = c->N; Pp = c->dX; np = c->N; self = 1;
-  } else if (which == DUGMA_KNN_SOURCE_SELF) {
-    Qp = c->dY; nq = c->M; Pp = c->dY; np = c->M; self = 1;
-  } else if (which == DUGMA_KNN_TARGET_TO_SOURCE) {
-    if (!R || !t) return fail(DUGMA_ERR_ARG, "dugma_knn: R and t are required");
-    Qp = c->dX; nq = c->N; Pp = nullptr; np = c->M;
-    need_m = ((size_t)c->M * D * 4 + 255) & ~(size_t)255;
-  } else {
-    return fail(DUGMA_ERR_ARG, "dugma_knn: unknown query set");
-  }
   const int K = k <= 8 ? 8 : (k <= 16 ? 16 : 32);
   if ((long long)nq * np >= kGridMinPairs)
-    return knn_grid(c, D, K, k, which == DUGMA_KNN_TARGET_TO_SOURCE ? R : nullptr, t, Qp, nq, Pp, np, self, idx,
-                    d2);
+    return knn_grid(c, D, K, k, R, t, Qp, nq, Pp, np, self, lists_d, lists_i);
+  // scratch layout: [transformed candidates] | partial d | partial idx | out d | out idx
+  const size_t need_m = R ? ((size_t)np * D * 4 + 255) & ~(size_t)255 : 0;
   const int nch = chunks_for(nq, np);
   const int chunk = (np + nch - 1) / nch;
   const int nchunks = (np + chunk - 1) / chunk;
@@ -872,7 +860,7 @@ extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
     if (rc) return rc;
     Pp = (const float *)base;
   }
-  off = need_m;
+  const size_t off = need_m;
   float *pd = (float *)(base + off);
   int *pi = (int *)(base + off + part_al);
   float *od = (float *)(base + off + 2 * part_al);
@@ -895,5 +883,237 @@ extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
   else
     hipLaunchKernelGGL(k_knn_merge<32>, mgrid, block, 0, c->stream, pd, pi, nq, nchunks, k, od, oi);
   DUGMA_HIP(hipGetLastError());
+  *lists_d = od;
+  *lists_i = oi;
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
+                         const double *t, int *idx, float *d2) {
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_knn: null context");
+  if (k < 1 || k > 32) return fail(DUGMA_ERR_ARG, "dugma_knn: k must be in 1..32");
+  dugma_ctx *c = ctx;
+  const float *Qp, *Pp = nullptr;
+  int nq, np, self = 0;
+  if (which == DUGMA_KNN_TARGET_SELF) {
+    Qp = c->dX; nq = c->N; Pp = c->dX; np = c->N; self = 1;
+    R = t = nullptr;
+  } else if (which == DUGMA_KNN_SOURCE_SELF) {
+    Qp = c->dY; nq = c->M; Pp = c->dY; np = c->M; self = 1;
+    R = t = nullptr;
+  } else if (which == DUGMA_KNN_TARGET_TO_SOURCE) {
+    if (!R || !t) return fail(DUGMA_ERR_ARG, "dugma_knn: R and t are required");
+    Qp = c->dX; nq = c->N; np = c->M;
+  } else {
+    return fail(DUGMA_ERR_ARG, "dugma_knn: unknown query set");
+  }
+  const float *od;
+  const int *oi;
+  int rc = knn_device(c, k, R, t, Qp, nq, Pp, np, self, &od, &oi);
+  if (rc) return rc;
   return download_lists(c, od, oi, (size_t)nq * k, d2, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Per-point sigma from the k nearest neighbours within a cloud (the k-NN
+// extension of update_sig.cu:181-269): the cloud is the points the energy
+// keeps, gathered from init's FP32 points; the lists are those of the k-NN
+// search above (self excluded, (distance, index) order).
+// ---------------------------------------------------------------------------
+namespace dugma {
+
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_gather_points(const float *__restrict__ Pall,
+                                                           const int *__restrict__ idx, int n,
+                                                           float *__restrict__ P) {
+  const int q = blockIdx.x * kSBlock + threadIdx.x;
+  if (q >= n) return;
+  const size_t i = (size_t)idx[q];
+#pragma unroll
+  for (int a = 0; a < D; ++a) P[(size_t)q * D + a] = Pall[i * D + a];
+}
+
+// sigma_q = (sqrtf(d2_1) + sqrtf(d2_2) + ... + sqrtf(d2_k)) / k, nearest
+// first, in FP32; var = fl32(sigma^2).  A point whose var is zero or not
+// finite keeps its covariance and is counted.  Otherwise its covariance
+// becomes var I everywhere the context holds it, exactly as dugma_init would
+// have stored that FP32 covariance: init's inverse and g (reference
+// handles, original index i), and the energy state (kept index q), scaled
+// by sc2 = scale^2 in FP64 and rounded once; isotropic records hold ln 4
+// times the variance.
+template <int D, bool TARGET>
+__global__ __launch_bounds__(kSBlock) void k_sigma_apply(
+    const float *__restrict__ lists, int n, int k, const int *__restrict__ idx, double sc2,
+    float *__restrict__ Cinv, float *__restrict__ g, void *__restrict__ recF, void *__restrict__ recI,
+    double *__restrict__ src, float *__restrict__ sig_out, int *__restrict__ unchanged) {
+  const int q = blockIdx.x * kSBlock + threadIdx.x;
+  if (q >= n) return;
+  const float *d2 = lists + (size_t)q * k;
+  float s = sqrtf(d2[0]);
+  for (int r = 1; r < k; ++r) s = s + sqrtf(d2[r]);
+  const float sigma = s / (float)k;
+  sig_out[q] = sigma;
+  const float var = sigma * sigma;
+  if (!(var > 0.0f) || !(var < INFINITY)) {
+    atomicAdd(unchanged, 1);
+    return;
+  }
+  const size_t i = (size_t)idx[q];
+  float a[D * D];
+#pragma unroll
+  for (int e = 0; e < D * D; ++e) a[e] = (e % (D + 1) == 0) ? var : 0.0f;
+  compat_inverse<D>(a, Cinv + i * D * D, g + i);
+  const float sv = (float)((double)var * sc2);
+  const float si = (float)((double)sv * kLn4);
+  if (TARGET) {
+    if (D == 3) {
+      Rec3F &r = reinterpret_cast<Rec3F *>(recF)[q];
+      r.s11 = r.s22 = r.s33 = sv;
+      r.s12 = r.s13 = r.s23 = 0.0f;
+      if (recI) {
+        Rec3I ri;
+        ri.hx = r.hx; ri.hy = r.hy; ri.hz = r.hz; ri.s = si;
+        ri.lx = ri.ly = ri.lz = ri.pad0 = 0.0f;
+        reinterpret_cast<Rec3I *>(recI)[q] = ri;
+      }
+    } else {
+      Rec2F &r = reinterpret_cast<Rec2F *>(recF)[q];
+      r.s11 = r.s22 = sv;
+      r.s12 = 0.0f;
+      if (recI) {
+        Rec2I ri;
+        ri.hx = r.hx; ri.hy = r.hy; ri.lx = ri.ly = 0.0f; ri.s = si;
+        ri.pad0 = ri.pad1 = ri.pad2 = 0.0f;
+        reinterpret_cast<Rec2I *>(recI)[q] = ri;
+      }
+    }
+  } else {
+    double *o = src + (size_t)q * (D + D * D) + D;
+#pragma unroll
+    for (int e = 0; e < D * D; ++e) o[e] = (e % (D + 1) == 0) ? (double)var * sc2 : 0.0;
+  }
+}
+
+// Isotropic target records from the full ones (every kept covariance s I):
+// s times ln 4, as dugma_init builds them.
+template <int D>
+__global__ __launch_bounds__(kSBlock) void k_iso_records(const void *__restrict__ recF, int n,
+                                                         void *__restrict__ recI) {
+  const int q = blockIdx.x * kSBlock + threadIdx.x;
+  if (q >= n) return;
+  if (D == 3) {
+    const Rec3F r = reinterpret_cast<const Rec3F *>(recF)[q];
+    Rec3I ri;
+    ri.hx = r.hx; ri.hy = r.hy; ri.hz = r.hz; ri.s = (float)((double)r.s11 * kLn4);
+    ri.lx = ri.ly = ri.lz = ri.pad0 = 0.0f;
+    reinterpret_cast<Rec3I *>(recI)[q] = ri;
+  } else {
+    const Rec2F r = reinterpret_cast<const Rec2F *>(recF)[q];
+    Rec2I ri;
+    ri.hx = r.hx; ri.hy = r.hy; ri.lx = ri.ly = 0.0f; ri.s = (float)((double)r.s11 * kLn4);
+    ri.pad0 = ri.pad1 = ri.pad2 = 0.0f;
+    reinterpret_cast<Rec2I *>(recI)[q] = ri;
+  }
+}
+
+}  // namespace dugma
+
+extern "C" int dugma_update_sigma_knn(dugma_ctx *ctx, int which, int k, float *sigma_target,
+                                      float *sigma_source, int *unchanged) {
+  if (unchanged) *unchanged = 0;
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_update_sigma_knn: null context");
+  if (which < 1 || which > 3)
+    return fail(DUGMA_ERR_ARG, "dugma_update_sigma_knn: which must be DUGMA_SIGMA_TARGET, DUGMA_SIGMA_SOURCE or both");
+  if (k < 1 || k > 32) return fail(DUGMA_ERR_ARG, "dugma_update_sigma_knn: k must be in 1..32");
+  dugma_ctx *c = ctx;
+  const int D = c->D;
+  const bool doT = (which & DUGMA_SIGMA_TARGET) != 0, doS = (which & DUGMA_SIGMA_SOURCE) != 0;
+  const int nT = doT ? c->nT : 0, nS = doS ? c->nS : 0;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  // sigbuf: target points | target sigma | source points | source sigma | 2 counts
+  const size_t oPT = 0, oST = al((size_t)nT * D * 4), oPS = oST + al((size_t)nT * 4),
+               oSS = oPS + al((size_t)nS * D * 4), oCnt = oSS + al((size_t)nS * 4);
+  int rc = c->sigbuf.ensure(oCnt + 256);
+  if (rc) return rc;
+  char *sb = (char *)c->sigbuf.ptr;
+  int *cnt = (int *)(sb + oCnt);
+  DUGMA_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(int), c->stream));
+  const bool was_iso = c->iso;
+  // isotropic target records are kept up to date while the context is isotropic
+  void *recI = was_iso ? c->dTrecI : nullptr;
+  const dim3 blk(kSBlock);
+  for (int cloud = 0; cloud < 2; ++cloud) {
+    const bool tgt = cloud == 0;
+    const int n = tgt ? nT : nS;
+    if (n == 0) continue;
+    if (!tgt && nT > 0) DUGMA_HIP(hipStreamSynchronize(c->stream));  // the target's lists are consumed
+    float *P = (float *)(sb + (tgt ? oPT : oPS));
+    float *sig = (float *)(sb + (tgt ? oST : oSS));
+    const int *idx = tgt ? c->dTidx : c->dSidx;
+    const float *Pall = tgt ? c->dX : c->dY;
+    const dim3 grid((n + kSBlock - 1) / kSBlock);
+    if (D == 3)
+      hipLaunchKernelGGL(k_gather_points<3>, grid, blk, 0, c->stream, Pall, idx, n, P);
+    else
+      hipLaunchKernelGGL(k_gather_points<2>, grid, blk, 0, c->stream, Pall, idx, n, P);
+    DUGMA_HIP(hipGetLastError());
+    const float *od;
+    const int *oi;
+    rc = knn_device(c, k, nullptr, nullptr, P, n, P, n, 1, &od, &oi);
+    if (rc) return rc;
+    const double sc2 = c->scale * c->scale;
+    float *Cinv = tgt ? c->dXCinv : c->dYCinv;
+    float *g = tgt ? c->dgX : c->dgY;
+#define SIGMA_APPLY(DD, TT)                                                                                    \
+  hipLaunchKernelGGL((k_sigma_apply<DD, TT>), grid, blk, 0, c->stream, od, n, k, idx, sc2, Cinv, g, c->dTrecF, \
+                     recI, c->dSsrc, sig, cnt + cloud)
+    if (D == 3) {
+      if (tgt) SIGMA_APPLY(3, true); else SIGMA_APPLY(3, false);
+    } else {
+      if (tgt) SIGMA_APPLY(2, true); else SIGMA_APPLY(2, false);
+    }
+#undef SIGMA_APPLY
+    DUGMA_HIP(hipGetLastError());
+  }
+  // counts (and the sigmas, when asked for) back to the host
+  const bool wantT = sigma_target && doT, wantS = sigma_source && doS;
+  const size_t hbytes = 16 + (wantT ? (size_t)nT * 4 : 0) + (wantS ? (size_t)nS * 4 : 0);
+  rc = c->hstage.ensure(hbytes);
+  if (rc) return rc;
+  char *h = (char *)c->hstage.ptr;
+  DUGMA_HIP(hipMemcpyAsync(h, cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  float *hT = (float *)(h + 16), *hS = hT + (wantT ? nT : 0);
+  if (wantT && nT) DUGMA_HIP(hipMemcpyAsync(hT, sb + oST, (size_t)nT * 4, hipMemcpyDeviceToHost, c->stream));
+  if (wantS && nS) DUGMA_HIP(hipMemcpyAsync(hS, sb + oSS, (size_t)nS * 4, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  const int badT = ((const int *)h)[0], badS = ((const int *)h)[1];
+  if (doT && nT > 0) c->isoT = c->isoT || badT == 0;
+  if (doS && nS > 0) {
+    c->isoS = c->isoS || badS == 0;
+    c->hSsrc_stale = true;
+  }
+  c->iso = c->isoT && c->isoS && c->nT > 0 && c->nS > 0;
+  if (c->iso && !was_iso) {
+    // the context has just become isotropic: its isotropic records
+    const size_t recszI = D == 3 ? sizeof(Rec3I) : sizeof(Rec2I);
+    if (!c->dTrecI) DUGMA_HIP(hipMalloc(&c->dTrecI, (size_t)c->nT * recszI));
+    if (!c->dSrecI) DUGMA_HIP(hipMalloc(&c->dSrecI, (size_t)c->nS * recszI));
+    const dim3 grid((c->nT + kSBlock - 1) / kSBlock);
+    if (D == 3)
+      hipLaunchKernelGGL(k_iso_records<3>, grid, blk, 0, c->stream, c->dTrecF, c->nT, c->dTrecI);
+    else
+      hipLaunchKernelGGL(k_iso_records<2>, grid, blk, 0, c->stream, c->dTrecF, c->nT, c->dTrecI);
+    DUGMA_HIP(hipGetLastError());
+  }
+  if (unchanged) *unchanged = badT + badS;
+  const float qnan = std::numeric_limits<float>::quiet_NaN();
+  if (wantT) {
+    std::fill(sigma_target, sigma_target + c->N, qnan);
+    for (int q = 0; q < nT; ++q) sigma_target[c->hTidx[q]] = hT[q];
+  }
+  if (wantS) {
+    std::fill(sigma_source, sigma_source + c->M, qnan);
+    for (int q = 0; q < nS; ++q) sigma_source[c->hSidx[q]] = hS[q];
+  }
+  return DUGMA_OK;
 }

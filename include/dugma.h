@@ -204,6 +204,34 @@ enum {
 int dugma_knn(dugma_ctx *ctx, int which, int k, const double *R,
               const double *t, int *idx, float *d2);
 
+/* ---- per-point sigma from the k nearest neighbours ----------------------- */
+
+enum {
+  DUGMA_SIGMA_TARGET = 1, /* bit mask of the clouds dugma_update_sigma_knn updates */
+  DUGMA_SIGMA_SOURCE = 2
+};
+
+/* Per-point isotropic covariances from the k <= 32 nearest neighbours within
+ * each selected cloud, on the device (extends the nearest-distance pass of
+ * update_sig.cu:181-269, which DUGMA.m:47,105 refreshes every iteration;
+ * no reference function computes per-point sigma).  The cloud is the points
+ * the energy keeps (finite coordinates and a positive-definite covariance at
+ * init); neighbours are ranked as dugma_knn ranks them (FP32 squared
+ * distances of init's FP32 points, (distance, index) order, self excluded).
+ *   sigma_i = (sqrtf(d2_1) + sqrtf(d2_2) + ... + sqrtf(d2_k)) / k
+ * in FP32, nearest first, and the point's covariance becomes fl32(sigma_i^2) I
+ * in every device buffer the context holds, exactly as dugma_init would
+ * have stored it: init's inverse and g (the reference handles), and the
+ * energy state.  No host round trip of point data.  A point whose
+ * fl32(sigma_i^2) is zero or not finite (coincident neighbours, fewer than
+ * k other points) keeps its covariance; *unchanged (may be NULL) counts
+ * them.  sigma_target (N floats) / sigma_source (M floats), either may be
+ * NULL, receive sigma_i of the kept points and NaN for the others.  A
+ * context whose covariances all become isotropic switches to the isotropic
+ * kernels (DUGMA_COV_AUTO). */
+int dugma_update_sigma_knn(dugma_ctx *ctx, int which, int k, float *sigma_target,
+                           float *sigma_source, int *unchanged);
+
 /* ---- batched registration (no reference counterpart) -------------------- */
 
 /* Many independent 3-D rigid registrations of uneven sizes at once (the

@@ -1,0 +1,158 @@
+"""GPU parity of the per-point sigma update (dugma_update_sigma_knn): sigma
+bit-exact against the oracle's k-NN mean, and a context updated in place
+equal to one initialised with the updated covariances."""
+import numpy as np
+import pytest
+
+import dugma_amd as dg
+from dugma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def iso_blocks(var, D):
+    n = len(var)
+    C = np.zeros((n, D, D))
+    for a in range(D):
+        C[:, a, a] = var
+    return W.blocks(C)
+
+
+def kept(P, C):
+    """Indices the energy keeps: finite coordinates, finite covariance with
+    positive leading minors (all tests below use diagonal or SPD blocks)."""
+    B = W.unblocks(C).astype(np.float32).astype(np.float64)
+    ok = np.isfinite(P.T.astype(np.float32)).all(1) & np.isfinite(B).all((1, 2))
+    ok &= np.array([np.all(np.linalg.eigvalsh(b) > 0) if f else False for b, f in zip(B, ok)])
+    return np.nonzero(ok)[0]
+
+
+def odd_cloud(rng, D, n):
+    """A cloud with planted trouble: a NaN coordinate, a NaN covariance, an
+    exact duplicate (ties), a lattice patch (many equal distances) and ten
+    coincident points (sigma 0: the update leaves them unchanged)."""
+    P = rng.normal(size=(D, n))
+    P[:, 5] = P[:, 6]                       # duplicate: distance 0 ties
+    g = np.stack(np.meshgrid(*[np.arange(4)] * D, indexing="ij"), -1).reshape(-1, D) * 0.05 + 3.0
+    P[:, 20:20 + len(g)] = g.T              # lattice: equal distances, index tie-break
+    P[:, 200:210] = -4.0                    # ten coincident points
+    P[0, 11] = np.nan                       # not kept (coordinate)
+    var = rng.uniform(0.01, 0.05, size=n)
+    C = iso_blocks(var, D)
+    C[0, D * 13] = np.nan                   # not kept (covariance)
+    return P, C
+
+
+@pytest.mark.parametrize("D", [3, 2])
+def test_sigma_bit_exact_and_context_equal_to_fresh_init(oracle, D):
+    rng = np.random.default_rng(40 + D)
+    X, XC = odd_cloud(rng, D, 700)
+    Y, YC = odd_cloud(rng, D, 520)
+    kx, ky = kept(X, XC), kept(Y, YC)
+    ok, h = dg.init(X, XC, Y, YC)
+    try:
+        sT, sS, unchanged = dg.update_sigma_knn(h, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8)
+        handles = [h.read_handle(w) for w in range(6)]
+        R = W.rot2(0.2) if D == 2 else W.rot_axis_angle([1, 2, 3], 0.2)
+        t = np.full(D, 0.03)
+        E, g = dg.energy_grad(h, R, t)
+    finally:
+        dg.fr_m(h)
+    refT = oracle.knn_sigma(X.T[kx], 8)
+    refS = oracle.knn_sigma(Y.T[ky], 8)
+    np.testing.assert_array_equal(sT[kx], refT)
+    np.testing.assert_array_equal(sS[ky], refS)
+    assert np.isnan(np.delete(sT, kx)).all() and np.isnan(np.delete(sS, ky)).all()
+    # the ten coincident points have sigma 0 and keep their covariance
+    assert unchanged == 20
+    # the same context built from scratch with sigma^2 I where it changed
+    var_x = W.unblocks(XC)[:, 0, 0].copy()
+    var_y = W.unblocks(YC)[:, 0, 0].copy()
+    vx = (refT.astype(np.float32) * refT.astype(np.float32)).astype(np.float64)
+    vy = (refS.astype(np.float32) * refS.astype(np.float32)).astype(np.float64)
+    var_x[kx[vx > 0]] = vx[vx > 0]
+    var_y[ky[vy > 0]] = vy[vy > 0]
+    XC2, YC2 = iso_blocks(var_x, D), iso_blocks(var_y, D)
+    XC2[0, D * 13] = np.nan
+    YC2[0, D * 13] = np.nan
+    ok, h2 = dg.init(X, XC2, Y, YC2)
+    try:
+        for w in range(6):
+            np.testing.assert_array_equal(handles[w], h2.read_handle(w))
+        E2, g2 = dg.energy_grad(h2, R, t)
+    finally:
+        dg.fr_m(h2)
+    assert abs(E / E2 - 1) <= 1e-12
+    np.testing.assert_allclose(g, g2, rtol=1e-10, atol=1e-12 * np.abs(g2).max())
+
+
+def test_full_covariance_context_becomes_isotropic(oracle):
+    """A full-covariance context whose covariances the update replaces
+    everywhere switches to the isotropic kernels and matches the fresh
+    isotropic context (energy, pose gradient, per-point gradients)."""
+    d = W.config2(n=900, m=800)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        assert h.energy_info()[2] == 0
+        sT, _, un = dg.update_sigma_knn(h, dg.SIGMA_TARGET, 8)
+        assert un == 0 and h.energy_info()[2] == 0  # sources still full
+        _, sS, un = dg.update_sigma_knn(h, dg.SIGMA_SOURCE, 8)
+        assert un == 0 and h.energy_info()[2] == 1
+        R, t = d["R_G"], np.ravel(d["t_G"])
+        E, g = dg.energy_grad(h, R, t)
+        pts = dg.energy_grad_points(h, R, t)
+    finally:
+        dg.fr_m(h)
+    XC2 = iso_blocks(sT.astype(np.float64) ** 2, 3)
+    YC2 = iso_blocks(sS.astype(np.float64) ** 2, 3)
+    np.testing.assert_array_equal(sT, oracle.knn_sigma(d["X"].T, 8))
+    np.testing.assert_array_equal(sS, oracle.knn_sigma(d["Y"].T, 8))
+    ok, h2 = dg.init(d["X"], XC2, d["Y"], YC2)
+    try:
+        E2, g2 = dg.energy_grad(h2, R, t)
+        pts2 = dg.energy_grad_points(h2, R, t)
+    finally:
+        dg.fr_m(h2)
+    assert abs(E / E2 - 1) <= 1e-12
+    np.testing.assert_allclose(g, g2, rtol=1e-10, atol=1e-12 * np.abs(g2).max())
+    for key in ("gX", "gY", "gXC", "gYC"):
+        np.testing.assert_allclose(pts[key], pts2[key], rtol=1e-10, atol=1e-12 * np.abs(pts2[key]).max())
+
+
+def test_sigma_full_size_config3(oracle):
+    """200,000 x 200,000 (the grid search): 512 sampled points of each cloud
+    bit-exact against the oracle's brute force, the rest finite and
+    positive, and the updated context's energy equal to a fresh one."""
+    d = W.config3()
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        sT, sS, un = dg.update_sigma_knn(h, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8)
+        R, t = d["R_G"], np.ravel(d["t_G"])
+        E, g = dg.energy_grad(h, R, t)
+    finally:
+        dg.fr_m(h)
+    assert un == 0
+    rng = np.random.default_rng(3)
+    for s, P in ((sT, d["X"]), (sS, d["Y"])):
+        assert np.all(np.isfinite(s)) and np.all(s > 0)
+        q = rng.choice(P.shape[1], 512, replace=False)
+        np.testing.assert_array_equal(s[q], oracle.knn_sigma(P.T, 8, queries=q))
+    ok, h2 = dg.init(d["X"], iso_blocks(sT.astype(np.float64) ** 2, 3), d["Y"],
+                     iso_blocks(sS.astype(np.float64) ** 2, 3))
+    try:
+        E2, g2 = dg.energy_grad(h2, R, t)
+    finally:
+        dg.fr_m(h2)
+    assert abs(E / E2 - 1) <= 1e-12
+    np.testing.assert_allclose(g, g2, rtol=1e-10, atol=1e-12 * np.abs(g2).max())
+
+
+def test_sigma_argument_errors():
+    d = W.config1(n=60, m=50)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        for which, k in ((0, 8), (4, 8), (1, 0), (1, 33)):
+            with pytest.raises(dg.DugmaError):
+                dg.update_sigma_knn(h, which, k)
+    finally:
+        dg.fr_m(h)
