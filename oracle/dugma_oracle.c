@@ -12,6 +12,8 @@
  *  dor_update_sig     brute force in numpy with the same FP32 order; closed
  *                     forms (Y = X, R = I -> 0; single shifted pair)
  *  dor_knn            numpy brute force with a stable (distance, index) sort
+ *  dor_knn_sigma      closed forms on lattices (mean of the k nearest
+ *                     distances, self excluded) and the dor_knn lists
  *  dor_energy         scipy.stats.multivariate_normal.pdf per pair, rigid and
  *                     permutation invariance, isotropic closed form,
  *                     Gaussian-product quadrature; gradients by central
@@ -19,6 +21,10 @@
  *  dor_compi          golden vectors evaluated from compi.cu's own per-pair
  *                     coefficient expressions, and F(dT) == brute force
  *                     surrogate at random dT
+ *  dor_compi_moments  equal, coefficient by coefficient, to dor_compi and to
+ *                     the golden vectors; end to end through the DUGMA.m loop,
+ *                     which with it reproduces the reference's published
+ *                     Kinect poses
  *  dor_transform_points  closed forms through dor_update_sig (Y = X, R = I
  *                     gives 0) and the brute-force nearest-distance check
  *  dor_tree_sum_f32   a hand-worked 5-element case on which the pairwise
@@ -218,6 +224,47 @@ void dor_knn(int D, int nq, const float *Q, int np, const float *P, int k,
       idx[(size_t)i * k + s] = bi[s];
       d2[(size_t)i * k + s] = bd[s];
     }
+  }
+}
+
+/* Per-point sigma from the k nearest neighbours within one cloud: the
+ * lists of dor_knn (self excluded), then the mean distance nearest first. */
+void dor_knn_sigma(int D, int nq, const int *qidx, int np, const float *P,
+                   int k, float *sigma) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < nq; ++i) {
+    const int self = qidx ? qidx[i] : i;
+    float bd[64];
+    int bi[64];
+    for (int s = 0; s < k; ++s) {
+      bd[s] = INFINITY;
+      bi[s] = -1;
+    }
+    const float *q = P + (size_t)self * D;
+    for (int j = 0; j < np; ++j) {
+      if (j == self) continue;
+      const float *p = P + (size_t)j * D;
+      float dx = q[0] - p[0], dy = q[1] - p[1];
+      float dd;
+      if (D == 3) {
+        float dz = q[2] - p[2];
+        dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      } else {
+        dd = fmaf(dy, dy, dx * dx);
+      }
+      if (!(dd < bd[k - 1])) continue;
+      int s = k - 1;
+      while (s > 0 && dd < bd[s - 1]) {
+        bd[s] = bd[s - 1];
+        bi[s] = bi[s - 1];
+        --s;
+      }
+      bd[s] = dd;
+      bi[s] = j;
+    }
+    float sum = sqrtf(bd[0]);
+    for (int s = 1; s < k; ++s) sum = sum + sqrtf(bd[s]);
+    sigma[i] = sum / (float)k;
   }
 }
 
@@ -697,6 +744,33 @@ static void dR_entry(int D, int a, int b, int *var, double *sgn) {
   }
 }
 
+/* Output vector in energy3D.m:15 / energy2D.m:7 order from the dense
+ * monomial table; returns the number of monomials outside that list whose
+ * value exceeds 1e-12 of the largest listed one (terms that cancel
+ * mathematically leave rounding residues when summed in another order). */
+static int gather_coef(int D, const double *dense, double *coef) {
+  const int nv = (D == 3) ? 12 : 4, B = nv + 1, ncode = B * B * B * B;
+  int nm = (D == 3) ? DUGMA_NMONO3 : DUGMA_NMONO2;
+  char *used = (char *)calloc((size_t)ncode, 1);
+  for (int k = 0; k < nm; ++k) {
+    const unsigned char *e = (D == 3) ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
+    unsigned char v[4];
+    int deg = 0;
+    for (int x = 0; x < nv; ++x)
+      for (int r = 0; r < e[x]; ++r) v[deg++] = (unsigned char)x;
+    int code = mono_code(nv, deg, v);
+    coef[k] = dense[code];
+    used[code] = 1;
+  }
+  double big = 0.0;
+  for (int k = 0; k < nm; ++k) big = fmax(big, fabs(coef[k]));
+  int missing = 0;
+  for (int c = 0; c < ncode; ++c)
+    if (!used[c] && fabs(dense[c]) > 1e-12 * big) ++missing;
+  free(used);
+  return missing;
+}
+
 int dor_compi(int D, int N, const float *Xf, const float *XCinv,
               const float *gX, int M, const float *Yf, const float *YCinv,
               const float *gY, const double *R, const double *t, double sig,
@@ -757,23 +831,160 @@ int dor_compi(int D, int N, const float *Xf, const float *XCinv,
         }
     }
   }
-  /* gather in energy3D.m / energy2D.m order */
-  int nm = (D == 3) ? DUGMA_NMONO3 : DUGMA_NMONO2;
-  char *used = (char *)calloc((size_t)ncode, 1);
-  for (int k = 0; k < nm; ++k) {
-    const unsigned char *e = (D == 3) ? DUGMA_MONO3[k] : DUGMA_MONO2[k];
-    unsigned char v[4];
-    int deg = 0;
-    for (int x = 0; x < nv; ++x)
-      for (int r = 0; r < e[x]; ++r) v[deg++] = (unsigned char)x;
-    int code = mono_code(nv, deg, v);
-    coef[k] = dense[code];
-    used[code] = 1;
+  int missing = gather_coef(D, dense, coef);
+  free(dense);
+  return missing;
+}
+
+/* ------------------------------------------------------------------ */
+/* compi in moment form (FP64): per source j the P-weighted target     */
+/* moments, then the polynomial of each source                          */
+/* ------------------------------------------------------------------ */
+static void sp_clear(spoly *p) { p->n = 0; }
+
+/* dst += s * src */
+static void sp_axpy(spoly *dst, double s, const spoly *src) {
+  for (int k = 0; k < src->n; ++k) sp_add_term(dst, src->deg[k], src->v[k], s * src->c[k]);
+}
+
+/* out = a * b */
+static void sp_mul(spoly *out, const spoly *a, const spoly *b) {
+  sp_clear(out);
+  for (int x = 0; x < a->n; ++x)
+    for (int y = 0; y < b->n; ++y) {
+      unsigned char v[4];
+      int deg = a->deg[x] + b->deg[y];
+      for (int q = 0; q < a->deg[x]; ++q) v[q] = a->v[x][q];
+      for (int q = 0; q < b->deg[y]; ++q) v[a->deg[x] + q] = b->v[y][q];
+      sort_small(v, deg);
+      sp_add_term(out, deg, v, a->c[x] * b->c[y]);
+    }
+}
+
+/* dense[code] += s * a */
+static void dense_add(double *dense, int nv, double s, const spoly *a) {
+  for (int x = 0; x < a->n; ++x) dense[mono_code(nv, a->deg[x], a->v[x])] += s * a->c[x];
+}
+
+/* Moments of source j (FP64): S0 = sum_i P, S1 = sum P n, S2 = sum P n n^T,
+ * SB = sum P B, SnB_b = sum P n_a B_ab, SBn_a = sum P B_ab n_b,
+ * SnBn = sum P n^T B n; plus m'_j and A'_j.  Stride kMomStride doubles. */
+enum { kMomStride = 1 + 3 + 9 + 9 + 3 + 3 + 1 + 3 + 9 };
+
+int dor_compi_moments(int D, int N, const float *Xf, const float *XCinv,
+                      const float *gX, int M, const float *Yf,
+                      const float *YCinv, const float *gY, const double *R,
+                      const double *t, double sig, double *coef) {
+  const int DD = D * D;
+  const int nv = (D == 3) ? 12 : 4;
+  const int B = nv + 1, ncode = B * B * B * B;
+  compi_scale sc = compi_scaling(D, sig);
+  double *mom = (double *)calloc((size_t)M * kMomStride + 1, sizeof(double));
+  float *bsa = (float *)malloc(sizeof(float) * ((size_t)N * DD + 1));
+  float *gxa = (float *)malloc(sizeof(float) * ((size_t)N + 1));
+  for (int i = 0; i < N; ++i)
+    scaled_state(D, XCinv + (size_t)i * DD, gX[i], sc, bsa + (size_t)i * DD, gxa + i);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int j = 0; j < M; ++j) {
+    float as[9], gys, mp[3], Ap[9];
+    scaled_state(D, YCinv + (size_t)j * DD, gY[j], sc, as, &gys);
+    double S0 = 0, S1[3] = {0}, S2[9] = {0}, SB[9] = {0}, SnB[3] = {0}, SBn[3] = {0}, SnBn = 0;
+    for (int i = 0; i < N; ++i) {
+      const float *n = Xf + (size_t)i * D;
+      const float *bs = bsa + (size_t)i * DD;
+      double P = pair_state(D, n, bs, gxa[i], Yf + (size_t)j * D, as, gys, R, t, mp, Ap);
+      if (P == 0.0) continue;
+      double nd[3];
+      for (int a = 0; a < D; ++a) nd[a] = (double)n[a];
+      S0 += P;
+      for (int a = 0; a < D; ++a) {
+        S1[a] += P * nd[a];
+        for (int b = 0; b < D; ++b) {
+          const double Bab = (double)bs[a + D * b];
+          S2[a + D * b] += P * nd[a] * nd[b];
+          SB[a + D * b] += P * Bab;
+          SnB[b] += P * nd[a] * Bab;
+          SBn[a] += P * Bab * nd[b];
+          SnBn += P * nd[a] * Bab * nd[b];
+        }
+      }
+    }
+    /* m'_j and A'_j do not depend on i: the values of the last pair (or of
+     * a pair evaluated here when every P was zero) */
+    pair_state(D, Xf, bsa, gxa[0], Yf + (size_t)j * D, as, gys, R, t, mp, Ap);
+    double *o = mom + (size_t)j * kMomStride;
+    o[0] = S0;
+    for (int a = 0; a < 3; ++a) {
+      o[1 + a] = S1[a];
+      o[22 + a] = SnB[a];
+      o[25 + a] = SBn[a];
+      o[29 + a] = a < D ? (double)mp[a] : 0.0;
+    }
+    for (int a = 0; a < 9; ++a) {
+      o[4 + a] = S2[a];
+      o[13 + a] = SB[a];
+      o[32 + a] = a < DD ? (double)Ap[a] : 0.0;
+    }
+    o[28] = SnBn;
   }
-  int missing = 0;
-  for (int c = 0; c < ncode; ++c)
-    if (!used[c] && dense[c] != 0.0) ++missing;
-  free(used);
+  free(bsa);
+  free(gxa);
+  /* F_j(dR, dt) = SnBn - SnB.v - SBn.v + v^T SB v
+   *             + sum_pq K_pq (S2_pq - S1_p v_q - S1_q v_p + S0 v_p v_q),
+   * v = dR m'_j + dt, K = dR A'_j dR^T; summed over j in order */
+  double *dense = (double *)calloc((size_t)ncode, sizeof(double));
+  for (int j = 0; j < M; ++j) {
+    const double *o = mom + (size_t)j * kMomStride;
+    const double S0 = o[0], *S1 = o + 1, *S2 = o + 4, *SB = o + 13, *SnB = o + 22, *SBn = o + 25,
+                 SnBn = o[28], *mp = o + 29, *Ap = o + 32;
+    if (S0 == 0.0) continue;
+    spoly v[3], vv[3][3], K[3][3], tmp, w;
+    unsigned char var1[4];
+    for (int a = 0; a < D; ++a) {
+      sp_clear(&v[a]);
+      var1[0] = (unsigned char)a;
+      sp_add_term(&v[a], 1, var1, 1.0); /* dt_a */
+      for (int b = 0; b < D; ++b) {
+        int var;
+        double sg;
+        dR_entry(D, a, b, &var, &sg);
+        var1[0] = (unsigned char)var;
+        sp_add_term(&v[a], 1, var1, sg * mp[b]);
+      }
+    }
+    for (int p = 0; p < D; ++p)
+      for (int q = 0; q < D; ++q) {
+        sp_mul(&vv[p][q], &v[p], &v[q]);
+        sp_clear(&K[p][q]);
+        for (int k = 0; k < D; ++k)
+          for (int l = 0; l < D; ++l) {
+            int va, vb;
+            double sa, sb2;
+            dR_entry(D, p, k, &va, &sa);
+            dR_entry(D, q, l, &vb, &sb2);
+            unsigned char v2[4] = {(unsigned char)(va < vb ? va : vb), (unsigned char)(va < vb ? vb : va), 0, 0};
+            sp_add_term(&K[p][q], 2, v2, sa * sb2 * Ap[k + D * l]);
+          }
+      }
+    unsigned char none[4] = {0, 0, 0, 0};
+    sp_clear(&tmp);
+    sp_add_term(&tmp, 0, none, SnBn);
+    for (int a = 0; a < D; ++a) sp_axpy(&tmp, -(SnB[a] + SBn[a]), &v[a]);
+    dense_add(dense, nv, 1.0, &tmp);
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) dense_add(dense, nv, SB[a + D * b], &vv[a][b]);
+    for (int p = 0; p < D; ++p)
+      for (int q = 0; q < D; ++q) {
+        sp_clear(&w);
+        sp_add_term(&w, 0, none, S2[p + D * q]);
+        sp_axpy(&w, -S1[p], &v[q]);
+        sp_axpy(&w, -S1[q], &v[p]);
+        sp_axpy(&w, S0, &vv[p][q]);
+        acc_product(dense, nv, 1.0, &K[p][q], &w);
+      }
+  }
+  free(mom);
+  int missing = gather_coef(D, dense, coef);
   free(dense);
   return missing;
 }

@@ -51,6 +51,14 @@ float dor_tree_sum_f32(int n, const float *v);
 void dor_knn(int D, int nq, const float *Q, int np, const float *P, int k,
              int self_exclude, int *idx, float *d2);
 
+/* Per-point sigma within one cloud P (np points): for each query point
+ * qidx[i] (i < nq; qidx NULL: point i), its k <= 64 nearest other points of
+ * P ranked as dor_knn ranks them (self excluded), and
+ *   sigma[i] = (sqrtf(d2_1) + ... + sqrtf(d2_k)) / k   in FP32, nearest first
+ * (+inf when P has fewer than k other points). */
+void dor_knn_sigma(int D, int nq, const int *qidx, int np, const float *P,
+                   int k, float *sigma);
+
 /* Convolved-uncertainty energy (new; no reference function computes it):
  *   E = sum_i sum_j N(y_i - (R x_j + t); Sy_i + R Sx_j R^T)
  * in FP64.  Gradient outputs may be NULL:
@@ -83,6 +91,17 @@ int dor_compi(int D, int N, const float *Xf, const float *XCinv,
               const float *gX, int M, const float *Yf, const float *YCinv,
               const float *gY, const double *R, const double *t, double sig,
               double *coef);
+
+/* The same coefficient vector from moments (FP64 throughout): for each
+ * source j the P_ij-weighted target moments sum_i P, P n, P n n^T, P B,
+ * P n^T B, P B n, P n^T B n (P_ij as dor_compi computes it), then the
+ * polynomial F_j(dR, dt) = sum_i P_ij [u^T A'_j u + d^T B_i d] expanded
+ * from them, the sources taken in order.  O(N M) pair work instead of
+ * O(N M x 637). */
+int dor_compi_moments(int D, int N, const float *Xf, const float *XCinv,
+                      const float *gX, int M, const float *Yf,
+                      const float *YCinv, const float *gY, const double *R,
+                      const double *t, double sig, double *coef);
 
 /* Brute-force value of the same surrogate at dT (3-D: t1 t2 t3 r11 .. r33
  * row-major dR; 2-D: t1 t2 r11 r21). */

@@ -5,13 +5,18 @@ bench.py's cpu_baseline leg, never by the dugma_amd package.
 
 Pinning of the functions written here (those of the C library are listed in
 dugma_oracle.c):
-  dugma_loop   parity unpinned vs fmincon: scipy's BFGS stands in for it as
-               an independent minimiser of the same surrogate, so the GPU
-               loop is compared at iterate level; end to end, the Kinect
-               fixture scenes are checked against their ground-truth poses
-  the ZYX Euler angles of its convergence test (DUGMA.m:80, rotm2eul)
-               parity unpinned: textbook ZYX, the Robotics Toolbox being
-               absent
+  dugma_loop   end to end against the reference's own outputs: on whole
+               Kinect scenes with the reference driver's settings it lands on
+               the pose published in Ours_k_result.mat (scenes 8 and 20 in
+               the CPU tests, within 0.01 degrees); one iteration against an
+               independent assembly (numpy normalisation, SLSQP under the
+               fcontr3D.m constraints, R <- dR R, t <- t + dt, DUGMA.m:122
+               denormalisation); noise-free clouds end at the truth up to
+               the sigma stopping rule, and started at the truth it returns
+               the truth through the denormalisation.  fmincon itself is
+               absent: scipy's BFGS on SO(3) stands in for it
+  rotm2eul_zyx the textbook Rz(z) Ry(y) Rx(x) construction
+  normalize    through dugma_loop (normalize.m:3-26)
 """
 import ctypes
 import os
@@ -43,6 +48,7 @@ def lib():
         L.dor_tree_sum_f32.argtypes = [c_int, _f]
         L.dor_tree_sum_f32.restype = ctypes.c_float
         L.dor_knn.argtypes = [c_int, c_int, _f, c_int, _f, c_int, c_int, _i, _f]
+        L.dor_knn_sigma.argtypes = [c_int, c_int, _i, c_int, _f, c_int, _f]
         L.dor_energy.argtypes = [c_int, c_int, _d, _d, c_int, _d, _d, _d, _d,
                                  _d, _d, _d, _d, _d, _d]
         L.dor_energy.restype = c_double
@@ -52,6 +58,9 @@ def lib():
         L.dor_compi.argtypes = [c_int, c_int, _f, _f, _f, c_int, _f, _f, _f,
                                 _d, _d, c_double, _d]
         L.dor_compi.restype = c_int
+        L.dor_compi_moments.argtypes = [c_int, c_int, _f, _f, _f, c_int, _f, _f, _f,
+                                        _d, _d, c_double, _d]
+        L.dor_compi_moments.restype = c_int
         L.dor_surrogate_eval.argtypes = [c_int, c_int, _f, _f, _f, c_int, _f,
                                          _f, _f, _d, _d, c_double, _d]
         L.dor_surrogate_eval.restype = c_double
@@ -132,6 +141,18 @@ def knn(Q, P, k, self_exclude=False):
     return idx, d2
 
 
+def knn_sigma(P, k=8, queries=None):
+    """sigma of points `queries` (indices into P; None: all) from their k
+    nearest other points of P (see dugma_oracle.h)."""
+    P = _c32(P)
+    n, D = P.shape
+    q = None if queries is None else np.ascontiguousarray(queries, np.int32)
+    nq = n if q is None else q.size
+    out = np.empty(nq, np.float32)
+    lib().dor_knn_sigma(D, nq, _p(q, _i), n, _p(P, _f), int(k), _p(out, _f))
+    return out
+
+
 def energy(Y, SY, X, SX, R, t, grads=False, per_target=False):
     """Convolved energy; Y (N,D), SY (N,D,D), X (M,D), SX (M,D,D)."""
     Y, X = _c64(Y), _c64(X)
@@ -186,6 +207,18 @@ def compi(Xf, XCinv, gX, Yf, YCinv, gY, R, t, sig):
     return coef, missing
 
 
+def compi_moments(Xf, XCinv, gX, Yf, YCinv, gY, R, t, sig):
+    """compi's coefficients from FP64 per-source moments (same arguments and
+    result as compi)."""
+    Xf, XCinv, gX, Yf, YCinv, gY = _compi_args(Xf, XCinv, gX, Yf, YCinv, gY)
+    N, D = Xf.shape
+    coef = np.zeros(637 if D == 3 else 45)
+    missing = lib().dor_compi_moments(D, N, _p(Xf, _f), _p(XCinv, _f), _p(gX, _f), Yf.shape[0],
+                                      _p(Yf, _f), _p(YCinv, _f), _p(gY, _f), _p(_colmajor(R), _d),
+                                      _p(_c64(t), _d), float(sig), _p(coef, _d))
+    return coef, missing
+
+
 def surrogate_eval(Xf, XCinv, gX, Yf, YCinv, gY, R, t, sig, dT):
     Xf, XCinv, gX, Yf, YCinv, gY = _compi_args(Xf, XCinv, gX, Yf, YCinv, gY)
     N, D = Xf.shape
@@ -216,18 +249,36 @@ def pair_weight(n, binv, gx, m, ainv, gy, R, t, sig):
                                  _p(_c64(t), _d), float(sig))
 
 
-def dugma_loop(X, Y, X_C, Y_C, R0, t0, max_iter, acc_rot=1e-4, acc_trans=1e-8, acc_sig=1e-9):
+def rotm2eul_zyx(R):
+    """ZYX Euler angles [z, y, x] (radians) of a rotation matrix, the
+    convention of MATLAB's rotm2eul that DUGMA.m:80 calls:
+    R = Rz(z) Ry(y) Rx(x)."""
+    sy = np.hypot(R[0, 0], R[1, 0])
+    return np.array([np.arctan2(R[1, 0], R[0, 0]), np.arctan2(-R[2, 0], sy), np.arctan2(R[2, 1], R[2, 2])])
+
+
+def normalize(X, Y):
+    """normalize.m:3-26: centre each cloud on its mean, divide both by the
+    larger RMS radius.  Returns (x, y, xd, yd, scale)."""
+    xd, yd = X.mean(1), Y.mean(1)
+    x, y = X - xd[:, None], Y - yd[:, None]
+    sc = max(np.sqrt((x ** 2).sum() / x.shape[1]), np.sqrt((y ** 2).sum() / y.shape[1]))
+    return x / sc, y / sc, xd, yd, sc
+
+
+def dugma_loop(X, Y, X_C, Y_C, R0, t0, max_iter, acc_rot=1e-4, acc_trans=1e-8, acc_sig=1e-9,
+               moments=False):
     """DUGMA.m:19-137 restated on the CPU: normalize.m, the oracle's
-    update_sig / compi, and scipy's BFGS over a left rotation perturbation
+    update_sig / compi (moments=True: the moment form, for clouds of
+    thousands of points), and scipy's BFGS over a left rotation perturbation
     and dt in place of fmincon (DUGMA.m:65,73).  Returns (R, t, iterations)."""
     from scipy.optimize import minimize
     D = X.shape[0]
     X = np.asarray(X, np.float64)
     Y = np.asarray(Y, np.float64)
-    xd, yd = X.mean(1), Y.mean(1)
-    x, y = X - xd[:, None], Y - yd[:, None]
-    sc = max(np.sqrt((x ** 2).sum() / x.shape[1]), np.sqrt((y ** 2).sum() / y.shape[1]))
-    Xf, Yf = (x / sc).T.astype(np.float32), (y / sc).T.astype(np.float32)
+    x, y, xd, yd, sc = normalize(X, Y)
+    Xf, Yf = x.T.astype(np.float32), y.T.astype(np.float32)
+    compi_fn = compi_moments if moments else compi
 
     def blocks_to_pts(C):
         n = C.shape[1] // D
@@ -259,7 +310,7 @@ def dugma_loop(X, Y, X_C, Y_C, R0, t0, max_iter, acc_rot=1e-4, acc_trans=1e-8, a
     it = 1
     con = True
     while it <= max_iter and (con or ((tprev - t) ** 2).sum() > acc_trans) and sig > acc_sig:
-        coef, _ = compi(Xf, XCi, gX, Yf, YCi, gY, R, t, sig)
+        coef, _ = compi_fn(Xf, XCi, gX, Yf, YCi, gY, R, t, sig)
         np_ = 6 if D == 3 else 3
         res = minimize(lambda p: poly_eval(coef, dT_of(p)), np.zeros(np_), method="BFGS",
                        options={"gtol": 1e-12 * max(1.0, abs(poly_eval(coef, dT_of(np.zeros(np_))))),
@@ -271,8 +322,7 @@ def dugma_loop(X, Y, X_C, Y_C, R0, t0, max_iter, acc_rot=1e-4, acc_trans=1e-8, a
         t = t + dt
         a0 = angle.copy()
         if D == 3:
-            sy = np.hypot(R[0, 0], R[1, 0])
-            angle = np.array([np.arctan2(R[1, 0], R[0, 0]), np.arctan2(-R[2, 0], sy), np.arctan2(R[2, 1], R[2, 2])])
+            angle = rotm2eul_zyx(R)
             con = bool(np.any(np.abs(a0 - angle) > acc_rot))
         else:
             angle = np.array([np.degrees(np.arcsin(np.clip(R[1, 0], -1, 1))), 0, 0])

@@ -318,6 +318,31 @@ def test_compi_polynomial_identity(oracle, D):
         assert abs(F1 - F2) <= 1e-12 * abs(F2)
 
 
+@pytest.mark.parametrize("D", [3, 2])
+def test_compi_moments_equal_pairwise_expansion(oracle, D):
+    """The moment form (FP64 moments per source, then one polynomial per
+    source) gives the coefficients of the pair-by-pair expansion, which the
+    golden vectors from compi.cu's own expressions pin, coefficient by
+    coefficient, on the golden problem and on 300 x 250 random clouds with
+    non-symmetric-rounded inverses."""
+    g = np.load(os.path.join(GOLD, "compi_golden.npz"))
+    p = "d%d_" % D
+    args = [g[p + k] for k in ("Xf", "XCinv", "gX", "Yf", "YCinv", "gY", "R", "t")]
+    cm, missing = oracle.compi_moments(*args, float(g[p + "sig"]))
+    assert missing == 0
+    ref = g[p + "coef"]
+    big = np.abs(ref) > 1e-9 * np.abs(ref).max()
+    np.testing.assert_allclose(cm[big], ref[big], rtol=1e-11)
+    rng = np.random.default_rng(60 + D)
+    Xf, XCi, gX, Yf, YCi, gY, R, t = _compi_problem(rng, D, N=300, M=250)
+    c1, _ = oracle.compi(Xf, XCi, gX, Yf, YCi, gY, R, t, 0.05)
+    c2, missing = oracle.compi_moments(Xf, XCi, gX, Yf, YCi, gY, R, t, 0.05)
+    assert missing == 0
+    big = np.abs(c1) > 1e-9 * np.abs(c1).max()
+    np.testing.assert_allclose(c2[big], c1[big], rtol=1e-10)
+    assert np.abs(c2 - c1).max() <= 1e-13 * np.abs(c1).max()
+
+
 def test_compi_identity_pose_value(oracle):
     """At dT = identity, F = sum P (n - m')^T (A' + B) (n - m')."""
     rng = np.random.default_rng(55)
@@ -348,3 +373,200 @@ def test_constraints(oracle):
     dT2[3] += 0.1
     assert np.abs(oracle.constraints(dT2)).max() > 1e-3
     assert abs(oracle.constraints(np.array([0, 0, 0.6, 0.8]))[0]) < 1e-15
+
+
+# ---------------------------------------------------------------- per-point sigma
+@pytest.mark.parametrize("D", [3, 2])
+def test_knn_sigma_lattice_closed_forms(oracle, D):
+    """Interior points of a cubic (square) lattice of spacing h: the 8
+    nearest other points are the 2D face neighbours at h and the rest at
+    h sqrt 2, so sigma = (2D h + (8 - 2D) h sqrt 2) / 8; with k = 2D it is h.
+    A point of a cloud with fewer than k other points gets +inf.  (Counting
+    the point itself, using squared distances or dividing by k + 1 all fail
+    these.)"""
+    h = 0.37
+    g = np.stack(np.meshgrid(*[np.arange(6)] * D, indexing="ij"), -1).reshape(-1, D) * h
+    inner = np.all((g > 0.5 * h) & (g < 4.5 * h), axis=1)
+    s8 = oracle.knn_sigma(g, 8)
+    ref = (2 * D * h + (8 - 2 * D) * h * math.sqrt(2)) / 8
+    np.testing.assert_allclose(s8[inner], ref, rtol=1e-6)
+    np.testing.assert_allclose(oracle.knn_sigma(g, 2 * D)[inner], h, rtol=1e-6)
+    q = np.nonzero(inner)[0][:5]
+    np.testing.assert_array_equal(oracle.knn_sigma(g, 8, queries=q), s8[q])
+    assert np.isinf(oracle.knn_sigma(g[:5], 8)).all()
+
+
+def test_knn_sigma_follows_knn_lists(oracle):
+    """On a random cloud with planted duplicates the sigma of every point is
+    the FP32 mean of the square roots of its dor_knn distances, nearest
+    first."""
+    rng = np.random.default_rng(71)
+    P = rng.normal(size=(400, 3)).astype(np.float32)
+    P[10:14] = P[3]
+    idx, d2 = oracle.knn(P, P, 8, True)
+    acc = np.sqrt(d2[:, 0])
+    for r in range(1, 8):
+        acc = (acc + np.sqrt(d2[:, r])).astype(np.float32)
+    np.testing.assert_array_equal(oracle.knn_sigma(P, 8), (acc / np.float32(8)).astype(np.float32))
+
+
+# ---------------------------------------------------------------- the DUGMA.m loop
+def _rz(a):
+    return np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1.0]])
+
+
+def _ry(a):
+    return np.array([[math.cos(a), 0, math.sin(a)], [0, 1.0, 0], [-math.sin(a), 0, math.cos(a)]])
+
+
+def _rx(a):
+    return np.array([[1.0, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+
+
+def test_rotm2eul_zyx_textbook(oracle):
+    """The convergence test's angles (DUGMA.m:80, rotm2eul's default ZYX):
+    R = Rz(z) Ry(y) Rx(x) gives back (z, y, x) for |y| < 90 degrees."""
+    rng = np.random.default_rng(70)
+    for _ in range(200):
+        z, x = rng.uniform(-math.pi, math.pi, 2)
+        y = rng.uniform(-1.5, 1.5)
+        ang = oracle.rotm2eul_zyx(_rz(z) @ _ry(y) @ _rx(x))
+        np.testing.assert_allclose(ang, [z, y, x], atol=1e-12)
+
+
+def _surface_points(rng, n):
+    from dugma_amd import workloads as W
+    s = W.Surface(rng, nt=40, nph=80)
+    return s.sample(rng, n) @ np.diag([1.6, 1.0, 0.6])
+
+
+def _noise_free(D, rng, deg):
+    from dugma_amd import workloads as W
+    if D == 3:
+        X = (_surface_points(rng, 400) * 2.0 + np.array([1.0, -2.0, 0.5])).T
+        R_G = W.rot_axis_angle([1.0, 2.0, -1.0], math.radians(deg))
+        t_G = np.array([0.4, -0.3, 0.25])
+    else:
+        X = (W.contour2d(rng, 300) * 2.0 + np.array([1.0, -2.0])).T
+        R_G = W.rot2(math.radians(deg))
+        t_G = np.array([0.4, -0.3])
+    Y = R_G.T @ (X - t_G[:, None])
+    C = W.blocks(np.tile(2.0 * np.eye(D), (X.shape[1], 1, 1)))
+    return X, Y, C, R_G, t_G
+
+
+@pytest.mark.parametrize("D", [3, 2])
+def test_dugma_loop_noise_free_recovers_truth(oracle, D):
+    """Identical points and covariances, Y = R_G^T (X - t_G), started 25
+    degrees off: the loop (normalise, sigma, compi, surrogate minimiser,
+    R <- dR R, t <- t + dt, denormalise) ends at (R_G, t_G) up to the point
+    where DUGMA's sigma stops it: sigma is the mean of the nearest squared
+    distances / D quantised to 1e-6 (update_sig.cu:256-259), which is 0 <
+    accuracy_sig once every point is within sqrt(D * 5e-7) of its twin
+    (DUGMA.m:57) -- about 1e-3 of the unit cloud."""
+    X, Y, C, R_G, t_G = _noise_free(D, np.random.default_rng(80 + D), 25.0)
+    R, t, it = oracle.dugma_loop(X, Y, C, C, np.eye(D), np.zeros(D), 300, 1e-10, 1e-18, 1e-12, moments=True)
+    x, y, xd, yd, s = oracle.normalize(X, Y)
+    Rn, tn = R, (t - xd + R @ yd) / s  # the loop's last pose in normalised units
+    assert oracle.update_sig(x.T, y.T, Rn, tn) == 0.0
+    assert it < 300
+    from dugma_amd import workloads as W
+    err = W.rotation_error_deg(R, R_G) if D == 3 else abs(math.degrees(math.atan2(R[1, 0], R[0, 0]) -
+                                                                     math.atan2(R_G[1, 0], R_G[0, 0])))
+    assert err < 0.05
+    assert np.linalg.norm(t - t_G) < 1e-3 * s
+
+
+@pytest.mark.parametrize("D", [3, 2])
+def test_dugma_loop_at_truth_denormalises(oracle, D):
+    """Started at the truth in normalised units, t_n = (R_G yd + t_G - xd) / s,
+    sigma is 0 at once, so no iteration runs and the loop returns R0 and
+    t = xd + t_n s - R yd (DUGMA.m:122) = t_G."""
+    X, Y, C, R_G, t_G = _noise_free(D, np.random.default_rng(85 + D), 25.0)
+    x, y, xd, yd, s = oracle.normalize(X, Y)
+    tn = (R_G @ yd + t_G - xd) / s
+    R, t, it = oracle.dugma_loop(X, Y, C, C, R_G, tn, 300, moments=True)
+    assert it == 0
+    np.testing.assert_array_equal(R, R_G)
+    np.testing.assert_allclose(t, t_G, rtol=0, atol=1e-13 * np.abs(X).max())
+
+
+def test_dugma_loop_one_iteration_composition(oracle):
+    """One iteration of DUGMA.m:57-123 put together independently: numpy
+    normalisation (normalize.m), the oracle's sigma and compi, and scipy's
+    SLSQP over the twelve fmincon variables under fcontr3D.m's equality
+    constraints (an fmincon-like constrained solver, not the loop's BFGS on
+    SO(3)); then R <- dR R, t <- t + dt (DUGMA.m:76-78) and t = xd + t s -
+    R yd (DUGMA.m:122).  The start has a large translation, so composing as
+    t <- dR t + dt would miss by far more than the tolerance."""
+    from scipy.optimize import minimize
+    from dugma_amd import workloads as W
+    rng = np.random.default_rng(90)
+    P = _surface_points(rng, 500)
+    R_G = W.rot_axis_angle([0.3, -1.0, 0.5], math.radians(20.0))
+    X = (P[:250] @ R_G.T + np.array([0.2, 0.1, -0.3]) + rng.normal(scale=0.01, size=(250, 3))).T
+    Y = P[250:].T
+    var = 0.02
+    XC = W.blocks(np.tile(var * np.eye(3), (250, 1, 1)))
+    R0 = W.rot_axis_angle([1.0, 0.0, 1.0], math.radians(12.0))
+    t0 = np.array([0.6, -0.4, 0.3])
+    R1, t1, it = oracle.dugma_loop(X, Y, XC, XC, R0, t0, 1, moments=True)
+    assert it == 1
+    # the same iteration from its parts
+    xd, yd = X.mean(1), Y.mean(1)
+    x, y = X - xd[:, None], Y - yd[:, None]
+    s = max(math.sqrt((x ** 2).sum() / 250), math.sqrt((y ** 2).sum() / 250))
+    Xf, Yf = (x / s).T.astype(np.float32), (y / s).T.astype(np.float32)
+    Ci, g, _ = oracle.prep_inverse(np.tile(var * np.eye(3), (250, 1, 1)), 3)
+    sig = oracle.update_sig(Xf, Yf, R0, t0)
+    coef, _ = oracle.compi_moments(Xf, Ci, g, Yf, Ci, g, R0, t0, sig)
+    scale = abs(oracle.poly_eval(coef, np.r_[0, 0, 0, np.eye(3).ravel()]))
+    # fcontr3D.m's ten equalities are redundant (R^T R - I is symmetric, and
+    # det R = 1 follows near the identity); SLSQP needs independent ones:
+    # the upper triangle of R^T R - I
+    upper = [1, 4, 5, 7, 8, 9]
+    res = minimize(lambda z: oracle.poly_eval(coef, z) / scale, np.r_[0, 0, 0, np.eye(3).ravel()],
+                   method="SLSQP", constraints=[{"type": "eq", "fun": lambda z: oracle.constraints(z)[upper]}],
+                   options={"ftol": 1e-16, "maxiter": 1000})
+    assert res.success
+    assert np.abs(oracle.constraints(res.x)).max() < 1e-9
+    dT = res.x
+    dR, dt = dT[3:].reshape(3, 3), dT[:3]
+    R_ref = dR @ R0
+    t_ref = xd + (t0 + dt) * s - R_ref @ yd
+    np.testing.assert_allclose(R1, R_ref, atol=2e-6)
+    np.testing.assert_allclose(t1, t_ref, atol=2e-6 * s)
+    t_wrong = xd + (dR @ t0 + dt) * s - R_ref @ yd
+    assert np.linalg.norm(t_wrong - t_ref) > 1e3 * 2e-6 * s
+
+
+def _full_scene(k):
+    g = np.load(os.path.join(GOLD, "kinect_full_scenes.npz"))
+    out = {"X": g["s%d_X" % k], "Y": g["s%d_Y" % k], "R_G": g["s%d_R_G" % k], "t_G": g["s%d_t_G" % k]}
+    for name in ("X", "Y"):
+        v = g["s%d_v%s" % (k, name)]
+        C = np.zeros((3, 3 * len(v)))
+        for a in range(3):
+            C[a, a::3] = v
+        out[name + "_C"] = C
+    return out
+
+
+@pytest.mark.parametrize("k", [8, 20])
+def test_dugma_loop_reproduces_published_kinect_outcomes(oracle, k):
+    """The whole loop on whole Kinect scenes with the reference's driver
+    settings (main_kinect_real_application.m:14-21) lands on the pose the
+    reference published for the scene (Our_result_in_Kinect_Application/
+    Ours_k_result.mat): within 0.01 degrees and 1e-4 of the translation.
+    This pins the loop end to end against the reference's own outputs:
+    normalisation, sigma, compi, the minimiser, the pose composition and the
+    denormalisation (a wrong one of these moves the result by far more)."""
+    from dugma_amd import workloads as W
+    d = _full_scene(k)
+    R, t, it = oracle.dugma_loop(d["X"], d["Y"], d["X_C"], d["Y_C"], np.eye(3), np.zeros(3), 300, 1e-4, 1e-8,
+                                 1e-9, moments=True)
+    g = np.load(os.path.join(GOLD, "kinect_fixtures.npz"))
+    Rp, tp = g["published_R"][k - 1], g["published_t"][k - 1]
+    assert W.rotation_error_deg(R, Rp) < 0.01
+    assert np.linalg.norm(t - tp) < 1e-4 * np.linalg.norm(d["t_G"])
+
