@@ -2,16 +2,18 @@
 """Benchmark of the DUGMA registration hot path on one MI355X.
 
 A step is one fused energy + pose-adjoint evaluation (dugma_energy_grad: the
-optimiser's gradient callback) over the 6,000 x 6,000 3-D full-covariance
-workload (structured-light noise, 10% outliers), inputs resident in HBM.
-The value is pair evaluations per second of that whole call.  Beside it the
-line reports:
-  plain      the same gradient by central finite differences (13 forward
-             launches per gradient), same GPU;
-  large      the fused call at 200,000 x 200,000 isotropic, with its own
-             finite-difference gradient, the k = 8 nearest-neighbour
-             search of the sigma update and a registration on the
-             convolved energy;
+optimiser's gradient callback) over 200,000 x 200,000 3-D points with
+isotropic sigma from the 8 nearest neighbours, inputs resident in HBM.  The
+value is pair evaluations per second of that whole call.  Beside it the line
+reports:
+  large      on the same clouds: the same gradient by central finite
+             differences (13 forward launches), the k = 8 nearest-neighbour
+             search, one full iteration (per-point sigma from the 8
+             nearest neighbours of both clouds on the device + the fused
+             step), and a registration on the convolved energy;
+  full_6k    the fused step on 6,000 x 6,000 full-covariance
+             structured-light clouds (10% outliers) and its finite-difference
+             gradient;
   points     energy with gradients wrt every point's position and
              covariance on two 320 x 240 depth views;
   contours_2d  the 2-D path: fused step and finite differences on
@@ -21,8 +23,8 @@ line reports:
   reference_path  update_sig, compi and a whole DUGMA.m loop;
   roofline   achieved FP32 FLOP/s of the all-pairs kernel (HIP events on the
              context's stream) against MI355X's 157.3 TFLOP/s;
-  cpu_baseline  the FP64 oracle (OpenMP, all host cores) on the same
-             workload.
+  cpu_baseline  OpenMP loops of the same computations on the host cores
+             (oracle/dugma_cpu_baseline.cpp).
 """
 import argparse
 import ctypes
@@ -52,24 +54,12 @@ FLOP_3D_FULL_FUSED = int(_RC["FLOP_3D_FULL_FUSED"])
 FLOP_3D_FULL_FWD = int(_RC["FLOP_3D_FULL_FWD"])
 FLOP_3D_ISO_FUSED = int(_RC["FLOP_3D_ISO_FUSED"])
 FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
-# HBM bytes per k_pairs launch (fused, MODE 1) on the 6,000 x 6,000
-# full-covariance workload, keyed by the launch it runs (lane_points, source
-# chunks): rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled by
-# what the counter reads for a 256 MiB copy
-# (profiles/k_pairs_hbm_traffic_6000x6000_full.txt).
-MEASURED_TRAFFIC_6K_FULL = {(1, 22): 6629000, (1, 43): 6661000, (1, 86): 6727000, (1, 167): 6852000,
-                            (1, 188): 6884000, (2, 43): 7042000, (2, 86): 7075000, (2, 167): 7137000,
-                            (2, 188): 7154000}
-
-
-def launch_chunks(n_stationary, n_streamed, lane_points, target_waves):
-    """Streamed chunks of a pair launch, as the library sizes them: 256-lane
-    workgroups of lane_points stationary points per lane, chunks of at least
-    32 streamed points, wave count near target_waves."""
-    blocks = -(-n_stationary // (256 * lane_points))
-    qc = max(1, min(-(-target_waves // (blocks * 4)), -(-n_streamed // 32)))
-    chunk = -(-n_streamed // qc)
-    return -(-n_streamed // chunk)
+# HBM bytes per launch of the headline kernel (fused, isotropic, 200,000 x
+# 200,000, the autotuned launch): rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
+# each scaled by what the counter reads for a copy kernel of known size in the
+# same process (profiles/k_pairs_traffic_200000x200000_iso.txt).
+PROFILED_TRAFFIC_200K_ISO = None
+TRAFFIC_PROFILE = "profiles/k_pairs_traffic_200000x200000_iso.txt"
 
 
 class Hip:
@@ -139,28 +129,45 @@ def start_pose(d, deg=2.0, dt=0.003, seed=0):
 TUNE = True
 
 
-def c_abi_step(dg, h, R, t, fd_h=None):
+class CStep:
     """One step as a C caller makes it: dugma_energy_grad (or, with fd_h,
     dugma_energy_grad_fd) through ctypes with the arguments bound once, so
     the timed loop holds the library call and not the Python mirror's
     argument handling.  E and the gradient land in the context's result
-    buffers on every call."""
-    from dugma_amd import _lib
-    b = h._pb
-    b.pose(R, t)
-    if fd_h is None:
-        fn, args = _lib.load().dugma_energy_grad, (h.ptr, b.Rp, b.tp, 0, b.Ep, b.gp)
-    else:
-        fn, args = _lib.load().dugma_energy_grad_fd, (h.ptr, b.Rp, b.tp, 0, ctypes.c_double(fd_h), b.Ep, b.gp)
-    return lambda: fn(*args)
+    buffers, preset to NaN; every status the calls return is kept."""
 
+    def __init__(self, dg, h, R, t, fd_h=None):
+        from dugma_amd import _lib
+        self.h = h
+        b = h._pb
+        b.pose(R, t)
+        if fd_h is None:
+            self.fn, self.args = _lib.load().dugma_energy_grad, (h.ptr, b.Rp, b.tp, 0, b.Ep, b.gp)
+        else:
+            self.fn, self.args = (_lib.load().dugma_energy_grad_fd,
+                                  (h.ptr, b.Rp, b.tp, 0, ctypes.c_double(fd_h), b.Ep, b.gp))
+        self.bad = 0
 
-def check_step(h):
-    """The timed calls ignore the status they return: check that the last
-    one left a finite energy and gradient."""
-    b = h._pb
-    if not (np.isfinite(b.E.value) and np.all(np.isfinite(b.g))):
-        raise RuntimeError("the timed steps did not produce a finite energy and gradient")
+    def __call__(self):
+        if self.fn(*self.args):
+            self.bad += 1
+
+    def arm(self):
+        b = self.h._pb
+        b.E.value = float("nan")
+        b.g[:] = np.nan
+        self.bad = 0
+        return self
+
+    def check(self):
+        """Raise unless every timed call returned 0 and the last one left a
+        finite energy and gradient."""
+        b = self.h._pb
+        if self.bad:
+            from dugma_amd import _lib
+            raise RuntimeError("%d timed calls failed: %s" % (self.bad, _lib.load().dugma_last_error().decode()))
+        if not (np.isfinite(b.E.value) and np.all(np.isfinite(b.g))):
+            raise RuntimeError("the timed steps did not produce a finite energy and gradient")
 
 
 def bench_fused(hip, dg, d, steps, warmup, label):
@@ -177,9 +184,9 @@ def bench_fused(hip, dg, d, steps, warmup, label):
                                           int(os.environ.get("DUGMA_TARGET_WAVES", 0)))
         for _ in range(warmup):
             dg.energy_grad(h, R, t)
-        step = c_abi_step(dg, h, R, t)
+        step = CStep(dg, h, R, t).arm()
         ms = hip.timed(h.stream(), step, steps)
-        check_step(h)
+        step.check()
         # kernel-only duration: HIP events around each pair-kernel launch
         h.set_timing(True)
         for _ in range(max(3, min(steps, 20))):
@@ -202,12 +209,36 @@ def bench_plain(hip, dg, d, steps, step_h=1e-4):
         if TUNE:
             h.autotune()
         dg.energy_grad_fd(h, R, t, step_h)
-        ms = hip.timed(h.stream(), c_abi_step(dg, h, R, t, fd_h=step_h), steps)
-        check_step(h)
+        step = CStep(dg, h, R, t, fd_h=step_h).arm()
+        ms = hip.timed(h.stream(), step, steps)
+        step.check()
         nT, nS, _ = h.energy_info()
         return dict(ms_per_gradient=ms / steps, pairs=nT * nS)
     finally:
         dg.fr_m(h)
+
+
+def bench_full_6k(hip, dg, d2, steps, warmup, plain=True):
+    """The fused step on the 6,000 x 6,000 full-covariance structured-light
+    clouds, and (plain) its 13-launch central-difference gradient."""
+    fused = bench_fused(hip, dg, d2, steps, warmup, "fused 6000x6000 full")
+    value = fused["pairs"] / (fused["ms_per_step"] / 1e3) / 1e9
+    ach = fused["pairs"] * FLOP_3D_FULL_FUSED / (fused["kernel_ms"] / 1e3) / 1e12
+    res = {"workload": "3-D 6,000 x 6,000 full-covariance structured-light clouds, 10% outliers; fused energy + "
+                       "6-DoF pose adjoint per step",
+           "value": round(value, 3), "unit": "Gpair-evals/s", "ms_per_step": round(fused["ms_per_step"], 4),
+           "kernel_ms": round(fused["kernel_ms"], 4), "achieved_tflops": round(ach, 3),
+           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_FULL_FUSED, "launch": fused["plan"],
+           "init_ms": round(fused["init_ms"], 3)}
+    if plain:
+        pl = bench_plain(hip, dg, d2, max(3, steps // 10))
+        pv = pl["pairs"] / (pl["ms_per_gradient"] / 1e3) / 1e9
+        res["plain_fd"] = {"value": round(pv, 3), "unit": "Gpair-evals/s",
+                           "ms_per_gradient": round(pl["ms_per_gradient"], 4),
+                           "scheme": "6-DoF central differences, h=1e-4, 13 forward launches",
+                           "fused_speedup": round(value / pv, 2)}
+        log("plain FD 6K: %.3f ms/gradient, fused speed-up %.2fx" % (pl["ms_per_gradient"], value / pv))
+    return res
 
 
 def bench_register_energy(dg, d):
@@ -285,10 +316,12 @@ def bench_contours_2d(hip, dg, W, steps=200):
         R, t = d["R_G"], np.ravel(d["t_G"])
         for _ in range(5):
             dg.energy_grad(h, R, t)
-        ms = hip.timed(h.stream(), c_abi_step(dg, h, R, t), steps) / steps
-        check_step(h)
-        fd = hip.timed(h.stream(), c_abi_step(dg, h, R, t, fd_h=1e-4), steps // 10) / (steps // 10)
-        check_step(h)
+        step = CStep(dg, h, R, t).arm()
+        ms = hip.timed(h.stream(), step, steps) / steps
+        step.check()
+        step = CStep(dg, h, R, t, fd_h=1e-4).arm()
+        fd = hip.timed(h.stream(), step, steps // 10) / (steps // 10)
+        step.check()
         nT, nS, iso = h.energy_info()
     finally:
         dg.fr_m(h)
@@ -443,24 +476,112 @@ def bench_reference_path(dg, W, d2):
                                                           "GTX 1080 Ti + MATLAB fmincon, printing included"}}
 
 
-def cpu_baseline(d, budget_s=15.0):
+def bench_iteration(hip, dg, d, steps=5):
+    """One iteration of the large configuration as the device runs it: the
+    per-point sigma of both clouds from their 8 nearest neighbours
+    (dugma_update_sigma_knn: two within-cloud k-NN searches, mean
+    distances, covariances rewritten in place), then the fused step at the
+    new covariances.  Both calls through ctypes with bound arguments."""
+    from dugma_amd import _lib
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = start_pose(d)
+        if TUNE:
+            h.autotune()
+        un = ctypes.c_int(0)
+        sfn = _lib.load().dugma_update_sigma_knn
+        sargs = (h.ptr, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8, None, None, ctypes.byref(un))
+        step = CStep(dg, h, R, t)
+        bad = [0]
+
+        def sigma():
+            if sfn(*sargs):
+                bad[0] += 1
+
+        def iteration():
+            sigma()
+            step()
+
+        for _ in range(2):
+            iteration()
+        step.arm()
+        sig_ms = hip.timed(h.stream(), sigma, steps) / steps
+        it_ms = hip.timed(h.stream(), iteration, steps) / steps
+        step.check()
+        if bad[0]:
+            raise RuntimeError("%d sigma updates failed: %s" % (bad[0], _lib.load().dugma_last_error().decode()))
+        nT, nS, iso = h.energy_info()
+    finally:
+        dg.fr_m(h)
+    log("iteration 200Kx200K: sigma update %.3f ms, sigma + fused step %.3f ms" % (sig_ms, it_ms))
+    return {"what": "per-point sigma of both clouds from their 8 nearest neighbours (device k-NN, mean distance, "
+                    "covariances rewritten in place), then the fused energy + pose adjoint",
+            "ms_per_iteration": round(it_ms, 3), "sigma_update_ms": round(sig_ms, 3), "isotropic": bool(iso),
+            "unchanged_points": int(un.value), "value": round(nT * nS / (it_ms / 1e3) / 1e9, 3),
+            "unit": "Gpair-evals/s (fused pairs per iteration time)"}
+
+
+def cpu_baseline(d3, d2, n200=1024, n_compi=300, reps=5):
+    """OpenMP loops of the same computations on the host cores
+    (oracle/dugma_cpu_baseline.cpp; steady_clock, median of `reps` runs after
+    one untimed run).  The headline leg is the fused FP32 energy + pose
+    gradient on n200 of the 200,000 targets against all 200,000 sources."""
     from dugma_amd import workloads as W
+    from oracle import cpu_baseline as CB
     from oracle import oracle as O
-    Y, SY, X, SX = d["X"].T, W.unblocks(d["X_C"]), d["Y"].T, W.unblocks(d["Y_C"])
-    R, t = start_pose(d)
-    O.energy(Y[:200], SY[:200], X, SX, R, t, grads=True)  # warm-up
-    times = []
-    t_start = time.time()
-    while (time.time() - t_start) < budget_s:
-        t0 = time.perf_counter()
-        O.energy(Y, SY, X, SX, R, t, grads=True)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    pairs = Y.shape[0] * X.shape[0]
-    return dict(value=pairs / med / 1e9, unit="Gpair-evals/s", cores=O.num_threads(), kind="port",
-                sample="full %dx%d workload, energy + pose gradient in FP64 (oracle, OpenMP), "
-                       "median of %d runs of %.2f s (%.1f s of CPU work)"
-                       % (Y.shape[0], X.shape[0], len(times), med, sum(times)))
+
+    def leg(pairs, seconds, **extra):
+        return {"value": round(pairs / seconds / 1e9, 4), "unit": "Gpair-evals/s", "seconds": round(seconds, 5),
+                **extra}
+
+    t_all = time.perf_counter()
+    Y, X = d3["X"].T, d3["Y"].T
+    vy, vx = W.unblocks(d3["X_C"])[:, 0, 0], W.unblocks(d3["Y_C"])[:, 0, 0]
+    R, t = start_pose(d3)
+    n200 = min(n200, Y.shape[0])
+    pairs = n200 * X.shape[0]
+    scale = Y.shape[0] / n200
+    large = {}
+    for name, mode, prec in (("forward_fp32", "forward", 32), ("fused_fp32", "fused", 32),
+                             ("fused_fp64", "fused", 64)):
+        _, _, ts = CB.energy(Y[:n200], vy[:n200], X, vx, R, t, mode=mode, prec=prec, reps=reps)
+        large[name] = leg(pairs, ts, full_size_seconds=round(ts * scale, 3))
+    large["fd_fp32_full_size_seconds"] = round(13 * large["forward_fp32"]["full_size_seconds"], 2)
+    large["sample"] = ("%d of the %d targets x all %d sources (%.3g pairs); full-size seconds scale the sample "
+                       "by %d/%d, finite differences extrapolated as 13 forward passes"
+                       % (n200, Y.shape[0], X.shape[0], pairs, Y.shape[0], n200))
+    # 6,000 x 6,000 full covariance
+    Y6, X6 = d2["X"].T, d2["Y"].T
+    SY6, SX6 = W.unblocks(d2["X_C"]), W.unblocks(d2["Y_C"])
+    R6, t6 = start_pose(d2)
+    p6 = Y6.shape[0] * X6.shape[0]
+    full = {}
+    for prec in (32, 64):
+        for mode in ("forward", "fused", "fd"):
+            _, _, ts = CB.energy(Y6, SY6, X6, SX6, R6, t6, mode=mode, prec=prec, reps=reps)
+            full["%s_fp%d" % (mode, prec)] = leg(p6, ts)
+    # the reference's own loops on the 6K clouds
+    Xf, Yf = d2["X"].T.astype(np.float32), d2["Y"].T.astype(np.float32)
+    sig, ts = CB.update_sig(Xf, Yf, R6, t6, reps=reps)
+    ref = {"update_sig": leg(p6, ts, what="quantised nearest distance, FP32 (update_sig.cu:238-260)")}
+    XCi, gX, _ = O.prep_inverse(SY6, 3)
+    YCi, gY, _ = O.prep_inverse(SX6, 3)
+    nc = min(n_compi, Xf.shape[0])
+    _, ts = CB.compi(Xf[:nc], XCi[:nc], gX[:nc], Yf, YCi, gY, R6, t6, sig, form=0, reps=max(1, reps // 2))
+    ref["compi_per_pair"] = leg(nc * Yf.shape[0], ts, full_size_seconds=round(ts * Xf.shape[0] / nc, 3),
+                                what="P (FP64 exp) and the 637-coefficient expansion of every pair "
+                                     "(compi.cu:747-3350), on %d of the 6,000 targets" % nc)
+    _, ts = CB.compi(Xf, XCi, gX, Yf, YCi, gY, R6, t6, sig, form=1, reps=reps)
+    ref["compi_moments"] = leg(p6, ts, what="P and 20 target features per pair, one expansion per source")
+    head = large["fused_fp32"]
+    return {"value": head["value"], "unit": "Gpair-evals/s", "cores": CB.num_threads(), "kind": "port",
+            "cpu_model": CB.cpu_model(), "nproc": os.cpu_count(),
+            "sample": "fused FP32 energy + pose gradient (FP64 per-target sums) on %d of the 200,000 targets x all "
+                      "200,000 sources of the headline workload, median of %d runs after one untimed run"
+                      % (n200, reps),
+            "build": "g++ -O3 -march=x86-64-v3 -ffast-math -fopenmp (vectorised expf/exp)",
+            "legs": {"large_200k_iso": large, "full_6k": full, "reference_loops_6k": ref},
+            "wall_s": round(time.perf_counter() - t_all, 2)}
 
 
 def main():
@@ -469,14 +590,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-large", action="store_true", help="skip the 200K x 200K measurement")
-    ap.add_argument("--no-plain", action="store_true", help="skip the finite-difference measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 200K extras (FD, k-NN, iteration, registration)")
+    ap.add_argument("--no-6k", action="store_true", help="skip the 6,000 x 6,000 full-covariance measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched registrations")
-    ap.add_argument("--no-points", action="store_true", help="skip the per-point gradients of the depth views")
+    ap.add_argument("--no-points", action="store_true", help="skip the per-point gradients and the 2-D contours")
     ap.add_argument("--no-loop", action="store_true", help="skip update_sig / compi / DUGMA-loop timings")
     ap.add_argument("--trials", type=int, default=4096, help="batched trials")
-    ap.add_argument("--only", choices=["fused", "plain", "large", "points", "batch", "loop"], default=None,
-                    help="run one measurement (for profiler runs)")
+    ap.add_argument("--only", choices=["fused", "fused6k", "plain", "iteration", "large", "points", "batch", "loop",
+                                       "cpu"], default=None, help="run one measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the launch shape from DUGMA_LANE_POINTS / DUGMA_TARGET_WAVES")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -495,35 +616,31 @@ def main():
     TUNE = not args.no_tune
     if args.copy_mb:
         hip.copy_calibration(args.copy_mb)
-    if args.only:
-        only = args.only
-        args.no_plain = only != "plain"
-        args.no_large = only != "large"
-        args.no_batch = only != "batch"
-        args.no_loop = only != "loop"
-        args.no_points = only != "points"
-        args.no_cpu = True
-        if only in ("large", "points", "batch", "loop"):
-            if only == "points":
-                print(json.dumps({"only": only, **bench_points(hip, dg, W),
-                                  "contours_2d": bench_contours_2d(hip, dg, W)}))
-            elif only == "loop":
-                print(json.dumps({"only": only, **bench_reference_path(dg, W, W.config2(n=6000, m=6000))}))
-            elif only == "large":
-                d3 = W.config3()
-                big = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
-                print(json.dumps({"only": only, "kernel_ms": big["kernel_ms"], "ms_per_step": big["ms_per_step"],
-                                  "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
-                                  "knn": bench_knn(dg, d3), "register_energy": bench_register_energy(dg, d3)}))
-            else:
-                print(json.dumps({"only": only, **bench_batched(dg, W, args.trials, min(128, args.trials))}))
-            return
+    only = args.only
+    if only == "cpu":
+        print(json.dumps({"only": only, **cpu_baseline(W.config3(), W.config2(n=6000, m=6000))}), flush=True)
+        return
+    if only in ("points", "batch", "loop", "fused6k", "plain"):
+        if only == "points":
+            res = {**bench_points(hip, dg, W), "contours_2d": bench_contours_2d(hip, dg, W)}
+        elif only == "loop":
+            res = bench_reference_path(dg, W, W.config2(n=6000, m=6000))
+        elif only == "batch":
+            res = bench_batched(dg, W, args.trials, min(128, args.trials))
+        else:
+            d2 = W.config2(n=6000, m=6000)
+            res = bench_full_6k(hip, dg, d2, args.steps, args.warmup, plain=(only == "plain"))
+        print(json.dumps({"only": only, **res}), flush=True)
+        return
 
-    log("generating 6000 x 6000 workload")
-    d2 = W.config2(n=6000, m=6000)
-    fused = bench_fused(hip, dg, d2, args.steps, args.warmup, "fused 6000x6000 full")
+    log("generating 200,000 x 200,000 workload")
+    d3 = W.config3()
+    if only == "iteration":
+        print(json.dumps({"only": only, **bench_iteration(hip, dg, d3)}), flush=True)
+        return
+    fused = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
     value = fused["pairs"] / (fused["ms_per_step"] / 1e3) / 1e9
-    achieved = fused["pairs"] * FLOP_3D_FULL_FUSED / (fused["kernel_ms"] / 1e3) / 1e12
+    achieved = fused["pairs"] * FLOP_3D_ISO_FUSED / (fused["kernel_ms"] / 1e3) / 1e12
     out = {
         "metric": "Gpair-evals/s energy+adjoint (fused vs plain FD), batched registrations/s",
         "value": round(value, 3),
@@ -537,51 +654,43 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "3-D 6,000 x 6,000 full-covariance structured-light clouds, 10% outliers; "
-                               "fused energy + 6-DoF pose adjoint per step (per-pair FP32, FP64 sums)",
-                   "target_points": 6000, "source_points": 6000, "parallelism": "single GPU"},
+        "config": {"workload": "3-D 200,000 x 200,000 points (spherical-harmonic surface), isotropic sigma from the "
+                               "8 nearest neighbours; fused energy + 6-DoF pose adjoint per step (per-pair FP32, "
+                               "FP64 sums)",
+                   "target_points": int(d3["X"].shape[1]), "source_points": int(d3["Y"].shape[1]),
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_FULL_FUSED,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": PROFILED_TRAFFIC_200K_ISO, "traffic_source": TRAFFIC_PROFILE,
+                     "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
                      "launch": fused["plan"]},
         "init_ms": round(fused["init_ms"], 3),
     }
-    lp = fused["plan"]
-    out["roofline"]["traffic"] = MEASURED_TRAFFIC_6K_FULL.get(
-        (lp["lane_points"], launch_chunks(6000, 6000, lp["lane_points"], lp["target_waves"])))
-    if args.only == "fused":
+    if only == "fused":
         print(json.dumps(out), flush=True)
         return
-    if not args.no_plain:
-        plain = bench_plain(hip, dg, d2, max(3, args.steps // 10))
-        pv = plain["pairs"] / (plain["ms_per_gradient"] / 1e3) / 1e9
-        out["plain_fd"] = {"value": round(pv, 3), "unit": "Gpair-evals/s",
-                           "ms_per_gradient": round(plain["ms_per_gradient"], 4),
-                           "scheme": "6-DoF central differences, h=1e-4, 13 forward launches",
-                           "fused_speedup": round(value / pv, 2)}
-        log("plain FD: %.3f ms/gradient, fused speed-up %.2fx" % (plain["ms_per_gradient"], value / pv))
     if not args.no_large:
-        log("generating 200000 x 200000 workload")
-        d3 = W.config3()
-        big = bench_fused(hip, dg, d3, 3, 1, "fused 200Kx200K iso")
-        bv = big["pairs"] / (big["ms_per_step"] / 1e3) / 1e9
-        ba = big["pairs"] * FLOP_3D_ISO_FUSED / (big["kernel_ms"] / 1e3) / 1e12
-        out["large"] = {"workload": "3-D 200,000 x 200,000 isotropic (8-NN sigma), fused energy + pose adjoint",
-                        "value": round(bv, 3), "unit": "Gpair-evals/s", "ms_per_step": round(big["ms_per_step"], 3),
-                        "kernel_ms": round(big["kernel_ms"], 3), "achieved_tflops": round(ba, 3),
-                        "frac": round(ba / PEAK_FP32_TFLOPS, 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
-                        "launch": big["plan"], "plain_fd": bench_plain_large(hip, dg, d3, big),
-                        "knn": bench_knn(dg, d3), "register_energy": bench_register_energy(dg, d3)}
+        out["large"] = {"plain_fd": bench_plain_large(hip, dg, d3, fused), "knn": bench_knn(dg, d3),
+                        "iteration": bench_iteration(hip, dg, d3),
+                        "register_energy": bench_register_energy(dg, d3)}
+        if only == "large":
+            print(json.dumps({"only": only, **out["large"]}), flush=True)
+            return
+    d2 = None
+    if not args.no_6k:
+        log("generating 6000 x 6000 workload")
+        d2 = W.config2(n=6000, m=6000)
+        out["full_6k"] = bench_full_6k(hip, dg, d2, args.steps, args.warmup)
     if not args.no_points:
         out["points"] = bench_points(hip, dg, W)
         out["contours_2d"] = bench_contours_2d(hip, dg, W)
     if not args.no_batch:
         out["batched"] = bench_batched(dg, W, args.trials, min(128, args.trials))
     if not args.no_loop:
-        out["reference_path"] = bench_reference_path(dg, W, d2)
+        out["reference_path"] = bench_reference_path(dg, W, d2 if d2 is not None else W.config2(n=6000, m=6000))
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(d2)
+        out["cpu_baseline"] = cpu_baseline(d3, d2 if d2 is not None else W.config2(n=6000, m=6000))
     print(json.dumps(out), flush=True)
 
 
