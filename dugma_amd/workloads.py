@@ -236,30 +236,17 @@ def config3(trial=0, n=200000, m=200000):
                 R_G=R, t_G=t)
 
 
-_C4_SURFACES = None
-
-
-def _c4_surfaces(nsurf=32):
-    global _C4_SURFACES
-    if _C4_SURFACES is None or len(_C4_SURFACES) != nsurf:
-        rng = np.random.default_rng(seed(4, 10 ** 6))
-        _C4_SURFACES = [Surface(rng, nt=60, nph=120) for _ in range(nsurf)]
-    return _C4_SURFACES
-
-
 def _c4_chunk(ks):
-    surfs = _c4_surfaces()
-    return [config4_trial(k, surfs, workers=1) for k in ks]
+    return [config4_trial(k, workers=1) for k in ks]
 
 
-def config4(ntrials=4096, nsurf=32, procs=None):
-    """All batched trials (shared pool of surfaces); large sets are generated
-    in worker processes (at most 16)."""
+def config4(ntrials=4096, procs=None):
+    """All batched trials, one seed (and one surface) per trial; large sets
+    are generated in worker processes (at most 16)."""
     import os
     procs = procs if procs is not None else min(16, os.cpu_count() or 1)
     if ntrials < 256 or procs <= 1:
-        surfs = _c4_surfaces(nsurf)
-        return [config4_trial(k, surfs) for k in range(ntrials)]
+        return [config4_trial(k) for k in range(ntrials)]
     import multiprocessing as mp
     chunks = [list(range(a, min(a + 64, ntrials))) for a in range(0, ntrials, 64)]
     with mp.get_context("fork").Pool(procs) as pool:
@@ -269,13 +256,14 @@ def config4(ntrials=4096, nsurf=32, procs=None):
     return out
 
 
-def config4_trial(trial, surfaces=None, workers=-1):
-    """One batched trial: N, M log-uniform in [500, 8000], angle U(0, 90) deg,
-    noise sigma U(0.001, 0.02), outlier fraction U(0, 0.3)."""
+def config4_trial(trial, workers=-1):
+    """One batched trial from its own seed: its own surface, N, M
+    log-uniform in [500, 8000], angle U(0, 90) deg, noise sigma
+    U(0.001, 0.02), outlier fraction U(0, 0.3)."""
     rng = np.random.default_rng(seed(4, trial))
     n = int(round(math.exp(rng.uniform(math.log(500), math.log(8000)))))
     m = int(round(math.exp(rng.uniform(math.log(500), math.log(8000)))))
-    surf = surfaces[trial % len(surfaces)] if surfaces else Surface(rng, nt=60, nph=120)
+    surf = Surface(rng, nt=60, nph=120)
     src = surf.sample(rng, m)
     tgt0 = surf.sample(rng, n)
     R = random_rotation(rng, 90.0)
@@ -292,7 +280,7 @@ def config4_trial(trial, surfaces=None, workers=-1):
     # isotropic variances per point (Batch takes them directly; iso_blocks
     # turns them into the D x (D*n) layout of init)
     return dict(X=tgt.T.copy(), sX=sx ** 2, Y=src.T.copy(), sY=sy ** 2,
-                R_G=R, t_G=t, noise=noise, outliers=outl)
+                R_G=R, t_G=t, noise=noise, outliers=outl, n=n, m=m)
 
 
 def kinect_like(trial=0, n=4600, m=4000, deg=30.0, var=2.25):

@@ -35,22 +35,31 @@ def small_trial(seed, n, m, deg=20.0, noise=0.003, outl=0.0, stretch=(1.0, 1.0, 
                 R_G=R, t_G=t)
 
 
-def numpy_step(tr, R0, t0, sigma=False):
+def numpy_step(tr, R0, t0, sigma=False, block=1024):
+    """One majorise-minimise step in FP64, over blocks of targets: weights
+    w_ij = s_ij^-5/2 exp(-|d_ij|^2 / 2 s_ij), weighted Procrustes by SVD."""
     f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
     y, x = f32(tr["X"].T), f32(tr["Y"].T)
     sy, sx = f32(tr["sX"]), f32(tr["sY"])
     xp = x @ R0.T + t0
-    d2 = ((y[:, None, :] - xp[None, :, :]) ** 2).sum(-1)
     g = 1.0
     if sigma:  # DUGMA's sig at the start pose, scaled by the mean variances
-        g = max(d2.min(1).mean() / 3.0 / (sy.mean() + sx.mean()), 1e-6)
-    s = g * (sy[:, None] + sx[None, :])
-    e = s ** -1.5 * np.exp(-0.5 * d2 / s)
-    w = e / s
-    Wt = w.sum()
-    my = (w.sum(1) @ y) / Wt
-    mx = (w.sum(0) @ xp) / Wt
-    H = (xp - mx).T @ w.T @ (y - my)  # sum_ij w_ij (x'_j - mx)(y_i - my)^T
+        mins = np.concatenate([((y[a:a + block, None, :] - xp[None]) ** 2).sum(-1).min(1)
+                               for a in range(0, len(y), block)])
+        g = max(mins.mean() / 3.0 / (sy.mean() + sx.mean()), 1e-6)
+    Wt, wy, wx, Sxy = 0.0, np.zeros(3), np.zeros(3), np.zeros((3, 3))
+    for a in range(0, len(y), block):
+        yb = y[a:a + block]
+        d2 = ((yb[:, None, :] - xp[None]) ** 2).sum(-1)
+        s = g * (sy[a:a + block, None] + sx[None, :])
+        w = s ** -2.5 * np.exp(-0.5 * d2 / s)
+        r, c = w.sum(1), w.sum(0)
+        Wt += r.sum()
+        wy += r @ yb
+        wx += c @ xp
+        Sxy += xp.T @ (w.T @ yb)  # sum_ij w_ij x'_j y_i^T
+    my, mx = wy / Wt, wx / Wt
+    H = Sxy - Wt * np.outer(mx, my)  # sum_ij w_ij (x'_j - mx)(y_i - my)^T
     U, S, Vt = np.linalg.svd(H)
     Dg = np.diag([1.0, 1.0, np.sign(np.linalg.det(Vt.T @ U.T))])
     dR = Vt.T @ Dg @ U.T
@@ -205,3 +214,88 @@ def test_config1_energy_maximum(oracle):
     assert np.linalg.norm(g[3:]) < 1e-2 * np.linalg.norm(g_tr[3:]), (g, g_tr)
     print("config 1: rotation error %.4f deg, translation error %.5f, %d iterations"
           % (W.rotation_error_deg(R, d["R_G"]), np.linalg.norm(t - d["t_G"]), out["iters"][0]))
+
+
+# A slice of the batched workload itself (4,096 trials, one seed and one
+# surface per trial): the smallest and largest clouds, the heaviest outlier
+# fractions, and every 40th trial.
+C4_SLICE = (0, 5, 10, 40, 68, 80, 81, 84, 110, 111, 120, 124, 127, 160, 163, 168, 181, 196, 200, 207, 210,
+            215, 233, 240, 246, 258, 280, 283, 285, 292, 297, 320, 324, 353, 358, 360, 370, 400, 401, 409,
+            440, 480, 504, 507, 509)
+# first-step check: every slice trial below 4e6 pairs, and the largest in
+# both clouds (5: 7,727 x 7,308), the most uneven (163: 7,864 x 1,534;
+# 297: 7,912 x 604)
+C4_STEP = (0, 5, 68, 80, 81, 110, 111, 120, 124, 163, 181, 207, 210, 215, 233, 240, 258, 280, 297, 353, 358,
+           370, 401, 480, 504, 509)
+# stationarity: the five smallest (500-930 points per cloud)
+C4_SMALL = (210, 215, 258, 280, 111)
+
+
+@pytest.fixture(scope="module")
+def c4_trials():
+    return {k: W.config4_trial(k) for k in C4_SLICE}
+
+
+@pytest.mark.parametrize("sigma", [False, True])
+def test_config4_first_step_matches_procrustes(c4_trials, sigma):
+    """The first step of every trial from identity (the workload's 0-90
+    degree starts, up to 30% outliers), with and without the sigma update,
+    against the FP64 blockwise Procrustes step."""
+    trials = [c4_trials[k] for k in C4_STEP]
+    print("config 4 first step: %d trials, %.3g pairs, sigma=%s"
+          % (len(trials), sum(tr["n"] * tr["m"] for tr in trials), sigma), flush=True)
+    b = dg.Batch(trials)
+    try:
+        out = b.register(max_iter=1, anneal=False, use_graph=False, sigma_update=sigma)
+    finally:
+        b.free()
+    for k, tr in enumerate(trials):
+        Rr, tt = numpy_step(tr, np.eye(3), np.zeros(3), sigma)
+        assert W.rotation_error_deg(out["R"][k], Rr) < 1e-4, (C4_STEP[k], tr["n"], tr["m"])
+        assert np.abs(out["t"][k] - tt).max() < 1e-6, (C4_STEP[k], tr["n"], tr["m"])
+
+
+def test_config4_graph_equals_one_trial_per_launch(c4_trials):
+    """All 45 trials of the slice, bench settings (annealing, sigma update,
+    over-relaxation, coarse phase): graph replay and one trial per launch
+    give bit-identical poses, iteration counts and status."""
+    trials = [c4_trials[k] for k in C4_SLICE]
+    b = dg.Batch(trials)
+    try:
+        kw = dict(max_iter=30, anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9, coarse_iters=12,
+                  tol_rot=1e-4, tol_trans=1e-4)
+        g = b.register(use_graph=True, check_every=10, **kw)
+        o = b.register(one_trial_per_launch=True, **kw)
+    finally:
+        b.free()
+    np.testing.assert_array_equal(g["R"], o["R"])
+    np.testing.assert_array_equal(g["t"], o["t"])
+    np.testing.assert_array_equal(g["iters"], o["iters"])
+    np.testing.assert_array_equal(g["status"], o["status"])
+
+
+def test_config4_small_trials_stationary(c4_trials, oracle):
+    """The five smallest trials maximised to the end without the sigma
+    update (the objective is then exactly the convolved energy): the FP64
+    oracle's pose gradient at the result vanishes next to its size one
+    degree / 0.01 units away."""
+    trials = [c4_trials[k] for k in C4_SMALL]
+    b = dg.Batch(trials)
+    try:
+        out = b.register(max_iter=2000, anneal=True, sigma_update=False, tol_rot=1e-8, tol_trans=1e-8)
+    finally:
+        b.free()
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    for k, tr in enumerate(trials):
+        assert out["status"][k] in (1, 2), (C4_SMALL[k], out["status"][k])
+        R, t = out["R"][k], out["t"][k]
+        Y, X = f32(tr["X"].T), f32(tr["Y"].T)
+        SY = f32(tr["sX"])[:, None, None] * np.eye(3)
+        SX = f32(tr["sY"])[:, None, None] * np.eye(3)
+        g = oracle.energy(Y, SY, X, SX, R, t, grads=True)["gpose"]
+        g_rot = oracle.energy(Y, SY, X, SX, _rot_about([1, 2, 3], 1.0) @ R, t, grads=True)["gpose"]
+        g_tr = oracle.energy(Y, SY, X, SX, R, t + np.array([0.01, -0.01, 0.01]), grads=True)["gpose"]
+        print("config 4 trial %d (%d x %d, outliers %.2f): rotation error %.3f deg, %d iterations"
+              % (C4_SMALL[k], tr["n"], tr["m"], tr["outliers"], W.rotation_error_deg(R, tr["R_G"]), out["iters"][k]))
+        assert np.linalg.norm(g[:3]) < 1e-2 * np.linalg.norm(g_rot[:3]), (C4_SMALL[k], g, g_rot)
+        assert np.linalg.norm(g[3:]) < 1e-2 * np.linalg.norm(g_tr[3:]), (C4_SMALL[k], g, g_tr)
