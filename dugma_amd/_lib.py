@@ -52,6 +52,7 @@ SIGNATURES = {
     "dugma_register": (_ci, [_d, _d, _d, _d, _ci, _ci, _ci, _d, _d, ctypes.POINTER(RegisterOpts), _d, _d, _i, _d]),
     "dugma_energy": (_ci, [_vp, _d, _d, _ci, _d]),
     "dugma_energy_grad": (_ci, [_vp, _d, _d, _ci, _d, _d]),
+    "dugma_energy_grad_noalloc": (_ci, [_vp, _d, _d, _ci, _d, _d]),
     "dugma_register_energy": (_ci, [_vp, _d, _d, ctypes.POINTER(RegisterEnergyOpts), _d, _d, _i, _d]),
     "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),
     "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),

@@ -318,6 +318,7 @@ extern "C" int dugma_init(const double *X, const double *X_C, const double *Y,
   {
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) TRY(hip_fail(e, "hipGetDevice"));
+    TRY(noalloc_setup(c->device));
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) TRY(hip_fail(e, "hipStreamCreateWithFlags"));
     // coherent: result slots are polled by the host while kernels run

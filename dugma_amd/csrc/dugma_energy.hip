@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -440,7 +441,7 @@ static bool sources_stationary() { return comp_enabled() && env_int("DUGMA_STATI
 // Energy-only or fused pose pass; res receives [E, rotation, translation]
 // in caller units.
 static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
-                     int mode, double *res) {
+                     int mode, double *res, double *fixed_parts = nullptr, size_t fixed_records = 0) {
   const int D = c->D;
   const int width = mode == kModeEnergy ? 1 : (D == 3 ? 7 : 4);
   for (int a = 0; a < width; ++a) res[a] = 0.0;
@@ -463,7 +464,12 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   const bool host_sum = nparts <= (size_t)dugma_ctx::kHostParts;
   double *parts = c->dparts;
   int rc;
-  if (!host_sum) {
+  if (!host_sum && fixed_parts) {
+    if (nparts > fixed_records)
+      return fail(DUGMA_ERR_SIZE, "dugma_energy_grad_noalloc: the launch needs more partial records than the "
+                                  "scratch array holds (lower DUGMA_TARGET_WAVES or the autotuned wave count)");
+    parts = fixed_parts;
+  } else if (!host_sum) {
     rc = c->part.ensure(nparts * kPosePart * sizeof(double));
     if (rc) return rc;
     parts = (double *)c->part.ptr;
@@ -600,9 +606,60 @@ static int point_pass(dugma_ctx *c, const void *P, int nP, const void *Q,
   return DUGMA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of dugma_energy_grad_noalloc: the pair kernel's per-workgroup
+// partials (kPosePart doubles each) in a file-scope device array instead of
+// the context's grow-only buffer.  Each GPU holds its own copy until the
+// process ends; its address on a device comes from hipGetSymbolAddress with
+// that device current (dugma_init calls noalloc_setup), and host code only
+// ever passes that address.  Every partial record of a launch is written by
+// its workgroup before k_reduce_parts reads it, so nothing is cleared.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocRecords = size_t(1) << 18;  // 16 MiB: 2^18 workgroups' partials
+static __device__ __align__(256) double scratch[kNoAllocRecords * kPosePart];
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoAllocDevices];
+
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return DUGMA_OK;  // the entry point reports it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DUGMA_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = static_cast<double *>(p);
+  return DUGMA_OK;
+}
+
+static double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 }  // namespace dugma
 
 using namespace dugma;
+
+extern "C" int dugma_energy_grad_noalloc(dugma_ctx *ctx, const double *R, const double *t, int cov_mode,
+                                         double *E, double *gpose) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  double *sp = noalloc_scratch(ctx->device);
+  if (!sp) return fail(DUGMA_ERR_STATE, "dugma_energy_grad_noalloc: no scratch address for this device");
+  bool iso;
+  rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  double res[kPosePart];
+  rc = pose_pass(ctx, R, t, iso, kModePose, res, sp, kNoAllocRecords);
+  if (rc) return rc;
+  if (E) *E = res[0];
+  const int nd = ctx->D == 3 ? 6 : 3;
+  if (gpose)
+    for (int a = 0; a < nd; ++a) gpose[a] = res[1 + a];
+  return DUGMA_OK;
+}
 
 extern "C" int dugma_energy(dugma_ctx *ctx, const double *R, const double *t,
                             int cov_mode, double *E) {

@@ -179,6 +179,9 @@ namespace dugma {
 // Bring the host copy of the source data (hSsrc) up to date after a sigma
 // update rewrote it on the device.
 int refresh_host_sources(dugma_ctx *c);
+// Device address of dugma_energy_grad_noalloc's scratch array on `device`
+// (the current device), looked up once per device.
+int noalloc_setup(int device);
 }  // namespace dugma
 
 // ---------------------------------------------------------------------------

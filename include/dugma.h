@@ -126,6 +126,19 @@ int dugma_energy(dugma_ctx *ctx, const double *R, const double *t, int cov_mode,
 int dugma_energy_grad(dugma_ctx *ctx, const double *R, const double *t,
                       int cov_mode, double *E, double *gpose);
 
+/* dugma_energy_grad without any allocation: no reference counterpart; for
+ * callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results (bit for bit) as dugma_energy_grad; the pair kernel's
+ * per-workgroup partials go to a file-scope device array of 2^18 records of
+ * 8 doubles (16 MiB, held by each GPU the library runs on until the process
+ * ends) instead of the context's grow-only buffer.  A launch that needs more
+ * partial records (beyond about 1.7e11 pairs at the default wave counts)
+ * fails with DUGMA_ERR_SIZE.  All calls on one GPU share that GPU's array:
+ * calls that overlap on one GPU (several threads or streams) race. */
+int dugma_energy_grad_noalloc(dugma_ctx *ctx, const double *R, const double *t,
+                              int cov_mode, double *E, double *gpose);
+
 /* Registration on the convolved energy itself: quasi-Newton (BFGS) descent
  * of -log E(R, t) over left pose increments R <- exp([w]x) R (2-D: a
  * rotation by theta on the left), t <- t + dt, re-centred on every accepted
