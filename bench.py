@@ -58,7 +58,8 @@ FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # 200,000, the autotuned launch): rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
 # each scaled by what the counter reads for a copy kernel of known size in the
 # same process (profiles/k_pairs_traffic_200000x200000_iso.txt).
-PROFILED_TRAFFIC_200K_ISO = None
+PROFILED_TRAFFIC_200K_ISO = 566933787
+TRAFFIC_PLAN = {"lane_points": 1, "target_waves": 524288}  # the launch those counters measured
 TRAFFIC_PROFILE = "profiles/k_pairs_traffic_200000x200000_iso.txt"
 
 
@@ -661,7 +662,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": PROFILED_TRAFFIC_200K_ISO, "traffic_source": TRAFFIC_PROFILE,
+                     "traffic": PROFILED_TRAFFIC_200K_ISO if fused["plan"] == TRAFFIC_PLAN else None,
+                     "traffic_source": TRAFFIC_PROFILE,
                      "kernel_ms": round(fused["kernel_ms"], 4), "flop_per_pair": FLOP_3D_ISO_FUSED,
                      "launch": fused["plan"]},
         "init_ms": round(fused["init_ms"], 3),

@@ -14,7 +14,14 @@
 //     or 12 (2-D) target features G = [1, n, n n^T, n^T B n, (B+B^T) n,
 //     sym(B)]; sources sit in lanes (64 per workgroup), the workgroup's
 //     target chunk is split over its 4 waves and streams through SGPRs; the
-//     waves' sums meet in LDS, one partial per (chunk, source).
+//     waves' sums meet in LDS, one partial per (chunk, source).  P's
+//     exponents are FP64, as in compi.cu:747-772: the pair difference from
+//     the FP64 transform, both Mahalanobis forms and log2 of the target's
+//     factor gX fl32(Isig^3/2) in FP64; each lane scales its sums by a
+//     running power of two (the largest exponent seen, raised in steps of at
+//     least 2^16), so that exp2 runs in FP32 on arguments <= 16 and pairs far
+//     below FP32's range keep their FP64 weight; the scale is applied to the
+//     lane's FP64 sums at the end.
 //  2. k_compi_mom: per block of 32 sources, the chunk partials are summed
 //     into the features, staged in LDS, and the 325 (85) moments
 //     sum_j W'_j,f * m'_j.. * A'_j,.. accumulated; k_compi_msum adds the
@@ -39,7 +46,8 @@ constexpr int kCWaves = kCBlock / 64;  // waves per workgroup of k_compi_w
 constexpr int kCSub = 64;
 constexpr int kMomSrc = 32;            // sources per workgroup of k_compi_mom
 constexpr int kFeat = 32;  // per-source feature stride (doubles) for the moments
-constexpr float kNegHalfLog2eC = -0.72134752044448170368f;
+constexpr double kNegHalfLog2eD = -0.72134752044448170368;  // -log2(e) / 2
+constexpr double kRescale = 16.0;  // a lane's scale rises when an exponent exceeds it by this (log2)
 
 struct CompiPose {
   double R[9];  // column-major
@@ -48,18 +56,34 @@ struct CompiPose {
   float gsc;    // sqrtf(Isig^3) in 3-D (compi.cu:662), Isig in 2-D (compi.cu:126)
 };
 
-// Streamed target record: n, sym(B), gX, then the features n n^T, n^T B n,
-// (B+B^T) n.  3-D: 3+6+1+6+1+3 = 20 floats (80 B).  2-D: 2+3+1+3+1+2 = 12
-// floats (48 B).
+// Streamed target record: the exponent's inputs in FP64 (the quadratic
+// form of B = fl32(inv(CX) Isig) as [B11 B22 B33 2B12 2B13 2B23] (2-D
+// [B11 B22 2B12]) and lg = log2(fl32(gX gsc))), then n, sym(B) and the
+// features n n^T, n^T B n, (B+B^T) n in FP32.  3-D 144 B, 2-D 80 B.
 struct alignas(16) TgtRec3 {
-  float n[3], b[6], g;
+  double q[6], lg;
+  float n[3], b[6];
   float nn[6], s, v[3];
 };
 struct alignas(16) TgtRec2 {
-  float n[2], b[3], g;
+  double q[3], lg;
+  float n[2], b[3];
   float nn[3], s, v[2];
 };
-static_assert(sizeof(TgtRec3) == 80 && sizeof(TgtRec2) == 48, "target record layout");
+static_assert(sizeof(TgtRec3) == 144 && sizeof(TgtRec2) == 80, "target record layout");
+
+// Per-lane source record: C = R A^T R^T (2-D R A R^T) as a quadratic form
+// in FP64 ([C11 C22 C33 2C12 2C13 2C23]), m' = R y + t as FP32 hi + lo, and
+// gY fl32(gsc).  3-D 80 B, 2-D 48 B.
+struct alignas(16) SrcRec3 {
+  double q[6];
+  float mh[3], ml[3], g, pad;
+};
+struct alignas(16) SrcRec2 {
+  double q[3];
+  float mh[2], ml[2], g, pad;
+};
+static_assert(sizeof(SrcRec3) == 80 && sizeof(SrcRec2) == 48, "source record layout");
 
 // sym index of (a,b), a <= b: 3-D 00 01 02 11 12 22 ; 2-D 00 01 11
 __host__ __device__ constexpr int sidx3(int a, int b) {
@@ -91,6 +115,7 @@ __device__ void compi_tgt(int i, const float *__restrict__ X, const float *__res
     v[a] = va;
   }
   const float g = gX[i] * ps.gsc;
+  const double lg = log2((double)g);  // -inf for the fallback g = 0
   if (D == 3) {
     TgtRec3 r;
     for (int a = 0; a < 3; ++a) {
@@ -99,10 +124,17 @@ __device__ void compi_tgt(int i, const float *__restrict__ X, const float *__res
     }
     for (int a = 0; a < 3; ++a)
       for (int b = a; b < 3; ++b) {
-        r.b[sidx3(a, b)] = (float)(0.5 * ((double)B[a + 3 * b] + (double)B[b + 3 * a]));
+        const double sab = 0.5 * ((double)B[a + 3 * b] + (double)B[b + 3 * a]);
+        r.b[sidx3(a, b)] = (float)sab;
         r.nn[sidx3(a, b)] = (float)(n[a] * n[b]);
       }
-    r.g = g;
+    r.q[0] = B[0];
+    r.q[1] = B[4];
+    r.q[2] = B[8];
+    r.q[3] = (double)B[3] + (double)B[1];
+    r.q[4] = (double)B[6] + (double)B[2];
+    r.q[5] = (double)B[7] + (double)B[5];
+    r.lg = lg;
     r.s = (float)s;
     reinterpret_cast<TgtRec3 *>(outv)[i] = r;
   } else {
@@ -116,7 +148,10 @@ __device__ void compi_tgt(int i, const float *__restrict__ X, const float *__res
         r.b[sidx2(a, b)] = (float)(0.5 * ((double)B[a + 2 * b] + (double)B[b + 2 * a]));
         r.nn[sidx2(a, b)] = (float)(n[a] * n[b]);
       }
-    r.g = g;
+    r.q[0] = B[0];
+    r.q[1] = B[3];
+    r.q[2] = (double)B[2] + (double)B[1];
+    r.lg = lg;
     r.s = (float)s;
     reinterpret_cast<TgtRec2 *>(outv)[i] = r;
   }
@@ -125,14 +160,11 @@ __device__ void compi_tgt(int i, const float *__restrict__ X, const float *__res
 // Per-call source state: m' = R y + t (FP64; hi = fl32(m') is the reference's
 // m', lo keeps the rest for the pair difference), C = R A^T R^T (FP64, 3-D;
 // R A R^T in 2-D as compi.cu:180-183 writes it), gY.
-// src (floats per source): 3-D [mh 3, ml 3, C sym 6, gY, pad 3] = 16
-//                          2-D [mh 2, ml 2, C sym 3, gY] = 8
 // feat (doubles per source, kFeat stride): [.. W set later .., m' at FM, A' at FA]
 template <int D>
 __device__ void compi_src(int j, const float *__restrict__ Y, const float *__restrict__ YCinv,
-                          const float *__restrict__ gY, const CompiPose &ps, float *__restrict__ src,
+                          const float *__restrict__ gY, const CompiPose &ps, void *__restrict__ srcv,
                           double *__restrict__ feat) {
-  constexpr int SS = D == 3 ? 16 : 8;
   constexpr int FM = D == 3 ? 20 : 12;
   constexpr int FA = FM + D;
   float A[9];
@@ -157,26 +189,47 @@ __device__ void compi_src(int j, const float *__restrict__ Y, const float *__res
       }
       C[p + D * q] = acc;
     }
-  float *o = src + (size_t)j * SS;
   double *f = feat + (size_t)j * kFeat;
+  float mh[3], ml[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     double v = ps.R[a] * (double)Y[(size_t)j * D];
 #pragma unroll
     for (int b = 1; b < D; ++b) v = v + ps.R[a + D * b] * (double)Y[(size_t)j * D + b];
     v = v + ps.t[a];
-    const float hi = (float)v;
-    o[a] = hi;
-    o[D + a] = (float)(v - (double)hi);
-    f[FM + a] = (double)hi;
+    mh[a] = (float)v;
+    ml[a] = (float)(v - (double)mh[a]);
+    f[FM + a] = (double)mh[a];
   }
-  int k = 2 * D;
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-#pragma unroll
-    for (int b = a; b < D; ++b) o[k++] = (float)(0.5 * (C[a + D * b] + C[b + D * a]));
-  o[k++] = gY[j] * ps.gsc;
-  for (; k < SS; ++k) o[k] = 0.0f;
+  const float g = gY[j] * ps.gsc;
+  if (D == 3) {
+    SrcRec3 r;
+    r.q[0] = C[0];
+    r.q[1] = C[4];
+    r.q[2] = C[8];
+    r.q[3] = C[3] + C[1];
+    r.q[4] = C[6] + C[2];
+    r.q[5] = C[7] + C[5];
+    for (int a = 0; a < 3; ++a) {
+      r.mh[a] = mh[a];
+      r.ml[a] = ml[a];
+    }
+    r.g = g;
+    r.pad = 0.0f;
+    reinterpret_cast<SrcRec3 *>(srcv)[j] = r;
+  } else {
+    SrcRec2 r;
+    r.q[0] = C[0];
+    r.q[1] = C[3];
+    r.q[2] = C[2] + C[1];
+    for (int a = 0; a < 2; ++a) {
+      r.mh[a] = mh[a];
+      r.ml[a] = ml[a];
+    }
+    r.g = g;
+    r.pad = 0.0f;
+    reinterpret_cast<SrcRec2 *>(srcv)[j] = r;
+  }
 #pragma unroll
   for (int q = 0; q < D * D; ++q) f[FA + q] = (double)(float)C[q];
 }
@@ -187,7 +240,7 @@ template <int D>
 __global__ __launch_bounds__(kCBlock) void k_compi_prep(
     const float *__restrict__ X, const float *__restrict__ XCinv, const float *__restrict__ gX, int N,
     const float *__restrict__ Y, const float *__restrict__ YCinv, const float *__restrict__ gY, int M,
-    CompiPose ps, void *__restrict__ tgt, float *__restrict__ src, double *__restrict__ feat) {
+    CompiPose ps, void *__restrict__ tgt, void *__restrict__ src, double *__restrict__ feat) {
   const int i = blockIdx.x * kCBlock + threadIdx.x;
   if (i < N) compi_tgt<D>(i, X, XCinv, gX, ps, tgt);
   if (i < M) compi_src<D>(i, Y, YCinv, gY, ps, src, feat);
@@ -196,28 +249,30 @@ __global__ __launch_bounds__(kCBlock) void k_compi_prep(
 // W_j = sum_i P_ij G_i.  Lane = source j (stationary, 64 per workgroup); the
 // workgroup's target chunk is split into kCWaves contiguous parts, one per
 // wave, streamed through SGPRs one record ahead (index clamped to the part).
-// The waves' FP64 sums meet in LDS in a fixed order: (w0 + w2) + (w1 + w3).
+// Per pair: d = n_i - m'_j in FP64, e1 = -log2(e)/2 d^T C_j d + lg_i,
+// e2 = -log2(e)/2 d^T B_i d + lg_i (FP64), P 2^-o = exp2(e1 - o) + exp2(e2 - o)
+// in FP32 with the lane's running scale o.  The sums are FP32 over sub-tiles
+// of kCSub targets, then FP64; at the end each wave's sums are multiplied by
+// 2^o in FP64 and the waves' sums meet in LDS in a fixed order:
+// (w0 + w2) + (w1 + w3).
 template <int D>
 __global__ __launch_bounds__(kCBlock) void k_compi_w(
-    const float *__restrict__ src, int M, const void *__restrict__ tgtv, int N,
+    const void *__restrict__ srcv, int M, const void *__restrict__ tgtv, int N,
     int chunk, double *__restrict__ part) {
   constexpr int NW = D == 3 ? 20 : 12;
-  constexpr int SS = D == 3 ? 16 : 8;
   using Tgt = typename std::conditional<D == 3, TgtRec3, TgtRec2>::type;
+  using Src = typename std::conditional<D == 3, SrcRec3, SrcRec2>::type;
   const Tgt *__restrict__ T = reinterpret_cast<const Tgt *>(tgtv);
   // the wave index is uniform: say so, so the target stream uses scalar loads
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = blockIdx.x * 64 + lane;
   const int jj = min(j, M - 1);
-  const float *o = src + (size_t)jj * SS;
-  float mh[3], ml[3], C[6];
+  const Src sr = reinterpret_cast<const Src *>(srcv)[jj];
+  double m[3], cq[6];
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    mh[a] = o[a];
-    ml[a] = o[D + a];
-  }
+  for (int a = 0; a < D; ++a) m[a] = (double)sr.mh[a] + (double)sr.ml[a];  // exact: m' to 2^-48
 #pragma unroll
-  for (int q = 0; q < D * (D + 1) / 2; ++q) C[q] = o[2 * D + q];
+  for (int q = 0; q < (D == 3 ? 6 : 3); ++q) cq[q] = sr.q[q];
   const int c0 = blockIdx.y * chunk;
   const int c1 = min(c0 + chunk, N);
   const int per = (c1 - c0 + kCWaves - 1) / kCWaves;
@@ -226,6 +281,7 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
   double W[NW];
 #pragma unroll
   for (int k = 0; k < NW; ++k) W[k] = 0.0;
+  double o = -INFINITY;  // running scale (log2) of this lane's sums
   if (i0 < i1) {
     Tgt tn = T[i0];
     for (int s0 = i0; s0 < i1; s0 += kCSub) {
@@ -236,21 +292,36 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
       for (int i = s0; i < s1; ++i) {
         const Tgt t = tn;
         tn = T[min(i + 1, i1 - 1)];
+        double e1, e2;
         if constexpr (D == 3) {
-          const float dx = (t.n[0] - mh[0]) - ml[0];
-          const float dy = (t.n[1] - mh[1]) - ml[1];
-          const float dz = (t.n[2] - mh[2]) - ml[2];
-          // mal1 = d^T C d, mal2 = d^T B d
-          const float c1 = fmaf(C[0], dx, fmaf(C[1], dy, C[2] * dz));
-          const float c2 = fmaf(C[1], dx, fmaf(C[3], dy, C[4] * dz));
-          const float c3 = fmaf(C[2], dx, fmaf(C[4], dy, C[5] * dz));
-          const float mal1 = fmaf(dx, c1, fmaf(dy, c2, dz * c3));
-          const float b1 = fmaf(t.b[0], dx, fmaf(t.b[1], dy, t.b[2] * dz));
-          const float b2 = fmaf(t.b[1], dx, fmaf(t.b[3], dy, t.b[4] * dz));
-          const float b3 = fmaf(t.b[2], dx, fmaf(t.b[4], dy, t.b[5] * dz));
-          const float mal2 = fmaf(dx, b1, fmaf(dy, b2, dz * b3));
-          const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
-                                 __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
+          const double dx = (double)t.n[0] - m[0], dy = (double)t.n[1] - m[1], dz = (double)t.n[2] - m[2];
+          const double xx = dx * dx, yy = dy * dy, zz = dz * dz, xy = dx * dy, xz = dx * dz, yz = dy * dz;
+          const double mal1 = fma(cq[0], xx, fma(cq[1], yy, fma(cq[2], zz, fma(cq[3], xy, fma(cq[4], xz, cq[5] * yz)))));
+          const double mal2 =
+              fma(t.q[0], xx, fma(t.q[1], yy, fma(t.q[2], zz, fma(t.q[3], xy, fma(t.q[4], xz, t.q[5] * yz)))));
+          e1 = fma(mal1, kNegHalfLog2eD, t.lg);
+          e2 = fma(mal2, kNegHalfLog2eD, t.lg);
+        } else {
+          const double dx = (double)t.n[0] - m[0], dy = (double)t.n[1] - m[1];
+          const double xx = dx * dx, yy = dy * dy, xy = dx * dy;
+          const double mal1 = fma(cq[0], xx, fma(cq[1], yy, cq[2] * xy));
+          const double mal2 = fma(t.q[0], xx, fma(t.q[1], yy, t.q[2] * xy));
+          e1 = fma(mal1, kNegHalfLog2eD, t.lg);
+          e2 = fma(mal2, kNegHalfLog2eD, t.lg);
+        }
+        const double em = fmax(e1, e2);
+        if (em > o + kRescale) {
+          // the lane's largest weight so far: rescale its sums (rare)
+          const float f = __builtin_amdgcn_exp2f((float)(o - em));
+#pragma unroll
+          for (int k = 0; k < NW; ++k) {
+            w[k] *= f;
+            W[k] *= (double)f;
+          }
+          o = em;
+        }
+        const float P = __builtin_amdgcn_exp2f((float)(e1 - o)) + __builtin_amdgcn_exp2f((float)(e2 - o));
+        if constexpr (D == 3) {
           w[0] += P;
 #pragma unroll
           for (int a = 0; a < 3; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
@@ -262,12 +333,6 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
 #pragma unroll
           for (int a = 0; a < 6; ++a) w[14 + a] = fmaf(P, t.b[a], w[14 + a]);
         } else {
-          const float dx = (t.n[0] - mh[0]) - ml[0];
-          const float dy = (t.n[1] - mh[1]) - ml[1];
-          const float mal1 = fmaf(dx, fmaf(C[0], dx, C[1] * dy), dy * fmaf(C[1], dx, C[2] * dy));
-          const float mal2 = fmaf(dx, fmaf(t.b[0], dx, t.b[1] * dy), dy * fmaf(t.b[1], dx, t.b[2] * dy));
-          const float P = t.g * (__builtin_amdgcn_exp2f(mal1 * kNegHalfLog2eC) +
-                                 __builtin_amdgcn_exp2f(mal2 * kNegHalfLog2eC));
           w[0] += P;
 #pragma unroll
           for (int a = 0; a < 2; ++a) w[1 + a] = fmaf(P, t.n[a], w[1 + a]);
@@ -284,6 +349,10 @@ __global__ __launch_bounds__(kCBlock) void k_compi_w(
       for (int k = 0; k < NW; ++k) W[k] += (double)w[k];
     }
   }
+  // back to the true scale in FP64 (o = -inf: nothing was added)
+  const double sc = o == -INFINITY ? 0.0 : exp2(o);
+#pragma unroll
+  for (int k = 0; k < NW; ++k) W[k] *= sc;
   __shared__ double red[2][NW][64];
   if (wave >= 2)
 #pragma unroll
@@ -316,11 +385,11 @@ struct MomTab {
 template <int D>
 __global__ __launch_bounds__(kCBlock) void k_compi_mom(
     const double *__restrict__ wpart, int M, int nchunks,
-    const float *__restrict__ src, const double *__restrict__ feat, int nmom,
+    const void *__restrict__ srcv, const double *__restrict__ feat, int nmom,
     MomTab tab, double *__restrict__ mpart) {
   constexpr int NW = D == 3 ? 20 : 12;
-  constexpr int SS = D == 3 ? 16 : 8;
-  constexpr int GI = D == 3 ? 12 : 7;  // gY position in the source record
+  using Src = typename std::conditional<D == 3, SrcRec3, SrcRec2>::type;
+  const Src *__restrict__ src = reinterpret_cast<const Src *>(srcv);
   __shared__ double fs[kMomSrc][kFeat];
   const int j0 = blockIdx.x * kMomSrc;
   for (int e = threadIdx.x; e < kMomSrc * kFeat; e += kCBlock) {
@@ -342,7 +411,7 @@ __global__ __launch_bounds__(kCBlock) void k_compi_mom(
         }
         for (; c < nchunks; ++c) acc[0] += p[(size_t)c * stride];
         const double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        v = sum * (double)src[(size_t)j * SS + GI];
+        v = sum * (double)src[j].g;
       } else if (k < NW + D + D * D) {  // m' and A'
         v = feat[(size_t)j * kFeat + k];
       }
@@ -669,7 +738,7 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   const int nmom = (int)moms.size();
   const int NW = Dd == 3 ? 20 : 12;
   const size_t trec = Dd == 3 ? sizeof(TgtRec3) : sizeof(TgtRec2);
-  const int SS = Dd == 3 ? 16 : 8;
+  const size_t srec = Dd == 3 ? sizeof(SrcRec3) : sizeof(SrcRec2);
   // launch shapes: 64 sources per workgroup of k_compi_w, target chunks so
   // that about 2,048 workgroups run, each wave with at least 64 targets
   const int sblocks = (m + 63) / 64;
@@ -681,7 +750,7 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   // scratch: tgt records | src | feat | W partials | moment partials | moments
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t o_src = al(trec * n);
-  const size_t o_feat = o_src + al(sizeof(float) * SS * (size_t)m);
+  const size_t o_feat = o_src + al(srec * (size_t)m);
   const size_t o_part = o_feat + al(sizeof(double) * kFeat * (size_t)m);
   const size_t o_mp = o_part + al(sizeof(double) * (size_t)nchunks * m * NW);
   const size_t o_mom = o_mp + al(sizeof(double) * (size_t)mblocks * nmom);
@@ -689,7 +758,7 @@ extern "C" int dugma_compi(dugma_ctx *ctx, const double *R, const double *t,
   if (rc) return rc;
   char *base = (char *)c->scratch2.ptr;
   void *tgt = base;
-  float *src = (float *)(base + o_src);
+  void *src = base + o_src;
   double *feat = (double *)(base + o_feat);
   double *wpart = (double *)(base + o_part);
   double *mpart = (double *)(base + o_mp);

@@ -69,6 +69,53 @@ def test_compi_3d_kinect_with_fallback_points(oracle):
         check(oracle, d, np.eye(3), np.zeros(3), 0.05)
 
 
+# per-coefficient bar at full size: P_ij is FP32 on the GPU (two exp2 of
+# FP32 quadratic forms of the hi/lo pair difference), FP64 in the reference
+# (compi.cu:747-772); coefficients are P-weighted sums, so each carries about
+# the per-pair relative error of P (a few FP32 ulps times the exponent).
+FULL_REL = 2e-5
+
+
+@pytest.mark.parametrize("pose", ["truth", "start"])
+def test_compi_full_size_per_coefficient(oracle, pose):
+    """6,000 x 6,000 structured-light clouds (the size bench.py times compi
+    on), sigma from update_sig at the pose: every coefficient above 1e-8 of
+    the largest within FULL_REL of the oracle's FP64 moment form (pinned
+    against the pair-by-pair expansion and compi.cu's own expressions), and
+    the polynomial's value at random updates near identity within FULL_REL
+    of the size of its terms."""
+    d = W.config2(n=6000, m=6000)
+    if pose == "truth":
+        R, t = d["R_G"], np.ravel(d["t_G"])
+    else:
+        rng = np.random.default_rng(3)
+        R, t = W.random_rotation(rng, 2.0) @ d["R_G"], np.ravel(d["t_G"]) + rng.uniform(-0.003, 0.003, size=3)
+    st = device_state(oracle, d)
+    sig = oracle.update_sig(st[0], st[3], R, t)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        assert dg.update_sig(h, R, t, h.N, h.M, 3) == sig
+        coef = dg.compi(h, R, t, h.N, h.M, 3, sig)
+    finally:
+        dg.fr_m(h)
+    ref, missing = oracle.compi_moments(*st, R, t, sig)
+    assert missing == 0
+    big = np.abs(ref) > 1e-8 * np.abs(ref).max()
+    rel = np.abs(coef[big] - ref[big]) / np.abs(ref[big])
+    print("compi 6K (%s pose): %d of 637 coefficients above the floor; per-coefficient relative error "
+          "max %.3g, median %.3g, 99th percentile %.3g"
+          % (pose, big.sum(), rel.max(), np.median(rel), np.percentile(rel, 99)))
+    assert rel.max() <= FULL_REL, (np.argmax(rel), rel.max())
+    rng = np.random.default_rng(11)
+    ident = np.concatenate([np.zeros(3), np.eye(3).ravel()])
+    for _ in range(5):
+        dT = ident + rng.normal(size=12) * np.r_[np.full(3, 1e-3), np.full(9, 1e-3)]
+        F1 = oracle.poly_eval(coef, dT)
+        F2 = oracle.poly_eval(ref, dT)
+        mag = oracle.poly_eval(np.abs(ref), np.abs(dT))
+        assert abs(F1 - F2) <= FULL_REL * mag
+
+
 def test_compi_2d(oracle):
     d = W.config5_2d(n=300, m=260)
     check(oracle, d, d["R_G"], d["t_G"], 0.01)
