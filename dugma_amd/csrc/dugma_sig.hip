@@ -55,7 +55,9 @@ __device__ __forceinline__ unsigned f2u_rn(float s) {
 
 // update_sig.cu:238-265: per target the minimum over a chunk of sources of
 // rn_u32(fl32(fl32(|n - m|^2) / D) * 1e6), merged with atomicMin (the
-// minimum does not depend on the order).
+// minimum does not depend on the order).  IEEE FP32 semantics in source
+// order; the reference's binaries use -use_fast_math (compile.m:2), whose
+// division may differ in the last bit.
 template <int D>
 __global__ __launch_bounds__(kSBlock) void k_nn1_quant(
     const float *__restrict__ X, int N, const float *__restrict__ Mp, int M,

@@ -1,5 +1,9 @@
 """GPU parity of init (the six reference handles), update_sig and k-NN:
-bit-exact against the oracle."""
+bit-exact against the oracle.  For init and update_sig the oracle restates
+the reference's FP32 operations in source order with IEEE semantics (no
+contraction, correctly rounded division and sqrtf); the reference's own
+binaries are built with -use_fast_math (compile.m:2), so equality with them
+is not established here."""
 import os
 
 import math
