@@ -21,7 +21,9 @@
 // annealed on top.
 //
 // One iteration for all trials is three kernels:
-//   k_batch_transform  x'_j = R_k x_j + t_k - c_k (FP64 -> FP32 hi/lo), g s_j + a
+//   k_batch_transform  the targets in each trial's source frame,
+//                      z_i = R_k^T (y_i - (t_k - c_k)) (FP64 -> FP32 hi/lo),
+//                      and g s_i + a; the sources stay as given (FP32)
 //   k_batch_pairs      persistent; workgroups pop (trial, target tile, source
 //                      chunk) items from a queue ordered longest first and
 //                      write FP64 sufficient statistics per item, plus the
@@ -96,34 +98,46 @@ struct BParams {
   int sigma;         // 1: DUGMA sigma update
 };
 
+// A streamed source point: the input coordinates (FP32) and its variance
+// times ln 4.  Static: the pose moves the targets into the source frame.
+struct alignas(16) BSrc {
+  float x, y, z, s;
+};
+
+// Per iteration, for every target of a running trial: z = R^T (y - (t - c))
+// in FP64 (y relative to the trial's target centroid c), split into FP32 hi
+// + lo, with (g s_i + a) ln 4 and g.  z - x_j = R^T (y - R x_j - t) has the
+// length of the pair difference in the target frame.
 __global__ __launch_bounds__(256) void k_batch_transform(
-    const double *__restrict__ src, const int *__restrict__ src_trial, int s0,
-    int s1, const BTrial *__restrict__ trials, const int *__restrict__ done,
-    Rec3I *__restrict__ S) {
-  const int j = s0 + blockIdx.x * 256 + threadIdx.x;
-  if (j >= s1) return;
-  const int k = src_trial[j];
+    const Rec3I *__restrict__ T, const int *__restrict__ tgt_trial, int i0,
+    int i1, const BTrial *__restrict__ trials, const int *__restrict__ done,
+    Rec3I *__restrict__ Z) {
+  const int i = i0 + blockIdx.x * 256 + threadIdx.x;
+  if (i >= i1) return;
+  const int k = tgt_trial[i];
   if (done[k]) return;
   const BTrial &tr = trials[k];
-  const double *x = src + (size_t)j * 4;
-  Rec3I r;
+  const Rec3I y = T[i];
+  const double u[3] = {(double)y.hx - (tr.t[0] - tr.c[0]), (double)y.hy - (tr.t[1] - tr.c[1]),
+                       (double)y.hz - (tr.t[2] - tr.c[2])};
   double v[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    double s = tr.t[a] - tr.c[a];
+    double s = 0.0;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) s = fma(tr.R[a + 3 * b], x[b], s);
+    for (int b = 0; b < 3; ++b) s = fma(tr.R[b + 3 * a], u[b], s);  // (R^T u)_a
     v[a] = s;
   }
+  Rec3I r;
   r.hx = (float)v[0];
   r.hy = (float)v[1];
   r.hz = (float)v[2];
   r.lx = (float)(v[0] - (double)r.hx);
   r.ly = (float)(v[1] - (double)r.hy);
   r.lz = (float)(v[2] - (double)r.hz);
-  r.s = (float)(fma(tr.g, x[3], tr.a) * kLn4);
-  r.pad0 = 0.0f;
-  S[j] = r;
+  r.s = (float)fma(tr.g, (double)y.s, tr.a * kLn4);  // y.s holds s_i ln 4
+  r.pad0 = (float)tr.g;
+  Z[i] = r;
 }
 
 __global__ __launch_bounds__(256) void k_batch_fill(unsigned *__restrict__ p, int n, unsigned v) {
@@ -138,32 +152,35 @@ __global__ __launch_bounds__(256) void k_batch_resig(BTrial *__restrict__ trials
   if (k < K) trials[k].sig_ready = 0;
 }
 
-// Statistics of one item, per target i (lane): W_i = sum_j w_ij,
-// B_i = sum_j w_ij x'_j, E_i = sum_j e_ij with w = e / s_ij,
-// s_ij = g s_i + (g s_j + a).  Item totals: [E, W, sum W_i y_i (3),
-// sum B_i (3), sum y_i B_i^T (9, row a = y_a)].
+// Statistics of one item, per target i (lane), in the trial's source frame:
+// W_i = sum_j w_ij, B'_i = sum_j w_ij x_j, E_i = sum_j e_ij with
+// w = e / s_ij, s_ij = g (s_i + s_j) + a.  Item totals: [E, W,
+// sum W_i y_i (3), sum B'_i (3), sum y_i B'_i^T (9, row a = y_a)], y_i in the
+// target frame relative to the centroid; the update kernel rotates the B'
+// parts into the target frame.
 // Every operand of the pair arithmetic that is not a streamed value sits in
 // a VGPR (VALU instructions reading an SGPR or a literal issue at half rate):
-//  - B_i comes from D_i = sum_j w_ij d_ij, d = y - x', as B_i = W_i y_i - D_i;
-//  - variances are held times ln 4 (records and g s_i), so that
-//    exp2(-|d|^2 / s') is the Gaussian exp(-|d|^2 / 2s) with no constant
-//    factor per pair; e and w then carry the constant factors (ln 4)^-3/2
-//    and (ln 4)^-5/2, which the pose step does not see (ratios of sums) and
-//    the update kernel takes out of E.
-// With SIG the item's
-// smallest |d_ij|^2 per target is folded into nn[i] (atomicMin on the bits of
-// a non-negative float, which order like the values).  The source stream is
-// software-pipelined: record j + 1 is requested before record j is used
-// (index clamped to the item), so the scalar-load latency overlaps the
-// arithmetic.
+//  - the pair difference is d' = (z_hi - x) + z_lo: one streamed operand
+//    per axis, the lo correction on VGPRs;
+//  - B'_i comes from D'_i = sum_j w_ij d'_ij as B'_i = W_i z_i - D'_i;
+//  - s_ij = fma(s_j, g, g s_i + a), g and g s_i + a in VGPRs;
+//  - variances are held times ln 4, so that exp2(-|d|^2 / s') is the
+//    Gaussian exp(-|d|^2 / 2s) with no constant factor per pair; e and w
+//    then carry the constant factors (ln 4)^-3/2 and (ln 4)^-5/2, which the
+//    pose step does not see (ratios of sums) and the update kernel takes out
+//    of E.
+// With SIG the item's smallest |d_ij|^2 per target is folded into nn[i]
+// (atomicMin on the bits of a non-negative float, which order like the
+// values).  The source stream is software-pipelined two deep: record j + 2
+// is requested while record j is used (index clamped to the item), so the
+// scalar-load latency overlaps the arithmetic.
 constexpr int kItemStat = 17;
 template <int W, bool SIG>
 __global__ __launch_bounds__(64 * W) void k_batch_pairs(
-    const Rec3I *__restrict__ T, const Rec3I *__restrict__ S,
+    const Rec3I *__restrict__ T, const Rec3I *__restrict__ Z, const BSrc *__restrict__ S,
     const BItem *__restrict__ items, const int *__restrict__ order, int nitems,
     unsigned *__restrict__ queue, const int *__restrict__ done,
-    const BTrial *__restrict__ trials, unsigned *__restrict__ nn,
-    double *__restrict__ part) {
+    unsigned *__restrict__ nn, double *__restrict__ part) {
   __shared__ int s_p;
   __shared__ double red[W][kItemStat];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -176,28 +193,28 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const int idx = __builtin_amdgcn_readfirstlane(order[p]);
     const BItem it = items[idx];
     if (done[it.trial]) continue;
-    const float g = (float)trials[it.trial].g;
     const int ti = it.t0 + wave * 64 + lane;
     const bool valid = (wave * 64 + lane) < it.tn;
-    const Rec3I pt = T[valid ? ti : it.t0];
-    const float gs = g * pt.s;
+    const Rec3I pz = Z[valid ? ti : it.t0];
+    const float g = pz.pad0, cs = pz.s;
     double dE = 0.0, dW = 0.0, dD[3] = {0.0, 0.0, 0.0};
     float dmin = __builtin_huge_valf();
     const int s_end = it.s0 + it.sn;
-    Rec3I qn = S[it.s0];  // it.sn >= 1
+    BSrc qa = S[it.s0], qb = S[min(it.s0 + 1, s_end - 1)];  // it.sn >= 1
     for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
       float fE = 0.0f, fW = 0.0f, fD0 = 0.0f, fD1 = 0.0f, fD2 = 0.0f;
       const int j1 = min(j0 + kBSub, s_end);
 #pragma unroll 2
       for (int j = j0; j < j1; ++j) {
-        const Rec3I q = qn;
-        qn = S[min(j + 1, s_end - 1)];
-        const float dx = (pt.hx - q.hx) - q.lx;
-        const float dy = (pt.hy - q.hy) - q.ly;
-        const float dz = (pt.hz - q.hz) - q.lz;
+        const BSrc q = qa;
+        qa = qb;
+        qb = S[min(j + 2, s_end - 1)];
+        const float dx = (pz.hx - q.x) + pz.lx;
+        const float dy = (pz.hy - q.y) + pz.ly;
+        const float dz = (pz.hz - q.z) + pz.lz;
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         if (SIG) dmin = fminf(dmin, d2);
-        const float rs = frsq(gs + q.s);
+        const float rs = frsq(fmaf(q.s, g, cs));
         const float inv = rs * rs;
         const float e = fexp2(-(d2 * inv)) * (rs * inv);
         const float w = e * inv;
@@ -215,9 +232,12 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     }
     if (SIG && valid) atomicMin(nn + ti, __float_as_uint(dmin));
     // item totals, one statistic at a time (few registers live): [E, W,
-    // W y (3), B (3), y B^T (9)]
+    // W y (3), B' (3), y B'^T (9)]
+    const Rec3I pt = T[valid ? ti : it.t0];
     const double y[3] = {(double)pt.hx, (double)pt.hy, (double)pt.hz};
-    const double dB[3] = {fma(dW, y[0], -dD[0]), fma(dW, y[1], -dD[1]), fma(dW, y[2], -dD[2])};
+    const double z[3] = {(double)pz.hx + (double)pz.lx, (double)pz.hy + (double)pz.ly,
+                         (double)pz.hz + (double)pz.lz};
+    const double dB[3] = {fma(dW, z[0], -dD[0]), fma(dW, z[1], -dD[1]), fma(dW, z[2], -dD[2])};
 #pragma unroll
     for (int a = 0; a < kItemStat; ++a) {
       double v;
@@ -332,6 +352,22 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   for (int i = titem[k]; i < titem[k + 1]; ++i)
     for (int a = 0; a < kItemStat; ++a) st[a] += part[(size_t)i * kStat + a];
   const double W = st[1];
+  {
+    // B' parts from the source frame to the target frame (relative to c):
+    // B = R B' + W (t - c);  sum y B^T = (sum y B'^T) R^T + (sum W y)(t - c)^T
+    double tc[3], Bp[3], YB[9];
+    for (int a = 0; a < 3; ++a) {
+      tc[a] = tr.t[a] - tr.c[a];
+      Bp[a] = st[5 + a];
+    }
+    for (int a = 0; a < 9; ++a) YB[a] = st[8 + a];
+    for (int a = 0; a < 3; ++a)
+      st[5 + a] = tr.R[a] * Bp[0] + tr.R[a + 3] * Bp[1] + tr.R[a + 6] * Bp[2] + W * tc[a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        st[8 + 3 * a + b] = YB[3 * a] * tr.R[b] + YB[3 * a + 1] * tr.R[b + 3] + YB[3 * a + 2] * tr.R[b + 6] +
+                            st[2 + a] * tc[b];
+  }
   tr.E = st[0] * kLn4Pow1p5;  // e was computed with variances times ln 4
   tr.iters += 1;
   if (!(W > 0.0) || !isfinite(W)) {
@@ -424,9 +460,9 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
 struct BLevel {
   int ntgt = 0, nsrc = 0, nitems = 0;
   Rec3I *dT = nullptr;       // targets (all trials), relative to each trial's centroid
-  Rec3I *dS = nullptr;       // per-iteration source records
-  double *dsrc = nullptr;    // sources: x y z s per point (FP64, FP32-rounded inputs)
-  int *dsrc_trial = nullptr;
+  Rec3I *dZ = nullptr;       // per-iteration targets in each trial's source frame
+  int *dtgt_trial = nullptr; // trial of each target
+  BSrc *dS = nullptr;        // sources (all trials), as given
   BItem *ditems = nullptr;
   int *dorder = nullptr;     // queue position -> item, longest first
   int *dident = nullptr;     // identity order for one-trial launches
@@ -461,7 +497,7 @@ struct dugma_batch {
 };
 
 static void free_level(BLevel &L) {
-  void *ptrs[] = {L.dT, L.dS, L.dsrc, L.dsrc_trial, L.ditems, L.dorder, L.dident,
+  void *ptrs[] = {L.dT, L.dZ, L.dtgt_trial, L.dS, L.ditems, L.dorder, L.dident,
                   L.dtitem, L.dttgt, L.dnn, L.dpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -555,17 +591,17 @@ static int make_all_items(dugma_batch *b) {
 // Host copies of one level's points, built per trial, then uploaded.
 struct LevelHost {
   std::vector<Rec3I> tg;
-  std::vector<double> sr;
-  std::vector<int> strial, ttgt;
+  std::vector<BSrc> sr;
+  std::vector<int> ttrial, ttgt;
 };
 
 static int upload_level(dugma_batch *b, BLevel &L, const LevelHost &H) {
   const int K = b->ntrials;
   L.ntgt = (int)H.tg.size();
-  L.nsrc = (int)(H.sr.size() / 4);
+  L.nsrc = (int)H.sr.size();
   int rc;
-  if ((rc = bmalloc(&L.dT, H.tg.size())) || (rc = bmalloc(&L.dS, (size_t)L.nsrc)) ||
-      (rc = bmalloc(&L.dsrc, H.sr.size())) || (rc = bmalloc(&L.dsrc_trial, H.strial.size())) ||
+  if ((rc = bmalloc(&L.dT, H.tg.size())) || (rc = bmalloc(&L.dZ, H.tg.size())) ||
+      (rc = bmalloc(&L.dtgt_trial, H.ttrial.size())) || (rc = bmalloc(&L.dS, H.sr.size())) ||
       (rc = bmalloc(&L.dtitem, (size_t)K + 1)) || (rc = bmalloc(&L.dttgt, (size_t)K + 1)) ||
       (rc = bmalloc(&L.dnn, H.tg.size())))
     return rc;
@@ -574,8 +610,8 @@ static int upload_level(dugma_batch *b, BLevel &L, const LevelHost &H) {
     const void *h;
     size_t n;
   } ups[] = {{L.dT, H.tg.data(), H.tg.size() * sizeof(Rec3I)},
-             {L.dsrc, H.sr.data(), H.sr.size() * sizeof(double)},
-             {L.dsrc_trial, H.strial.data(), H.strial.size() * sizeof(int)},
+             {L.dtgt_trial, H.ttrial.data(), H.ttrial.size() * sizeof(int)},
+             {L.dS, H.sr.data(), H.sr.size() * sizeof(BSrc)},
              {L.dttgt, H.ttgt.data(), H.ttgt.size() * sizeof(int)}};
   for (const Up &u : ups)
     if (u.n) DUGMA_HIP(hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice));
@@ -683,15 +719,20 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
           vt += tv[i];
         }
         h.tg.push_back(r);
+        h.ttrial.push_back(k);
       }
       L.nt[k] = (int)h.tg.size() - L.tb[k];
-      L.sb[k] = (int)(h.sr.size() / 4);
+      L.sb[k] = (int)h.sr.size();
       for (int j = 0; j < ns; j += step) {
-        for (int a = 0; a < 4; ++a) h.sr.push_back(sy[4 * (size_t)j + a]);
-        h.strial.push_back(k);
+        BSrc q;
+        q.x = (float)sy[4 * (size_t)j];
+        q.y = (float)sy[4 * (size_t)j + 1];
+        q.z = (float)sy[4 * (size_t)j + 2];
+        q.s = (float)(sy[4 * (size_t)j + 3] * kLn4);
+        h.sr.push_back(q);
         if (l == 0) vs += sy[4 * (size_t)j + 3];
       }
-      L.ns[k] = (int)(h.sr.size() / 4) - L.sb[k];
+      L.ns[k] = (int)h.sr.size() - L.sb[k];
       h.ttgt[k + 1] = h.ttgt[k] + L.nt[k];
       L.pairs += (long long)L.nt[k] * L.ns[k];
     }
@@ -771,11 +812,11 @@ extern "C" int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
 template <int W>
 static void launch_pairs(dugma_batch *b, const BLevel &L, int grid, const int *order, int ni, bool sigma) {
   if (sigma)
-    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dS, L.ditems,
-                       order, ni, b->dqueue, b->ddone, b->dtrials, L.dnn, L.dpart);
+    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ, L.dS,
+                       L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
   else
-    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dS, L.ditems,
-                       order, ni, b->dqueue, b->ddone, b->dtrials, L.dnn, L.dpart);
+    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ, L.dS,
+                       L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
 }
 
 // One iteration of trials [k0, k1) on level l.  grid: workgroups of the
@@ -783,7 +824,7 @@ static void launch_pairs(dugma_batch *b, const BLevel &L, int grid, const int *o
 static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams &bp,
                             bool one_trial, int grid) {
   const BLevel &L = b->lv[l];
-  int s0 = 0, s1 = L.nsrc;
+  int t0 = 0, t1 = L.ntgt;
   const int *order = L.dorder;
   int ni = L.nitems;
   if (one_trial) {
@@ -791,12 +832,12 @@ static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams
     const int i0 = L.titem[k0];
     ni = L.titem[k0 + 1] - i0;
     order = L.dident + i0;
-    s0 = L.sb[k0];
-    s1 = s0 + L.ns[k0];
+    t0 = L.tb[k0];
+    t1 = t0 + L.nt[k0];
   }
-  if (s1 > s0)
-    hipLaunchKernelGGL(k_batch_transform, dim3((s1 - s0 + 255) / 256), dim3(256), 0, b->stream, L.dsrc,
-                       L.dsrc_trial, s0, s1, b->dtrials, b->ddone, L.dS);
+  if (t1 > t0)
+    hipLaunchKernelGGL(k_batch_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT,
+                       L.dtgt_trial, t0, t1, b->dtrials, b->ddone, L.dZ);
   if (ni > 0) {
     if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
     grid = std::min(grid, ni);

@@ -70,6 +70,25 @@ def test_energy_config2_full_covariance(oracle):
         dg.fr_m(h)
 
 
+def test_energy_config2_full_size(oracle):
+    """The whole 6,000 x 6,000 structured-light workload (full covariances,
+    10% outliers): energy and pose gradient against the FP64 oracle at the
+    truth and at the bench's start pose (2 degrees, 3 mm off), to the bars
+    above; the 13-launch finite-difference call returns the same energy."""
+    d = W.config2(n=6000, m=6000)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        rng = np.random.default_rng(0)
+        poses = [(d["R_G"], np.ravel(d["t_G"])),
+                 (W.random_rotation(rng, 2.0) @ d["R_G"], np.ravel(d["t_G"]) + rng.uniform(-0.003, 0.003, size=3))]
+        for R, t in poses:
+            ref = check_energy(h, d, R, t, oracle)
+            E, _ = dg.energy_grad_fd(h, R, t, 1e-4)
+            assert abs(E / ref["E"] - 1) <= E_TOL
+    finally:
+        dg.fr_m(h)
+
+
 def test_energy_grad_points_config2(oracle):
     d = W.config2(n=500, m=400)
     R, t = perturbed_pose(d, deg=2.0, dt=0.002, seed=7)
