@@ -317,7 +317,9 @@ class Batch:
     """Many independent 3-D registrations in one device context
     (dugma_batch_create / dugma_batch_register).  ``trials`` is a list of
     dicts with X, X_C, Y, Y_C in the layout of ``init`` (isotropic
-    covariances)."""
+    covariances), or X, sX, Y, sY (one variance per point).  Trials that
+    also hold CX and CY (n x 3 x 3 full covariances per point) get them set
+    for the refinement phase (dugma_batch_set_covariances)."""
 
     def __init__(self, trials):
         K = len(trials)
@@ -344,6 +346,24 @@ class Batch:
         del keep  # the library holds its own copy
         self._ptr = out
         self.ntrials = K
+        if "CX" in trials[0]:
+            self.set_covariances([tr["CX"] for tr in trials], [tr["CY"] for tr in trials])
+
+    def set_covariances(self, CX, CY):
+        """Full per-point covariances (lists of (n, 3, 3) arrays) for the
+        refinement phase (dugma_batch_set_covariances)."""
+        K = self.ntrials
+        px = (ctypes.POINTER(ctypes.c_double) * K)()
+        py = (ctypes.POINTER(ctypes.c_double) * K)()
+        keep = []
+        for k in range(K):
+            # (n, 3, 3) -> one column-major 3 x 3 block per point
+            a = np.ascontiguousarray(np.transpose(np.asarray(CX[k], np.float64), (0, 2, 1)))
+            b = np.ascontiguousarray(np.transpose(np.asarray(CY[k], np.float64), (0, 2, 1)))
+            keep.append((a, b))
+            px[k], py[k] = _dp(a), _dp(b)
+        _lib.check(_lib.load().dugma_batch_set_covariances(self._ptr, px, py))
+        del keep
 
     def info(self):
         k, n, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
@@ -372,13 +392,13 @@ class Batch:
 
     def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
                  tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
-                 check_every=10, grid=0, sigma_update=True, relax=1.0, coarse_iters=0):
+                 check_every=10, grid=0, sigma_update=True, relax=1.0, coarse_iters=0, refine_iters=0):
         K = self.ntrials
         R0c, t0c = self._poses(R0, t0)
         o = _lib.BatchOpts(int(max_iter), int(bool(anneal)), float(anneal_decay), float(tol_rot),
                            float(tol_trans), int(bool(use_graph)), int(bool(one_trial_per_launch)),
                            int(check_every), int(grid), int(bool(sigma_update)), float(relax),
-                           int(coarse_iters))
+                           int(coarse_iters), int(refine_iters))
         R = np.zeros((K, 9))
         t = np.zeros((K, 3))
         iters = np.zeros(K, np.int32)

@@ -23,7 +23,7 @@ class BatchOpts(ctypes.Structure):
                 ("tol_trans", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int), ("relax", ctypes.c_double),
-                ("coarse_iters", ctypes.c_int)]
+                ("coarse_iters", ctypes.c_int), ("refine_iters", ctypes.c_int)]
 
 
 class RegisterOpts(ctypes.Structure):
@@ -62,6 +62,7 @@ SIGNATURES = {
     "dugma_ctx_plan": (_ci, [_vp, _ci, _i, _i]),
     "dugma_batch_create": (_ci, [_ci, _ci, _i, _i, _pp, _pp, _pp, _pp, ctypes.POINTER(_vp)]),
     "dugma_batch_create_iso": (_ci, [_ci, _i, _i, _pp, _pp, _pp, _pp, ctypes.POINTER(_vp)]),
+    "dugma_batch_set_covariances": (_ci, [_vp, _pp, _pp]),
     "dugma_batch_register": (_ci, [_vp, _d, _d, ctypes.POINTER(BatchOpts), _d, _d, _i, _i, _d]),
     "dugma_batch_info": (_ci, [_vp, _i, _i, ctypes.POINTER(ctypes.c_longlong)]),
     "dugma_batch_free": (_ci, [_vp]),

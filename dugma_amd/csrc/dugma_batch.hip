@@ -41,6 +41,21 @@
 // after one pass that measures sig again on them.  Each resolution has its
 // own work items and its own captured iteration.
 //
+// Refinement (dugma_batch_set_covariances + refine_iters): after the
+// isotropic iterations, each trial maximises the convolved energy with
+// full per-point covariances (for surfaces sampled independently, covariances
+// flat along the surface -- e.g. the sample covariance of a point's nearest
+// neighbours -- make the energy's maximum sit at the true pose) by BFGS on
+// -log E over left pose increments, one fused energy + pose-gradient pass per
+// trial pose (the logic of dugma_register_energy, one thread per trial):
+//   k_batchf_transform  targets and their covariances in each trial's source
+//                       frame at the candidate pose (FP64 -> FP32 hi/lo)
+//   k_batchf_pairs      the same persistent item queue; per target lane E,
+//                       EW and B (full covariance), then the pose gradient
+//                       by the target-side identity, in the source frame
+//   k_batchf_update     sums the items, rotates the gradient back, Armijo
+//                       test, BFGS update, next candidate pose
+//
 // Launch shape: waves per workgroup (target tile = 64 x waves), the number
 // of source chunks per target tile for each size class of trials (by source
 // count), and the resident waves of the persistent kernel are chosen by
@@ -454,6 +469,305 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
   trials[k] = tr;
 }
 
+// ---------------------------------------------------------------------------
+// Refinement: BFGS on -log E with full covariances, one pass per trial pose
+// ---------------------------------------------------------------------------
+struct alignas(16) BCov {  // a target's covariance (FP32, symmetric)
+  float s11, s12, s13, s22, s23, s33, pad0, pad1;
+};
+
+// Per-trial optimiser state (all FP64).  The pose parameters are
+// p = [w L, dt] for R <- exp([w]x) R, t <- t + dt (L: the trial's RMS source
+// radius, so that both blocks are lengths); f = -log E without (2 pi)^-3/2.
+struct BRef {
+  double R[9], t[3];    // accepted pose
+  double cR[9], ct[3];  // candidate pose: the next pass evaluates it
+  double gq[6], d[6];   // gradient of f at the accepted pose; search direction
+  double H[36];         // inverse Hessian approximation
+  double f, slope, alpha, L, w0, smax, E;
+  int state;            // 0: the start pose is being evaluated; 1: line search
+  int scaled;           // the first curvature pair has scaled H
+  int bt;               // halvings in the current line search
+  int it;               // accepted steps
+  int status;           // 0 running, 1 converged, 2 step limit, 3 no weight
+  int pad[3];
+};
+
+struct RParams {
+  double tol_rot, tol_trans;
+  int max_iter;
+  int pad;
+};
+
+// Candidate pose as every pass sees it: z = cR^T (y - (ct - c)) and
+// S' = cR^T S cR, FP64 -> FP32 (position hi + lo).
+__global__ __launch_bounds__(256) void k_batchf_transform(
+    const Rec3I *__restrict__ T, const BCov *__restrict__ TC, const int *__restrict__ tgt_trial, int i0,
+    int i1, const BTrial *__restrict__ trials, const BRef *__restrict__ ref, const int *__restrict__ done,
+    Rec3F *__restrict__ Z) {
+  const int i = i0 + blockIdx.x * 256 + threadIdx.x;
+  if (i >= i1) return;
+  const int k = tgt_trial[i];
+  if (done[k]) return;
+  const BRef &r = ref[k];
+  const double *c = trials[k].c;
+  const Rec3I y = T[i];
+  const double u[3] = {(double)y.hx - (r.ct[0] - c[0]), (double)y.hy - (r.ct[1] - c[1]),
+                       (double)y.hz - (r.ct[2] - c[2])};
+  double z[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) z[a] = r.cR[3 * a] * u[0] + r.cR[3 * a + 1] * u[1] + r.cR[3 * a + 2] * u[2];
+  const BCov cv = TC[i];
+  const double S[9] = {cv.s11, cv.s12, cv.s13, cv.s12, cv.s22, cv.s23, cv.s13, cv.s23, cv.s33};
+  double SR[9], Sp[9];  // S cR, then cR^T S cR (column-major)
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      SR[a + 3 * b] = S[a] * r.cR[3 * b] + S[a + 3] * r.cR[1 + 3 * b] + S[a + 6] * r.cR[2 + 3 * b];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      Sp[a + 3 * b] = r.cR[3 * a] * SR[3 * b] + r.cR[3 * a + 1] * SR[1 + 3 * b] + r.cR[3 * a + 2] * SR[2 + 3 * b];
+  Rec3F o;
+  o.hx = (float)z[0];
+  o.hy = (float)z[1];
+  o.hz = (float)z[2];
+  o.lx = (float)(z[0] - (double)o.hx);
+  o.ly = (float)(z[1] - (double)o.hy);
+  o.lz = (float)(z[2] - (double)o.hz);
+  o.s11 = (float)Sp[0];
+  o.s22 = (float)Sp[4];
+  o.s33 = (float)Sp[8];
+  o.s12 = (float)(0.5 * (Sp[3] + Sp[1]));
+  o.s13 = (float)(0.5 * (Sp[6] + Sp[2]));
+  o.s23 = (float)(0.5 * (Sp[7] + Sp[5]));
+  o.pad0 = o.pad1 = o.pad2 = o.pad3 = 0.0f;
+  Z[i] = o;
+}
+
+// Per item: the targets of a tile (lanes, hi + lo, in the source frame)
+// against a chunk of sources (static records, streamed two deep through
+// SGPRs): per target E, EW = sum e S^-1 d and B = sum e (S^-1 d d^T S^-1 -
+// S^-1), FP32 over sub-tiles of kBSub sources, then FP64; the pose
+// gradient follows from these target-side sums (pose_contrib: rotation
+// about the origin of the frame, i.e. about t, then translation).  Item
+// totals: [E, rotation (3), translation (3)], in the source frame.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_batchf_pairs(
+    const Rec3F *__restrict__ Z, const Rec3F *__restrict__ S, const BItem *__restrict__ items,
+    const int *__restrict__ order, int nitems, unsigned *__restrict__ queue, const int *__restrict__ done,
+    double *__restrict__ part) {
+  __shared__ int s_p;
+  __shared__ double red[W][7];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (;;) {
+    if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(s_p);
+    __syncthreads();
+    if (p >= nitems) break;
+    const int idx = __builtin_amdgcn_readfirstlane(order[p]);
+    const BItem it = items[idx];
+    if (done[it.trial]) continue;
+    const int ti = it.t0 + wave * 64 + lane;
+    const bool valid = (wave * 64 + lane) < it.tn;
+    const Rec3F pz = Z[valid ? ti : it.t0];
+    double dacc[10];
+#pragma unroll
+    for (int a = 0; a < 10; ++a) dacc[a] = 0.0;
+    const int s_end = it.s0 + it.sn;
+    Rec3F qa = S[it.s0], qb = S[min(it.s0 + 1, s_end - 1)];  // it.sn >= 1
+    for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
+      float acc[10];
+#pragma unroll
+      for (int a = 0; a < 10; ++a) acc[a] = 0.0f;
+      const int j1 = min(j0 + kBSub, s_end);
+      for (int j = j0; j < j1; ++j) {
+        const Rec3F q = qa;
+        qa = qb;
+        qb = S[min(j + 2, s_end - 1)];
+        pair_eval<1, 2>(pz, q, acc);
+      }
+#pragma unroll
+      for (int a = 0; a < 10; ++a) dacc[a] += (double)acc[a];
+    }
+    double r[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double zero[3] = {0.0, 0.0, 0.0};
+    if (valid) pose_contrib(pz, dacc, zero, r);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+      const double v = wave_sum(r[a]);
+      if (lane == 0) red[wave][a] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double v = red[0][threadIdx.x];
+#pragma unroll
+      for (int w = 1; w < W; ++w) v += red[w][threadIdx.x];
+      part[(size_t)idx * kStat + threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// R <- exp([w]x) R, t <- t + dt for p = [w L, dt] (Rodrigues, FP64).
+__device__ void ref_step(const double *R, const double *t, const double *s, double L, double *Rn,
+                         double *tn) {
+  const double w[3] = {s[0] / L, s[1] / L, s[2] / L};
+  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double A = th < 1e-12 ? 1.0 : sin(th) / th, B = th < 1e-12 ? 0.5 : (1.0 - cos(th)) / (th * th);
+  const double K[9] = {0, w[2], -w[1], -w[2], 0, w[0], w[1], -w[0], 0};  // [w]x, column-major
+  double E[9];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      double k2 = 0.0;
+      for (int b = 0; b < 3; ++b) k2 += K[a + 3 * b] * K[b + 3 * c];
+      E[a + 3 * c] = (a == c ? 1.0 : 0.0) + A * K[a + 3 * c] + B * k2;
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) Rn[a + 3 * c] = E[a] * R[3 * c] + E[a + 3] * R[1 + 3 * c] + E[a + 6] * R[2 + 3 * c];
+  for (int a = 0; a < 3; ++a) tn[a] = t[a] + s[3 + a];
+}
+
+// The next line search: d = -H g (the gradient if that is no descent
+// direction), first step w0 (a kernel width) until H is scaled, at most smax;
+// the candidate pose follows.  Returns false at a zero gradient.
+__device__ bool ref_direction(BRef &r) {
+  double slope = 0.0, dn = 0.0;
+  for (int a = 0; a < 6; ++a) {
+    double v = 0.0;
+    for (int b = 0; b < 6; ++b) v -= r.H[6 * a + b] * r.gq[b];
+    r.d[a] = v;
+    slope += r.gq[a] * v;
+  }
+  if (!(slope < 0.0)) {  // restart from the gradient
+    for (int a = 0; a < 36; ++a) r.H[a] = (a % 7 == 0) ? 1.0 : 0.0;
+    r.scaled = 0;
+    slope = 0.0;
+    for (int a = 0; a < 6; ++a) {
+      r.d[a] = -r.gq[a];
+      slope -= r.gq[a] * r.gq[a];
+    }
+    if (!(slope < 0.0)) return false;
+  }
+  for (int a = 0; a < 6; ++a) dn += r.d[a] * r.d[a];
+  dn = sqrt(dn);
+  double alpha = r.scaled ? 1.0 : r.w0 / dn;
+  if (alpha * dn > r.smax) alpha = r.smax / dn;
+  r.slope = slope;
+  r.alpha = alpha;
+  r.bt = 0;
+  double s[6];
+  for (int a = 0; a < 6; ++a) s[a] = alpha * r.d[a];
+  ref_step(r.R, r.t, s, r.L, r.cR, r.ct);
+  return true;
+}
+
+// One thread per trial: the pass's energy and gradient at the candidate
+// pose, then the optimiser's next move (dugma_register_energy's logic).
+__global__ __launch_bounds__(256) void k_batchf_update(
+    const double *__restrict__ part, const int *__restrict__ titem, int k0, int k1,
+    BRef *__restrict__ ref, int *__restrict__ done, unsigned *__restrict__ queue, RParams rp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+  const int k = k0 + blockIdx.x * 256 + threadIdx.x;
+  if (k >= k1 || done[k]) return;
+  BRef r = ref[k];
+  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = titem[k]; i < titem[k + 1]; ++i)
+    for (int a = 0; a < 7; ++a) st[a] += part[(size_t)i * kStat + a];
+  const double E = st[0];
+  // gradient (rotation, translation) back from the source frame: g = cR g'
+  double g[6];
+  for (int a = 0; a < 3; ++a) {
+    g[a] = r.cR[a] * st[1] + r.cR[a + 3] * st[2] + r.cR[a + 6] * st[3];
+    g[3 + a] = r.cR[a] * st[4] + r.cR[a + 3] * st[5] + r.cR[a + 6] * st[6];
+  }
+  const bool good = E > 0.0 && isfinite(E);
+  double gq[6];  // gradient of -log E in p
+  for (int a = 0; a < 6; ++a) gq[a] = a < 3 ? -g[a] / (E * r.L) : -g[a] / E;
+  bool finished = false;
+  if (r.state == 0) {
+    if (!good) {
+      r.status = 3;
+      finished = true;
+    } else {
+      for (int a = 0; a < 9; ++a) r.R[a] = r.cR[a];
+      for (int a = 0; a < 3; ++a) r.t[a] = r.ct[a];
+      r.f = -log(E);
+      r.E = E;
+      for (int a = 0; a < 6; ++a) r.gq[a] = gq[a];
+      for (int a = 0; a < 36; ++a) r.H[a] = (a % 7 == 0) ? 1.0 : 0.0;
+      r.scaled = 0;
+      r.state = 1;
+      if (rp.max_iter < 1 || !ref_direction(r)) {
+        r.status = rp.max_iter < 1 ? 2 : 1;
+        finished = true;
+      }
+    }
+  } else if (good && -log(E) <= r.f + 1e-4 * r.alpha * r.slope) {
+    // accepted: BFGS update with s = alpha d, y = gq_new - gq
+    r.it += 1;
+    double s[6], y[6], sy = 0.0, yy = 0.0, ss = 0.0;
+    for (int a = 0; a < 6; ++a) {
+      s[a] = r.alpha * r.d[a];
+      y[a] = gq[a] - r.gq[a];
+      sy += s[a] * y[a];
+      yy += y[a] * y[a];
+      ss += s[a] * s[a];
+    }
+    if (sy > 1e-12 * sqrt(ss * yy)) {
+      if (!r.scaled) {
+        for (int a = 0; a < 36; ++a) r.H[a] *= sy / yy;
+        r.scaled = 1;
+      }
+      const double rr = 1.0 / sy;
+      double Hy[6], yHy = 0.0;
+      for (int a = 0; a < 6; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < 6; ++b) v += r.H[6 * a + b] * y[b];
+        Hy[a] = v;
+        yHy += y[a] * v;
+      }
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b)
+          r.H[6 * a + b] += -rr * (Hy[a] * s[b] + s[a] * Hy[b]) + (rr * rr * yHy + rr) * s[a] * s[b];
+    }
+    for (int a = 0; a < 9; ++a) r.R[a] = r.cR[a];
+    for (int a = 0; a < 3; ++a) r.t[a] = r.ct[a];
+    r.f = -log(E);
+    r.E = E;
+    for (int a = 0; a < 6; ++a) r.gq[a] = gq[a];
+    const double ang = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) / r.L;
+    const double tr = sqrt(s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    if (ang < rp.tol_rot && tr < rp.tol_trans) {
+      r.status = 1;
+      finished = true;
+    } else if (r.it >= rp.max_iter) {
+      r.status = 2;
+      finished = true;
+    } else if (!ref_direction(r)) {
+      r.status = 1;
+      finished = true;
+    }
+  } else {
+    // no sufficient decrease: halve the step
+    r.alpha *= 0.5;
+    r.bt += 1;
+    if (r.bt >= 40) {
+      r.status = 1;  // stationary to working precision
+      finished = true;
+    } else {
+      double s[6];
+      for (int a = 0; a < 6; ++a) s[a] = r.alpha * r.d[a];
+      ref_step(r.R, r.t, s, r.L, r.cR, r.ct);
+    }
+  }
+  ref[k] = r;
+  if (finished) done[k] = 1;
+}
+
 // One resolution of every trial's clouds.  Level 0 holds all kept points;
 // level 1 every kCoarseStep-th target and source of each trial (same
 // centroid, same per-point variances), for the early, broad iterations.
@@ -490,6 +804,14 @@ struct dugma_batch {
   int *ddone = nullptr;
   unsigned *dqueue = nullptr;
   std::vector<double> a0, astop, cent, sbar;  // per trial
+  // refinement with full covariances (level-0 points)
+  BCov *dTC = nullptr;                      // per target: covariance
+  Rec3F *dSF = nullptr;                     // per source: position and covariance (static)
+  Rec3F *dZF = nullptr;                     // per target at each pass: in the trial's source frame
+  BRef *dref = nullptr;                     // per trial: optimiser state
+  std::vector<int> keepT, keepS;            // kept input indices per trial (level-0 order)
+  std::vector<BSrc> hsrc0;                  // host copy of the level-0 sources
+  std::vector<double> refL, refW0;          // per trial: RMS source radius, kernel width
   // launch shape
   int waves = 4;                            // waves per workgroup: target tile = 64 * waves
   int chunks[kClasses] = {1, 1, 2, 4};      // source chunks per target tile, per size class
@@ -507,7 +829,7 @@ static void free_batch(dugma_batch *b) {
   if (!b) return;
   free_level(b->lv[0]);
   free_level(b->lv[1]);
-  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue};
+  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue, b->dTC, b->dSF, b->dZF, b->dref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -686,6 +1008,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
         c[a] += (double)(float)p[a];
       }
       tv.push_back(v);
+      b->keepT.push_back(i);
     }
     for (int j = 0; j < M[k]; ++j) {
       const double *p = Y[k] + (size_t)j * 3;
@@ -697,6 +1020,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       if (v < 0 || !std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
       for (int a = 0; a < 3; ++a) sy.push_back((double)(float)p[a]);
       sy.push_back(v);
+      b->keepS.push_back(j);
     }
     const int nt = (int)tv.size(), ns = (int)(sy.size() / 4);
     if (nt > 0)
@@ -748,6 +1072,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) rc = hip_fail(e, "hipStreamCreateWithFlags");
   }
+  b->hsrc0 = H[0].sr;
   if (!rc) rc = upload_level(b, b->lv[0], H[0]);
   if (!rc) rc = upload_level(b, b->lv[1], H[1]);
   if (!rc) rc = bmalloc(&b->dtrials, (size_t)ntrials);
@@ -883,6 +1208,131 @@ static int start_trials(dugma_batch *b, const double *R0, const double *t0, bool
   return DUGMA_OK;
 }
 
+template <int W>
+static void launch_pairsf(dugma_batch *b, const BLevel &L, int grid, const int *order, int ni) {
+  hipLaunchKernelGGL((k_batchf_pairs<W>), dim3(grid), dim3(64 * W), 0, b->stream, b->dZF, b->dSF, L.ditems, order,
+                     ni, b->dqueue, b->ddone, L.dpart);
+}
+
+// One refinement pass of trials [k0, k1): transform at the candidate poses,
+// pairs, optimiser step.
+static int launch_refine(dugma_batch *b, int k0, int k1, const RParams &rp, bool one_trial, int grid) {
+  const BLevel &L = b->lv[0];
+  int t0 = 0, t1 = L.ntgt;
+  const int *order = L.dorder;
+  int ni = L.nitems;
+  if (one_trial) {
+    const int i0 = L.titem[k0];
+    ni = L.titem[k0 + 1] - i0;
+    order = L.dident + i0;
+    t0 = L.tb[k0];
+    t1 = t0 + L.nt[k0];
+  }
+  if (t1 > t0)
+    hipLaunchKernelGGL(k_batchf_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT, b->dTC,
+                       L.dtgt_trial, t0, t1, b->dtrials, b->dref, b->ddone, b->dZF);
+  if (ni > 0) {
+    if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
+    grid = std::min(grid, ni);
+    switch (b->waves) {
+      case 1: launch_pairsf<1>(b, L, grid, order, ni); break;
+      case 2: launch_pairsf<2>(b, L, grid, order, ni); break;
+      default: launch_pairsf<4>(b, L, grid, order, ni); break;
+    }
+  }
+  hipLaunchKernelGGL(k_batchf_update, dim3((k1 - k0 + 255) / 256), dim3(256), 0, b->stream, L.dpart, L.dtitem, k0,
+                     k1, b->dref, b->ddone, b->dqueue, rp);
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
+
+// The refinement after the isotropic iterations: every trial starts from the
+// pose those left (in dtrials), with its done flag cleared; at most
+// 1 + refine_iters line searches of passes (each line search needs one pass
+// when its first step is accepted).
+static int run_refine(dugma_batch *b, const dugma_batch_opts *opts) {
+  const int K = b->ntrials;
+  std::vector<BTrial> tr(K);
+  DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
+  DUGMA_HIP(hipStreamSynchronize(b->stream));
+  std::vector<BRef> ref(K);
+  for (int k = 0; k < K; ++k) {
+    BRef &r = ref[k];
+    memset(&r, 0, sizeof(BRef));
+    for (int a = 0; a < 9; ++a) r.cR[a] = r.R[a] = tr[k].R[a];
+    for (int a = 0; a < 3; ++a) r.ct[a] = r.t[a] = tr[k].t[a];
+    r.L = b->refL[k];
+    r.w0 = b->refW0[k];
+    r.smax = 0.25 * r.L;
+  }
+  DUGMA_HIP(hipMemcpyAsync(b->dref, ref.data(), sizeof(BRef) * K, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_batch_fill, dim3((K + 255) / 256), dim3(256), 0, b->stream, (unsigned *)b->ddone, K, 0u);
+  DUGMA_HIP(hipGetLastError());
+  RParams rp;
+  memset(&rp, 0, sizeof(rp));
+  rp.tol_rot = opts->tol_rot;
+  rp.tol_trans = opts->tol_trans;
+  rp.max_iter = opts->refine_iters;
+  // passes: each accepted step takes one pass at least; the halvings of a
+  // line search take more, so allow 4 passes per step on average
+  const int passes = 1 + 4 * opts->refine_iters;
+  const int grid = opts->grid;
+  int rc = DUGMA_OK;
+  if (opts->one_trial_per_launch) {
+    int *hdone = nullptr;
+    DUGMA_HIP(hipHostMalloc((void **)&hdone, sizeof(int), hipHostMallocDefault));
+    for (int k = 0; k < K && !rc; ++k)
+      for (int it = 0; it < passes && !rc; ++it) {
+        rc = launch_refine(b, k, k + 1, rp, true, grid);
+        if (rc) break;
+        hipError_t e = hipMemcpyAsync(hdone, b->ddone + k, sizeof(int), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "one-trial refinement");
+        if (*hdone) break;
+      }
+    (void)hipHostFree(hdone);
+    return rc;
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  if (opts->use_graph) {
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    rc = launch_refine(b, 0, K, rp, false, grid);
+    e = hipStreamEndCapture(b->stream, &g);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    if (!rc) {
+      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  const int check = opts->check_every > 0 ? opts->check_every : 10;
+  for (int it = 0; it < passes && !rc; ++it) {
+    if (ge) {
+      hipError_t e = hipGraphLaunch(ge, b->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphLaunch");
+    } else {
+      rc = launch_refine(b, 0, K, rp, false, grid);
+    }
+    if (rc) break;
+    if ((it + 1) % check == 0) {
+      std::vector<int> hd(K);
+      hipError_t e = hipMemcpyAsync(hd.data(), b->ddone, sizeof(int) * K, hipMemcpyDeviceToHost, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "refinement pass");
+        break;
+      }
+      int nd = 0;
+      for (int v : hd) nd += v;
+      if (nd == K) break;
+    }
+  }
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (g) (void)hipGraphDestroy(g);
+  return rc;
+}
+
 extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
                                     const double *t0, const dugma_batch_opts *opts,
                                     double *R, double *t, int *iters,
@@ -890,6 +1340,9 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   if (!b || !R0 || !t0 || !opts) return fail(DUGMA_ERR_ARG, "dugma_batch_register: bad argument");
   if (opts->max_iter < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_register: max_iter must be >= 1");
   if (opts->coarse_iters < 0) return fail(DUGMA_ERR_ARG, "dugma_batch_register: coarse_iters must be >= 0");
+  if (opts->refine_iters < 0) return fail(DUGMA_ERR_ARG, "dugma_batch_register: refine_iters must be >= 0");
+  if (opts->refine_iters > 0 && !b->dref)
+    return fail(DUGMA_ERR_STATE, "dugma_batch_register: refine_iters needs dugma_batch_set_covariances first");
   const int K = b->ntrials;
   const bool sigma = opts->sigma_update != 0;
   int rc = start_trials(b, R0, t0, opts->anneal != 0, sigma);
@@ -987,19 +1440,103 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     }
   }
   if (rc) return rc;
+  if (opts->refine_iters > 0) rc = run_refine(b, opts);
+  if (rc) return rc;
   std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
+  std::vector<BRef> ref(opts->refine_iters > 0 ? K : 0);
+  if (!ref.empty())
+    DUGMA_HIP(hipMemcpyAsync(ref.data(), b->dref, sizeof(BRef) * K, hipMemcpyDeviceToHost, b->stream));
   DUGMA_HIP(hipStreamSynchronize(b->stream));
   const double cn = 1.0 / (6.283185307179586476925 * std::sqrt(6.283185307179586476925));
   for (int k = 0; k < K; ++k) {
+    const bool refined = !ref.empty() && ref[k].state == 1;
+    const double *Rk = refined ? ref[k].R : tr[k].R, *tk = refined ? ref[k].t : tr[k].t;
     if (R)
-      for (int a = 0; a < 9; ++a) R[9 * (size_t)k + a] = tr[k].R[a];
+      for (int a = 0; a < 9; ++a) R[9 * (size_t)k + a] = Rk[a];
     if (t)
-      for (int a = 0; a < 3; ++a) t[3 * (size_t)k + a] = tr[k].t[a];
-    if (iters) iters[k] = tr[k].iters;
-    if (status) status[k] = tr[k].status;
-    if (E) E[k] = cn * tr[k].E;
+      for (int a = 0; a < 3; ++a) t[3 * (size_t)k + a] = tk[a];
+    if (iters) iters[k] = tr[k].iters + (refined ? ref[k].it : 0);
+    if (status) status[k] = ref.empty() ? tr[k].status : ref[k].status;
+    if (E) E[k] = refined ? cn * ref[k].E : cn * tr[k].E;
   }
+  return DUGMA_OK;
+}
+
+// Full covariances of every trial's points for the refinement (layout of
+// dugma_init: one column-major D x D block per point).  Only the points the
+// batch keeps are read; each must have a finite, positive-definite FP32
+// covariance.  Also sets each trial's length scales: L, the RMS radius of
+// its sources about their centroid, and w0, the per-axis kernel width
+// sqrt(2 x mean source variance).
+extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *X_C, const double *const *Y_C) {
+  if (!b || !X_C || !Y_C) return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: null argument");
+  const int K = b->ntrials;
+  const BLevel &L = b->lv[0];
+  std::vector<BCov> tc((size_t)L.ntgt);
+  std::vector<Rec3F> sf((size_t)L.nsrc);
+  std::vector<double> refL(K), refW0(K);
+  auto sym32 = [](const double *C, float *o) -> bool {
+    // o = [s11 s12 s13 s22 s23 s33] of the symmetric part, rounded to FP32
+    const float v[6] = {(float)C[0], (float)(0.5 * (C[3] + C[1])), (float)(0.5 * (C[6] + C[2])), (float)C[4],
+                        (float)(0.5 * (C[7] + C[5])), (float)C[8]};
+    for (int a = 0; a < 6; ++a) {
+      if (!std::isfinite(v[a])) return false;
+      o[a] = v[a];
+    }
+    const double a11 = v[0], a12 = v[1], a13 = v[2], a22 = v[3], a23 = v[4], a33 = v[5];
+    const double m2 = a11 * a22 - a12 * a12;
+    const double m3 = a11 * (a22 * a33 - a23 * a23) - a12 * (a12 * a33 - a23 * a13) + a13 * (a12 * a23 - a22 * a13);
+    return a11 > 0 && m2 > 0 && m3 > 0;
+  };
+  for (int k = 0; k < K; ++k) {
+    if (!X_C[k] || !Y_C[k]) return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: null covariance array");
+    for (int q = 0; q < L.nt[k]; ++q) {
+      const int i = b->keepT[L.tb[k] + q];
+      float o[6];
+      if (!sym32(X_C[k] + (size_t)i * 9, o))
+        return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: a target covariance is not positive definite");
+      BCov &c = tc[L.tb[k] + q];
+      c.s11 = o[0]; c.s12 = o[1]; c.s13 = o[2]; c.s22 = o[3]; c.s23 = o[4]; c.s33 = o[5];
+      c.pad0 = c.pad1 = 0.0f;
+    }
+    double cen[3] = {0, 0, 0}, var = 0.0, r2 = 0.0;
+    const int ns = L.ns[k];
+    for (int q = 0; q < ns; ++q) {
+      const int j = b->keepS[L.sb[k] + q];
+      float o[6];
+      if (!sym32(Y_C[k] + (size_t)j * 9, o))
+        return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: a source covariance is not positive definite");
+      const BSrc &x = b->hsrc0[L.sb[k] + q];
+      Rec3F &r = sf[L.sb[k] + q];
+      memset(&r, 0, sizeof(r));
+      r.hx = x.x; r.hy = x.y; r.hz = x.z;
+      r.s11 = o[0]; r.s12 = o[1]; r.s13 = o[2]; r.s22 = o[3]; r.s23 = o[4]; r.s33 = o[5];
+      cen[0] += x.x; cen[1] += x.y; cen[2] += x.z;
+      var += ((double)o[0] + o[3] + o[5]) / 3.0;
+    }
+    if (ns > 0) {
+      for (int a = 0; a < 3; ++a) cen[a] /= ns;
+      for (int q = 0; q < ns; ++q) {
+        const BSrc &x = b->hsrc0[L.sb[k] + q];
+        r2 += (x.x - cen[0]) * (x.x - cen[0]) + (x.y - cen[1]) * (x.y - cen[1]) + (x.z - cen[2]) * (x.z - cen[2]);
+      }
+      var /= ns;
+      r2 /= ns;
+    }
+    refL[k] = r2 > 0 ? std::sqrt(r2) : 1.0;
+    refW0[k] = var > 0 ? std::sqrt(2.0 * var) : 1e-3 * refL[k];
+  }
+  if (!b->dTC) {
+    int rc;
+    if ((rc = bmalloc(&b->dTC, tc.size())) || (rc = bmalloc(&b->dSF, sf.size())) ||
+        (rc = bmalloc(&b->dZF, tc.size())) || (rc = bmalloc(&b->dref, (size_t)K)))
+      return rc;
+  }
+  if (!tc.empty()) DUGMA_HIP(hipMemcpy(b->dTC, tc.data(), tc.size() * sizeof(BCov), hipMemcpyHostToDevice));
+  if (!sf.empty()) DUGMA_HIP(hipMemcpy(b->dSF, sf.data(), sf.size() * sizeof(Rec3F), hipMemcpyHostToDevice));
+  b->refL = refL;
+  b->refW0 = refW0;
   return DUGMA_OK;
 }
 

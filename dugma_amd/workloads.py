@@ -122,6 +122,31 @@ def knn_sigma(P, k=8, workers=-1):
     return d[:, 1:].mean(axis=1)
 
 
+# Surface covariances of the batched trials' refinement: half the sample
+# covariance of a point's 8 nearest neighbours (flat along the surface: its
+# spread is the sampling spacing in the tangent plane and the noise and
+# curvature along the normal) plus 0.002 times its 8-NN sigma^2 as an
+# isotropic floor.  With these, the energy of two independently sampled
+# clouds peaks at the true pose (FP64 trials of 500-2,000 points: 32 of 32
+# within 2 degrees from the truth and from 20 degrees off, against 0 of 32
+# with sigma^2 I).
+KNN_COV_SCALE = 0.5
+KNN_COV_FLOOR = 0.002
+
+
+def knn_sigma_cov(P, k=8, scale=KNN_COV_SCALE, floor=KNN_COV_FLOOR, workers=-1):
+    """(sigma, C): the mean distance to the k nearest neighbours (self
+    excluded) and the covariance scale * (their sample covariance) +
+    floor * sigma^2 I, (n, 3, 3)."""
+    from scipy.spatial import cKDTree
+    d, idx = cKDTree(P).query(P, k=k + 1, workers=workers)
+    sig = d[:, 1:].mean(axis=1)
+    nb = P[idx[:, 1:]]
+    c = nb - nb.mean(axis=1, keepdims=True)
+    C = np.einsum("nka,nkb->nab", c, c) / k
+    return sig, scale * C + (floor * sig ** 2)[:, None, None] * np.eye(3)
+
+
 def iso_blocks(sig2):
     """D x (D*n) covariance blocks s_k I for a vector of variances (3-D)."""
     n = len(sig2)
@@ -236,19 +261,21 @@ def config3(trial=0, n=200000, m=200000):
                 R_G=R, t_G=t)
 
 
-def _c4_chunk(ks):
-    return [config4_trial(k, workers=1) for k in ks]
+def _c4_chunk(args):
+    ks, cov = args
+    return [config4_trial(k, workers=1, covariances=cov) for k in ks]
 
 
-def config4(ntrials=4096, procs=None):
+def config4(ntrials=4096, procs=None, covariances=False):
     """All batched trials, one seed (and one surface) per trial; large sets
-    are generated in worker processes (at most 16)."""
+    are generated in worker processes (at most 16).  covariances: also the
+    surface covariances CX, CY of the refinement (knn_sigma_cov)."""
     import os
     procs = procs if procs is not None else min(16, os.cpu_count() or 1)
     if ntrials < 256 or procs <= 1:
-        return [config4_trial(k) for k in range(ntrials)]
+        return [config4_trial(k, covariances=covariances) for k in range(ntrials)]
     import multiprocessing as mp
-    chunks = [list(range(a, min(a + 64, ntrials))) for a in range(0, ntrials, 64)]
+    chunks = [(list(range(a, min(a + 64, ntrials))), covariances) for a in range(0, ntrials, 64)]
     with mp.get_context("fork").Pool(procs) as pool:
         out = []
         for part in pool.imap(_c4_chunk, chunks):
@@ -256,7 +283,7 @@ def config4(ntrials=4096, procs=None):
     return out
 
 
-def config4_trial(trial, workers=-1):
+def config4_trial(trial, workers=-1, covariances=False):
     """One batched trial from its own seed: its own surface, N, M
     log-uniform in [500, 8000], angle U(0, 90) deg, noise sigma
     U(0.001, 0.02), outlier fraction U(0, 0.3)."""
@@ -276,9 +303,14 @@ def config4_trial(trial, workers=-1):
         lo, hi = tgt.min(axis=0), tgt.max(axis=0)
         idx = rng.choice(n, size=k, replace=False)
         tgt[idx] = rng.uniform(lo, hi, size=(k, 3))
-    sx, sy = knn_sigma(tgt, workers=workers), knn_sigma(src, workers=workers)
     # isotropic variances per point (Batch takes them directly; iso_blocks
     # turns them into the D x (D*n) layout of init)
+    if covariances:
+        sx, CX = knn_sigma_cov(tgt, workers=workers)
+        sy, CY = knn_sigma_cov(src, workers=workers)
+        return dict(X=tgt.T.copy(), sX=sx ** 2, Y=src.T.copy(), sY=sy ** 2, CX=CX, CY=CY,
+                    R_G=R, t_G=t, noise=noise, outliers=outl, n=n, m=m)
+    sx, sy = knn_sigma(tgt, workers=workers), knn_sigma(src, workers=workers)
     return dict(X=tgt.T.copy(), sX=sx ** 2, Y=src.T.copy(), sY=sy ** 2,
                 R_G=R, t_G=t, noise=noise, outliers=outl, n=n, m=m)
 

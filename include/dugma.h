@@ -273,6 +273,10 @@ typedef struct {
                                 (rotation angle and centroid move times relax) */
   int coarse_iters;          /* first iterations on every 4th target and source point
                                 (no convergence test there); then the full clouds */
+  int refine_iters;          /* > 0: then at most this many accepted BFGS steps per trial
+                                on -log E with the full covariances of
+                                dugma_batch_set_covariances (one fused energy + pose-gradient
+                                pass per trial pose; stops at tol_rot / tol_trans) */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */
@@ -286,9 +290,20 @@ int dugma_batch_create_iso(int ntrials, const int *N, const int *M,
                            const double *const *X, const double *const *sX,
                            const double *const *Y, const double *const *sY,
                            dugma_batch **out);
+/* Full per-point covariances for the refinement phase (refine_iters), in
+ * the layout of dugma_init (X_C[k]: 3 x (3 N[k]) column-major blocks); only
+ * the points the batch keeps are read, and each of those must be finite and
+ * positive definite in FP32.  For independently sampled surfaces, covariances
+ * flat along the surface (e.g. the sample covariance of each point's nearest
+ * neighbours plus a small isotropic floor) make the energy peak at the true
+ * pose where isotropic ones do not.  No reference counterpart (the
+ * reference's covariances are the sensor's, DUGMA.m:44). */
+int dugma_batch_set_covariances(dugma_batch *b, const double *const *X_C,
+                                const double *const *Y_C);
 /* R0 / R: ntrials x 9 column-major, t0 / t: ntrials x 3; iters, status
  * (1 converged, 2 iteration limit, 3 no weight), E: ntrials each (any output
- * may be NULL). */
+ * may be NULL).  With refine_iters > 0, R, t, status and E are those of the
+ * refinement and iters counts both phases' steps. */
 int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
                          const dugma_batch_opts *opts, double *R, double *t,
                          int *iters, int *status, double *E);

@@ -299,3 +299,99 @@ def test_config4_small_trials_stationary(c4_trials, oracle):
               % (C4_SMALL[k], tr["n"], tr["m"], tr["outliers"], W.rotation_error_deg(R, tr["R_G"]), out["iters"][k]))
         assert np.linalg.norm(g[:3]) < 1e-2 * np.linalg.norm(g_rot[:3]), (C4_SMALL[k], g, g_rot)
         assert np.linalg.norm(g[3:]) < 1e-2 * np.linalg.norm(g_tr[3:]), (C4_SMALL[k], g, g_tr)
+
+
+# ---- refinement with surface covariances --------------------------------
+C4_REFINE = (0, 68, 80, 81, 110, 111, 120, 181, 207, 210, 215, 233, 258, 280, 353, 401, 480, 509)
+
+
+@pytest.fixture(scope="module")
+def c4_cov_trials():
+    return {k: W.config4_trial(k, covariances=True) for k in C4_REFINE}
+
+
+def _oracle_arrays(tr):
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    sym = lambda C: f32(0.5 * (C + np.transpose(C, (0, 2, 1))))  # noqa: E731
+    return f32(tr["X"].T), sym(tr["CX"]), f32(tr["Y"].T), sym(tr["CY"])
+
+
+def test_refine_energy_matches_oracle(c4_cov_trials, oracle):
+    """The refinement's energy at the pose it returns (full covariances, one
+    fused pass per trial pose) against the FP64 oracle's at that pose."""
+    trials = [c4_cov_trials[k] for k in C4_REFINE[:8]]
+    b = dg.Batch(trials)
+    try:
+        out = b.register(max_iter=60, anneal=True, sigma_update=True, refine_iters=3)
+    finally:
+        b.free()
+    for k, tr in enumerate(trials):
+        Y, SY, X, SX = _oracle_arrays(tr)
+        ref = oracle.energy(Y, SY, X, SX, out["R"][k], out["t"][k])["E"]
+        assert abs(out["E"][k] / ref - 1) <= 1e-5, (C4_REFINE[k], out["E"][k], ref)
+
+
+def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials):
+    trials = [c4_cov_trials[k] for k in C4_REFINE]
+    b = dg.Batch(trials)
+    try:
+        kw = dict(max_iter=40, anneal=True, sigma_update=True, coarse_iters=10, refine_iters=20,
+                  tol_rot=1e-4, tol_trans=1e-4)
+        g = b.register(use_graph=True, check_every=7, **kw)
+        p = b.register(use_graph=False, **kw)
+        o = b.register(one_trial_per_launch=True, **kw)
+    finally:
+        b.free()
+    for a in (p, o):
+        np.testing.assert_array_equal(g["R"], a["R"])
+        np.testing.assert_array_equal(g["t"], a["t"])
+        np.testing.assert_array_equal(g["iters"], a["iters"])
+        np.testing.assert_array_equal(g["status"], a["status"])
+
+
+def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle):
+    """Bench settings then the refinement: every trial ends where the FP64
+    oracle's pose gradient (surface covariances) vanishes next to its size
+    one degree / 0.01 units away, and within 2 degrees of the ground truth
+    in at least 80% of these trials (500-8,000 points, up to 30% outliers,
+    0-90 degree starts)."""
+    trials = [c4_cov_trials[k] for k in C4_REFINE]
+    b = dg.Batch(trials)
+    try:
+        out = b.register(max_iter=460, anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9,
+                         coarse_iters=160, tol_rot=1e-5, tol_trans=1e-5, refine_iters=200)
+    finally:
+        b.free()
+    errs = []
+    for k, tr in enumerate(trials):
+        R, t = out["R"][k], out["t"][k]
+        e = W.rotation_error_deg(R, tr["R_G"])
+        errs.append(e)
+        print("config 4 trial %d (%d x %d, outliers %.2f): refined rotation error %.3f deg, status %d, %d steps"
+              % (C4_REFINE[k], tr["n"], tr["m"], tr["outliers"], e, out["status"][k], out["iters"][k]))
+        assert out["status"][k] in (1, 2)
+        Y, SY, X, SX = _oracle_arrays(tr)
+        g = oracle.energy(Y, SY, X, SX, R, t, grads=True)["gpose"]
+        g_rot = oracle.energy(Y, SY, X, SX, _rot_about([1, 2, 3], 1.0) @ R, t, grads=True)["gpose"]
+        g_tr = oracle.energy(Y, SY, X, SX, R, t + np.array([0.01, -0.01, 0.01]), grads=True)["gpose"]
+        assert np.linalg.norm(g[:3]) < 1e-2 * np.linalg.norm(g_rot[:3]), (C4_REFINE[k], g, g_rot)
+        assert np.linalg.norm(g[3:]) < 1e-2 * np.linalg.norm(g_tr[3:]), (C4_REFINE[k], g, g_tr)
+    assert np.mean(np.array(errs) < 2.0) >= 0.8, errs
+
+
+def test_refine_argument_errors():
+    tr = small_trial(50, 300, 300)
+    b = dg.Batch([tr])
+    try:
+        with pytest.raises(dg.DugmaError, match="set_covariances"):
+            b.register(max_iter=3, refine_iters=2)
+        C = np.tile(np.eye(3), (300, 1, 1)) * 1e-3
+        bad = C.copy()
+        bad[7] = -bad[7]
+        with pytest.raises(dg.DugmaError, match="positive definite"):
+            b.set_covariances([bad], [C])
+        b.set_covariances([C], [C])
+        with pytest.raises(dg.DugmaError, match="refine_iters"):
+            b.register(max_iter=3, refine_iters=-1)
+    finally:
+        b.free()
