@@ -340,9 +340,13 @@ def bench_contours_2d(hip, dg, W, steps=200):
 # under the stopping rules of the reference's own driver
 # (main_kinect_real_application.m:14-17: at most 300 iterations, rotation
 # step 1e-4 rad, translation step 1e-4).
+# Then the refinement: at most BATCH_REFINE_ITERS BFGS steps per trial on
+# the energy with surface covariances (half the 8-NN sample covariance plus
+# 0.002 sigma^2 I), one fused energy + pose-gradient pass per trial pose.
 BATCH_COARSE_ITERS = 160
+BATCH_REFINE_ITERS = 100
 BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=1e-4, tol_trans=1e-4, relax=1.9,
-                  coarse_iters=BATCH_COARSE_ITERS)
+                  coarse_iters=BATCH_COARSE_ITERS, refine_iters=BATCH_REFINE_ITERS)
 BATCH_MAX_ITER = 300 + BATCH_COARSE_ITERS
 
 
@@ -351,7 +355,7 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
     iteration over a (trial, tile) queue, iterations replayed from a hipGraph;
     against the same kernels one trial per launch on a subset."""
     t0 = time.time()
-    trials = W.config4(ntrials=ntrials)
+    trials = W.config4(ntrials=ntrials, covariances=True)
     log("generated %d trials in %.1f s" % (ntrials, time.time() - t0))
     b = dg.Batch(trials)
     try:
@@ -370,10 +374,23 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
         t_G = np.array([tr["t_G"] for tr in trials])
         outg = b.register(R_G, t_G, max_iter=max_iter - BATCH_COARSE_ITERS, use_graph=True,
                           **{**BATCH_MODE, "anneal": False, "coarse_iters": 0})
+        # the isotropic iterations alone (no refinement), for comparison
+        t3 = time.perf_counter()
+        outi = b.register(max_iter=max_iter, use_graph=True, check_every=10, **{**BATCH_MODE, "refine_iters": 0})
+        wall_iso = time.perf_counter() - t3
     finally:
         b.free()
     errs = np.array([W.rotation_error_deg(out["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
     errs_g = np.array([W.rotation_error_deg(outg["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
+    errs_i = np.array([W.rotation_error_deg(outi["R"][k], tr["R_G"]) for k, tr in enumerate(trials)])
+    mn = np.array([min(tr["n"], tr["m"]) for tr in trials])
+    bands = {}
+    for lo, hi in ((500, 1000), (1000, 2000), (2000, 4000), (4000, 8001)):
+        sel = (mn >= lo) & (mn < hi)
+        if sel.any():
+            bands["%d-%d" % (lo, min(hi, 8000))] = {"trials": int(sel.sum()),
+                                                    "success": round(float((errs[sel] < 2.0).mean()), 4),
+                                                    "success_from_ground_truth": round(float((errs_g[sel] < 2.0).mean()), 4)}
     sub = trials[:baseline_trials]
     b1 = dg.Batch(sub)
     try:
@@ -390,7 +407,7 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
     log("batched: %d trials in %.3f s (%.1f reg/s), one-trial-per-launch %.1f reg/s, success %.3f, shape %s"
         % (ntrials, wall, rps, rps1, float((errs < 2.0).mean()), shape))
     return {"workload": "4,096 trials, N and M log-uniform in [500, 8000], rotation U(0,90) deg, "
-                        "noise U(0.001,0.02), outliers U(0,0.3), isotropic 8-NN sigma",
+                        "noise U(0.001,0.02), outliers U(0,0.3); isotropic 8-NN sigma, then surface covariances",
             "value": round(rps, 2), "unit": "registrations/s", "wall_s": round(wall, 4),
             "iteration": "majorise-minimise pose step; DUGMA sigma update%s; steps over-relaxed by %g after "
                          "annealing; first %d iterations on every 4th point; stop at rotation step < %g rad "
@@ -406,6 +423,13 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
             "rotation_error_below_5deg": round(float((errs < 5.0).mean()), 4),
             "rotation_error_median_deg": round(float(np.median(errs)), 4),
             "success_rate_from_ground_truth": round(float((errs_g < 2.0).mean()), 4),
+            "success_by_min_points": bands,
+            "refine": "at most %d BFGS steps per trial on the energy with surface covariances (half the 8-NN "
+                      "sample covariance + 0.002 sigma^2 I), one fused energy + pose-gradient pass per trial pose"
+                      % BATCH_MODE["refine_iters"],
+            "without_refinement": {"value": round(ntrials / wall_iso, 2), "unit": "registrations/s",
+                                   "success_rate": round(float((errs_i < 2.0).mean()), 4),
+                                   "rotation_error_below_5deg": round(float((errs_i < 5.0).mean()), 4)},
             "one_trial_per_launch": {"value": round(rps1, 2), "trials": baseline_trials,
                                      "wall_s": round(wall1, 4), "same_poses": same},
             "speedup": round(rps / rps1, 2)}
@@ -520,6 +544,54 @@ def bench_iteration(hip, dg, d, steps=5):
             "ms_per_iteration": round(it_ms, 3), "sigma_update_ms": round(sig_ms, 3), "isotropic": bool(iso),
             "unchanged_points": int(un.value), "value": round(nT * nS / (it_ms / 1e3) / 1e9, 3),
             "unit": "Gpair-evals/s (fused pairs per iteration time)"}
+
+
+def bench_iteration_surface(hip, dg, d, steps=3):
+    """The same clouds with surface covariances: dugma_update_cov_knn on both
+    clouds (8-NN sample covariance x 0.5 + 0.002 sigma^2 I, on the device),
+    then the fused step with full covariances (107 FLOP per pair)."""
+    from dugma_amd import _lib
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        R, t = start_pose(d)
+        un = ctypes.c_int(0)
+        cfn = _lib.load().dugma_update_cov_knn
+        cargs = (h.ptr, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8, ctypes.c_double(0.5), ctypes.c_double(0.002),
+                 None, None, ctypes.byref(un))
+        bad = [0]
+
+        def cov():
+            if cfn(*cargs):
+                bad[0] += 1
+
+        cov()
+        if TUNE:
+            h.autotune()
+        step = CStep(dg, h, R, t)
+        step()
+        cov_ms = hip.timed(h.stream(), cov, steps) / steps
+        step.arm()
+        ms = hip.timed(h.stream(), step, steps) / steps
+        step.check()
+        if bad[0]:
+            raise RuntimeError("%d covariance updates failed: %s" % (bad[0], _lib.load().dugma_last_error().decode()))
+        h.set_timing(True)
+        dg.energy_grad(h, R, t)
+        kms, launches = h.pair_time()
+        h.set_timing(False)
+        nT, nS, iso = h.energy_info()
+    finally:
+        dg.fr_m(h)
+    kms /= max(launches, 1)
+    ach = nT * nS * FLOP_3D_FULL_FUSED / (kms / 1e3) / 1e12
+    log("surface iteration 200Kx200K: covariance update %.3f ms, full-covariance fused step %.3f ms (kernel %.3f, "
+        "%.1f%% of peak)" % (cov_ms, ms, kms, 100 * ach / PEAK_FP32_TFLOPS))
+    return {"what": "surface covariances of both clouds from their 8 nearest neighbours on the device "
+                    "(0.5 x sample covariance + 0.002 sigma^2 I), then the fused energy + pose adjoint with full "
+                    "covariances", "covariance_update_ms": round(cov_ms, 3), "ms_per_step": round(ms, 3),
+            "kernel_ms": round(kms, 3), "isotropic": bool(iso), "value": round(nT * nS / (ms / 1e3) / 1e9, 3),
+            "unit": "Gpair-evals/s", "achieved_tflops": round(ach, 3), "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+            "flop_per_pair": FLOP_3D_FULL_FUSED, "unchanged_points": int(un.value)}
 
 
 def cpu_baseline(d3, d2, n200=1024, n_compi=300, reps=5):
@@ -637,7 +709,8 @@ def main():
     log("generating 200,000 x 200,000 workload")
     d3 = W.config3()
     if only == "iteration":
-        print(json.dumps({"only": only, **bench_iteration(hip, dg, d3)}), flush=True)
+        print(json.dumps({"only": only, **bench_iteration(hip, dg, d3),
+                          "surface": bench_iteration_surface(hip, dg, d3)}), flush=True)
         return
     fused = bench_fused(hip, dg, d3, args.steps, args.warmup, "fused 200Kx200K iso")
     value = fused["pairs"] / (fused["ms_per_step"] / 1e3) / 1e9
@@ -674,6 +747,7 @@ def main():
     if not args.no_large:
         out["large"] = {"plain_fd": bench_plain_large(hip, dg, d3, fused), "knn": bench_knn(dg, d3),
                         "iteration": bench_iteration(hip, dg, d3),
+                        "iteration_surface": bench_iteration_surface(hip, dg, d3),
                         "register_energy": bench_register_energy(dg, d3)}
         if only == "large":
             print(json.dumps({"only": only, **out["large"]}), flush=True)

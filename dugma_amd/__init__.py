@@ -309,6 +309,21 @@ def update_sigma_knn(h, which=SIGMA_TARGET | SIGMA_SOURCE, k=8, want_sigma=True)
     return sT, sS, un.value
 
 
+def update_cov_knn(h, which=SIGMA_TARGET | SIGMA_SOURCE, k=8, scale=0.5, floor=0.002, want_sigma=True):
+    """Per-point surface covariances scale * (sample covariance of the k
+    nearest neighbours) + floor * sigma_i^2 I within each selected cloud, on
+    the device (see include/dugma.h).  Returns (sigma_target, sigma_source,
+    unchanged) as update_sigma_knn."""
+    fp = ctypes.POINTER(ctypes.c_float)
+    sT = np.empty(h.N, np.float32) if (want_sigma and which & SIGMA_TARGET) else None
+    sS = np.empty(h.M, np.float32) if (want_sigma and which & SIGMA_SOURCE) else None
+    un = ctypes.c_int(0)
+    _lib.check(_lib.load().dugma_update_cov_knn(
+        h.ptr, int(which), int(k), float(scale), float(floor), None if sT is None else sT.ctypes.data_as(fp),
+        None if sS is None else sS.ctypes.data_as(fp), ctypes.byref(un)))
+    return sT, sS, un.value
+
+
 def version():
     return _lib.load().dugma_version().decode()
 

@@ -58,6 +58,7 @@ SIGNATURES = {
     "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),
     "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),
     "dugma_update_sigma_knn": (_ci, [_vp, _ci, _ci, _f, _f, _i]),
+    "dugma_update_cov_knn": (_ci, [_vp, _ci, _ci, ctypes.c_double, ctypes.c_double, _f, _f, _i]),
     "dugma_autotune": (_ci, [_vp, _ci]),
     "dugma_ctx_plan": (_ci, [_vp, _ci, _i, _i]),
     "dugma_batch_create": (_ci, [_ci, _ci, _i, _i, _pp, _pp, _pp, _pp, ctypes.POINTER(_vp)]),

@@ -996,6 +996,91 @@ __global__ __launch_bounds__(kSBlock) void k_sigma_apply(
   }
 }
 
+// Surface covariance from the same lists (dugma_update_cov_knn): sigma as
+// above, then, nearest first in FP32, mu = (p_1 + ... + p_k) / k,
+// c_ab = ((p_1a - mu_a)(p_1b - mu_b) + ...) / k and
+// C_ab = scale c_ab + (a == b ? floor (sigma sigma) : 0).  A point with
+// fewer than k other points, or whose C fails the energy's validity rule
+// (positive leading principal minors, in FP64 of the FP32 values), keeps
+// its covariance and is counted.  Otherwise C is written everywhere the
+// context holds the point's covariance, as dugma_init would have stored it:
+// init's inverse and g, and the full-covariance energy state (scaled by
+// sc2 in FP64, rounded once).
+template <int D, bool TARGET>
+__global__ __launch_bounds__(kSBlock) void k_cov_apply(
+    const float *__restrict__ lists, const int *__restrict__ li, const float *__restrict__ P, int n, int k,
+    float scale, float floor, const int *__restrict__ idx, double sc2, float *__restrict__ Cinv,
+    float *__restrict__ g, void *__restrict__ recF, double *__restrict__ src, float *__restrict__ sig_out,
+    int *__restrict__ unchanged) {
+  const int q = blockIdx.x * kSBlock + threadIdx.x;
+  if (q >= n) return;
+  const float *d2 = lists + (size_t)q * k;
+  const int *nb = li + (size_t)q * k;
+  float s = sqrtf(d2[0]);
+  for (int r = 1; r < k; ++r) s = s + sqrtf(d2[r]);
+  const float sigma = s / (float)k;
+  sig_out[q] = sigma;
+  if (nb[k - 1] < 0) {
+    atomicAdd(unchanged, 1);
+    return;
+  }
+  float mu[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float m = P[(size_t)nb[0] * D + a];
+    for (int r = 1; r < k; ++r) m = m + P[(size_t)nb[r] * D + a];
+    mu[a] = m / (float)k;
+  }
+  const float fl = floor * (sigma * sigma);
+  float C[D * D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      float c = 0.0f;
+      for (int r = 0; r < k; ++r) {
+        const float u = P[(size_t)nb[r] * D + a] - mu[a], v = P[(size_t)nb[r] * D + b] - mu[b];
+        c = c + u * v;
+      }
+      float cab = scale * (c / (float)k);
+      if (a == b) cab = cab + fl;
+      C[a + D * b] = cab;
+    }
+  // the energy's validity rule (FP64 of the FP32 values)
+  bool ok = true;
+#pragma unroll
+  for (int e = 0; e < D * D; ++e) ok = ok && isfinite(C[e]);
+  const double S0 = C[0], m2 = (double)C[0] * C[D + 1] - (double)C[1] * C[D];
+  ok = ok && S0 > 0 && m2 > 0;
+  if (D == 3) {
+    const double det = (double)C[0] * ((double)C[4] * C[8] - (double)C[7] * C[5]) -
+                       (double)C[3] * ((double)C[1] * C[8] - (double)C[7] * C[2]) +
+                       (double)C[6] * ((double)C[1] * C[5] - (double)C[4] * C[2]);
+    ok = ok && det > 0;
+  }
+  if (!ok) {
+    atomicAdd(unchanged, 1);
+    return;
+  }
+  const size_t i = (size_t)idx[q];
+  compat_inverse<D>(C, Cinv + i * D * D, g + i);
+  auto sv = [&](int a, int b) { return (float)((double)C[a + D * b] * sc2); };
+  if (TARGET) {
+    if (D == 3) {
+      Rec3F &r = reinterpret_cast<Rec3F *>(recF)[q];
+      r.s11 = sv(0, 0); r.s22 = sv(1, 1); r.s33 = sv(2, 2);
+      r.s12 = sv(0, 1); r.s13 = sv(0, 2); r.s23 = sv(1, 2);
+    } else {
+      Rec2F &r = reinterpret_cast<Rec2F *>(recF)[q];
+      r.s11 = sv(0, 0); r.s22 = sv(1, 1); r.s12 = sv(0, 1);
+    }
+  } else {
+    double *o = src + (size_t)q * (D + D * D) + D;
+#pragma unroll
+    for (int e = 0; e < D * D; ++e) o[e] = (double)C[e] * sc2;
+  }
+}
+
 // Isotropic target records from the full ones (every kept covariance s I):
 // s times ln 4, as dugma_init builds them.
 template <int D>
@@ -1107,6 +1192,94 @@ extern "C" int dugma_update_sigma_knn(dugma_ctx *ctx, int which, int k, float *s
       hipLaunchKernelGGL(k_iso_records<2>, grid, blk, 0, c->stream, c->dTrecF, c->nT, c->dTrecI);
     DUGMA_HIP(hipGetLastError());
   }
+  if (unchanged) *unchanged = badT + badS;
+  const float qnan = std::numeric_limits<float>::quiet_NaN();
+  if (wantT) {
+    std::fill(sigma_target, sigma_target + c->N, qnan);
+    for (int q = 0; q < nT; ++q) sigma_target[c->hTidx[q]] = hT[q];
+  }
+  if (wantS) {
+    std::fill(sigma_source, sigma_source + c->M, qnan);
+    for (int q = 0; q < nS; ++q) sigma_source[c->hSidx[q]] = hS[q];
+  }
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_update_cov_knn(dugma_ctx *ctx, int which, int k, double scale, double floor,
+                                    float *sigma_target, float *sigma_source, int *unchanged) {
+  if (unchanged) *unchanged = 0;
+  if (!ctx) return fail(DUGMA_ERR_ARG, "dugma_update_cov_knn: null context");
+  if (which < 1 || which > 3)
+    return fail(DUGMA_ERR_ARG, "dugma_update_cov_knn: which must be DUGMA_SIGMA_TARGET, DUGMA_SIGMA_SOURCE or both");
+  if (k < 2 || k > 32) return fail(DUGMA_ERR_ARG, "dugma_update_cov_knn: k must be in 2..32");
+  if (!(scale >= 0) || !(floor >= 0) || !std::isfinite(scale) || !std::isfinite(floor) || !(scale + floor > 0))
+    return fail(DUGMA_ERR_ARG, "dugma_update_cov_knn: scale and floor must be >= 0, not both 0");
+  dugma_ctx *c = ctx;
+  const int D = c->D;
+  const bool doT = (which & DUGMA_SIGMA_TARGET) != 0, doS = (which & DUGMA_SIGMA_SOURCE) != 0;
+  const int nT = doT ? c->nT : 0, nS = doS ? c->nS : 0;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  // sigbuf: target points | target sigma | source points | source sigma | 2 counts
+  const size_t oPT = 0, oST = al((size_t)nT * D * 4), oPS = oST + al((size_t)nT * 4),
+               oSS = oPS + al((size_t)nS * D * 4), oCnt = oSS + al((size_t)nS * 4);
+  int rc = c->sigbuf.ensure(oCnt + 256);
+  if (rc) return rc;
+  char *sb = (char *)c->sigbuf.ptr;
+  int *cnt = (int *)(sb + oCnt);
+  DUGMA_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(int), c->stream));
+  const dim3 blk(kSBlock);
+  const float fs = (float)scale, ff = (float)floor;
+  for (int cloud = 0; cloud < 2; ++cloud) {
+    const bool tgt = cloud == 0;
+    const int n = tgt ? nT : nS;
+    if (n == 0) continue;
+    if (!tgt && nT > 0) DUGMA_HIP(hipStreamSynchronize(c->stream));  // the target's lists are consumed
+    float *P = (float *)(sb + (tgt ? oPT : oPS));
+    float *sig = (float *)(sb + (tgt ? oST : oSS));
+    const int *idx = tgt ? c->dTidx : c->dSidx;
+    const float *Pall = tgt ? c->dX : c->dY;
+    const dim3 grid((n + kSBlock - 1) / kSBlock);
+    if (D == 3)
+      hipLaunchKernelGGL(k_gather_points<3>, grid, blk, 0, c->stream, Pall, idx, n, P);
+    else
+      hipLaunchKernelGGL(k_gather_points<2>, grid, blk, 0, c->stream, Pall, idx, n, P);
+    DUGMA_HIP(hipGetLastError());
+    const float *od;
+    const int *oi;
+    rc = knn_device(c, k, nullptr, nullptr, P, n, P, n, 1, &od, &oi);
+    if (rc) return rc;
+    const double sc2 = c->scale * c->scale;
+    float *Cinv = tgt ? c->dXCinv : c->dYCinv;
+    float *g = tgt ? c->dgX : c->dgY;
+#define COV_APPLY(DD, TT)                                                                                       \
+  hipLaunchKernelGGL((k_cov_apply<DD, TT>), grid, blk, 0, c->stream, od, oi, P, n, k, fs, ff, idx, sc2, Cinv, g, \
+                     c->dTrecF, c->dSsrc, sig, cnt + cloud)
+    if (D == 3) {
+      if (tgt) COV_APPLY(3, true); else COV_APPLY(3, false);
+    } else {
+      if (tgt) COV_APPLY(2, true); else COV_APPLY(2, false);
+    }
+#undef COV_APPLY
+    DUGMA_HIP(hipGetLastError());
+  }
+  const bool wantT = sigma_target && doT, wantS = sigma_source && doS;
+  const size_t hbytes = 16 + (wantT ? (size_t)nT * 4 : 0) + (wantS ? (size_t)nS * 4 : 0);
+  rc = c->hstage.ensure(hbytes);
+  if (rc) return rc;
+  char *h = (char *)c->hstage.ptr;
+  DUGMA_HIP(hipMemcpyAsync(h, cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  float *hT = (float *)(h + 16), *hS = hT + (wantT ? nT : 0);
+  if (wantT && nT) DUGMA_HIP(hipMemcpyAsync(hT, sb + oST, (size_t)nT * 4, hipMemcpyDeviceToHost, c->stream));
+  if (wantS && nS) DUGMA_HIP(hipMemcpyAsync(hS, sb + oSS, (size_t)nS * 4, hipMemcpyDeviceToHost, c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  const int badT = ((const int *)h)[0], badS = ((const int *)h)[1];
+  // full covariances from now on (unless no point changed)
+  if (doT && nT > badT) c->isoT = false;
+  if (doS && nS > badS) {
+    c->isoS = false;
+    c->hSsrc_stale = true;
+  }
+  c->iso = c->isoT && c->isoS && c->nT > 0 && c->nS > 0;
   if (unchanged) *unchanged = badT + badS;
   const float qnan = std::numeric_limits<float>::quiet_NaN();
   if (wantT) {

@@ -245,6 +245,24 @@ enum {
 int dugma_update_sigma_knn(dugma_ctx *ctx, int which, int k, float *sigma_target,
                            float *sigma_source, int *unchanged);
 
+/* Per-point surface covariances from the same k (2..32) nearest neighbours,
+ * on the device (the sample-covariance update; extends update_sig.cu:181-269
+ * like dugma_update_sigma_knn, no reference function computes it): with
+ * sigma_i as above and, in FP32, nearest first,
+ *   mu = (p_1 + ... + p_k) / k,
+ *   c_ab = ((p_1a - mu_a)(p_1b - mu_b) + ... + (p_ka - mu_a)(p_kb - mu_b)) / k,
+ *   C_ab = fl32(scale) c_ab + (a == b ? fl32(floor) (sigma_i sigma_i) : 0),
+ * written everywhere the context holds the point's covariance, exactly as
+ * dugma_init would have stored that FP32 covariance (init's inverse and g,
+ * and the energy state), with no host round trip.  Flat along the surface,
+ * these covariances make the energy of two independently sampled clouds peak
+ * at the true pose.  A point whose C is not positive definite, or with fewer
+ * than k other points, keeps its covariance; *unchanged (may be NULL) counts
+ * them.  sigma_target / sigma_source (may be NULL) as dugma_update_sigma_knn.
+ * The context then uses the full-covariance kernels. */
+int dugma_update_cov_knn(dugma_ctx *ctx, int which, int k, double scale, double floor,
+                         float *sigma_target, float *sigma_source, int *unchanged);
+
 /* ---- batched registration (no reference counterpart) -------------------- */
 
 /* Many independent 3-D rigid registrations of uneven sizes at once (the

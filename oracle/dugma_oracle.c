@@ -229,42 +229,84 @@ void dor_knn(int D, int nq, const float *Q, int np, const float *P, int k,
 
 /* Per-point sigma from the k nearest neighbours within one cloud: the
  * lists of dor_knn (self excluded), then the mean distance nearest first. */
+/* The k <= 64 nearest other points of point `self` of P, in (distance,
+ * index) order (as dor_knn ranks them); missing ones: +inf, -1. */
+static void knn_self(int D, int self, int np, const float *P, int k, float *bd, int *bi) {
+  for (int s = 0; s < k; ++s) {
+    bd[s] = INFINITY;
+    bi[s] = -1;
+  }
+  const float *q = P + (size_t)self * D;
+  for (int j = 0; j < np; ++j) {
+    if (j == self) continue;
+    const float *p = P + (size_t)j * D;
+    float dx = q[0] - p[0], dy = q[1] - p[1];
+    float dd;
+    if (D == 3) {
+      float dz = q[2] - p[2];
+      dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    } else {
+      dd = fmaf(dy, dy, dx * dx);
+    }
+    if (!(dd < bd[k - 1])) continue;
+    int s = k - 1;
+    while (s > 0 && dd < bd[s - 1]) {
+      bd[s] = bd[s - 1];
+      bi[s] = bi[s - 1];
+      --s;
+    }
+    bd[s] = dd;
+    bi[s] = j;
+  }
+}
+
 void dor_knn_sigma(int D, int nq, const int *qidx, int np, const float *P,
                    int k, float *sigma) {
 #pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < nq; ++i) {
-    const int self = qidx ? qidx[i] : i;
     float bd[64];
     int bi[64];
-    for (int s = 0; s < k; ++s) {
-      bd[s] = INFINITY;
-      bi[s] = -1;
-    }
-    const float *q = P + (size_t)self * D;
-    for (int j = 0; j < np; ++j) {
-      if (j == self) continue;
-      const float *p = P + (size_t)j * D;
-      float dx = q[0] - p[0], dy = q[1] - p[1];
-      float dd;
-      if (D == 3) {
-        float dz = q[2] - p[2];
-        dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      } else {
-        dd = fmaf(dy, dy, dx * dx);
-      }
-      if (!(dd < bd[k - 1])) continue;
-      int s = k - 1;
-      while (s > 0 && dd < bd[s - 1]) {
-        bd[s] = bd[s - 1];
-        bi[s] = bi[s - 1];
-        --s;
-      }
-      bd[s] = dd;
-      bi[s] = j;
-    }
+    knn_self(D, qidx ? qidx[i] : i, np, P, k, bd, bi);
     float sum = sqrtf(bd[0]);
     for (int s = 1; s < k; ++s) sum = sum + sqrtf(bd[s]);
     sigma[i] = sum / (float)k;
+  }
+}
+
+void dor_knn_cov(int D, int nq, const int *qidx, int np, const float *P, int k,
+                 float scale, float floor, float *sigma, float *cov) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < nq; ++i) {
+    float bd[64];
+    int bi[64];
+    knn_self(D, qidx ? qidx[i] : i, np, P, k, bd, bi);
+    float sum = sqrtf(bd[0]);
+    for (int s = 1; s < k; ++s) sum = sum + sqrtf(bd[s]);
+    const float sg = sum / (float)k;
+    sigma[i] = sg;
+    float *o = cov + (size_t)i * D * D;
+    if (bi[k - 1] < 0) {
+      for (int e = 0; e < D * D; ++e) o[e] = NAN;
+      continue;
+    }
+    float mu[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) {
+      float m = P[(size_t)bi[0] * D + a];
+      for (int s = 1; s < k; ++s) m = m + P[(size_t)bi[s] * D + a];
+      mu[a] = m / (float)k;
+    }
+    const float fl = floor * (sg * sg);
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) {
+        float c = 0.0f;
+        for (int s = 0; s < k; ++s) {
+          const float u = P[(size_t)bi[s] * D + a] - mu[a], v = P[(size_t)bi[s] * D + b] - mu[b];
+          c = c + u * v;
+        }
+        float cab = scale * (c / (float)k);
+        if (a == b) cab = cab + fl;
+        o[a + D * b] = cab;
+      }
   }
 }
 

@@ -59,6 +59,16 @@ void dor_knn(int D, int nq, const float *Q, int np, const float *P, int k,
 void dor_knn_sigma(int D, int nq, const int *qidx, int np, const float *P,
                    int k, float *sigma);
 
+/* Per-point surface covariance from the same k <= 64 nearest neighbours
+ * (self excluded, dor_knn's ranking), in FP32, nearest first:
+ *   sigma[i] as dor_knn_sigma;
+ *   mu_a = (p_1a + ... + p_ka) / k,
+ *   c_ab = ((p_1a - mu_a)(p_1b - mu_b) + ... + (p_ka - mu_a)(p_kb - mu_b)) / k,
+ *   C_ab = scale c_ab + (a == b ? floor (sigma sigma) : 0),
+ * cov[i] column-major D x D; NaN when P has fewer than k other points. */
+void dor_knn_cov(int D, int nq, const int *qidx, int np, const float *P, int k,
+                 float scale, float floor, float *sigma, float *cov);
+
 /* Convolved-uncertainty energy (new; no reference function computes it):
  *   E = sum_i sum_j N(y_i - (R x_j + t); Sy_i + R Sx_j R^T)
  * in FP64.  Gradient outputs may be NULL:

@@ -49,6 +49,7 @@ def lib():
         L.dor_tree_sum_f32.restype = ctypes.c_float
         L.dor_knn.argtypes = [c_int, c_int, _f, c_int, _f, c_int, c_int, _i, _f]
         L.dor_knn_sigma.argtypes = [c_int, c_int, _i, c_int, _f, c_int, _f]
+        L.dor_knn_cov.argtypes = [c_int, c_int, _i, c_int, _f, c_int, ctypes.c_float, ctypes.c_float, _f, _f]
         L.dor_energy.argtypes = [c_int, c_int, _d, _d, c_int, _d, _d, _d, _d,
                                  _d, _d, _d, _d, _d, _d]
         L.dor_energy.restype = c_double
@@ -151,6 +152,21 @@ def knn_sigma(P, k=8, queries=None):
     out = np.empty(nq, np.float32)
     lib().dor_knn_sigma(D, nq, _p(q, _i), n, _p(P, _f), int(k), _p(out, _f))
     return out
+
+
+def knn_cov(P, k=8, scale=0.5, floor=0.002, queries=None):
+    """(sigma, C) of points `queries` of P (None: all): sigma as knn_sigma
+    and the surface covariance scale * (sample covariance of the k nearest
+    other points) + floor * sigma^2 I, C as (nq, D, D) (see dugma_oracle.h)."""
+    P = _c32(P)
+    n, D = P.shape
+    q = None if queries is None else np.ascontiguousarray(queries, np.int32)
+    nq = n if q is None else q.size
+    sig = np.empty(nq, np.float32)
+    cov = np.empty((nq, D * D), np.float32)
+    lib().dor_knn_cov(D, nq, _p(q, _i), n, _p(P, _f), int(k), float(scale), float(floor), _p(sig, _f),
+                      _p(cov, _f))
+    return sig, np.transpose(cov.reshape(nq, D, D), (0, 2, 1))
 
 
 def energy(Y, SY, X, SX, R, t, grads=False, per_target=False):

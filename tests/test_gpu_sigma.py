@@ -156,3 +156,88 @@ def test_sigma_argument_errors():
                 dg.update_sigma_knn(h, which, k)
     finally:
         dg.fr_m(h)
+
+
+# ---- surface covariances from the k nearest neighbours ----------------------
+def valid_cov(C):
+    """The energy's validity rule on FP32 covariances (n, D, D): finite,
+    positive leading principal minors in FP64."""
+    D = C.shape[1]
+    ok = np.isfinite(C).all((1, 2)) & (C[:, 0, 0] > 0)
+    ok &= C[:, 0, 0] * C[:, 1, 1] - C[:, 0, 1] * C[:, 1, 0] > 0
+    if D == 3:
+        ok &= np.linalg.det(np.nan_to_num(C)) > 0
+    return ok
+
+
+@pytest.mark.parametrize("D", [3, 2])
+def test_cov_bit_exact_and_context_equal_to_fresh_init(oracle, D):
+    rng = np.random.default_rng(60 + D)
+    X, XC = odd_cloud(rng, D, 700)
+    Y, YC = odd_cloud(rng, D, 520)
+    kx, ky = kept(X, XC), kept(Y, YC)
+    ok, h = dg.init(X, XC, Y, YC)
+    try:
+        sT, sS, unchanged = dg.update_cov_knn(h, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8, 0.5, 0.002)
+        handles = [h.read_handle(w) for w in range(6)]
+        assert h.energy_info()[2] == 0
+        R = W.rot2(0.2) if D == 2 else W.rot_axis_angle([1, 2, 3], 0.2)
+        t = np.full(D, 0.03)
+        E, g = dg.energy_grad(h, R, t)
+    finally:
+        dg.fr_m(h)
+    refT, CT = oracle.knn_cov(X.T[kx], 8, 0.5, 0.002)
+    refS, CS = oracle.knn_cov(Y.T[ky], 8, 0.5, 0.002)
+    np.testing.assert_array_equal(sT[kx], refT)
+    np.testing.assert_array_equal(sS[ky], refS)
+    vT, vS = valid_cov(CT.astype(np.float64)), valid_cov(CS.astype(np.float64))
+    # the ten coincident points of each cloud have C = 0 and keep theirs
+    assert unchanged == int((~vT).sum() + (~vS).sum()) and unchanged >= 20
+    XC2, YC2 = W.unblocks(XC).copy(), W.unblocks(YC).copy()
+    XC2[kx[vT]] = CT[vT]
+    YC2[ky[vS]] = CS[vS]
+    ok, h2 = dg.init(X, W.blocks(XC2), Y, W.blocks(YC2))
+    try:
+        for w in range(6):
+            np.testing.assert_array_equal(handles[w], h2.read_handle(w))
+        E2, g2 = dg.energy_grad(h2, R, t)
+    finally:
+        dg.fr_m(h2)
+    assert abs(E / E2 - 1) <= 1e-12
+    np.testing.assert_allclose(g, g2, rtol=1e-10, atol=1e-12 * np.abs(g2).max())
+
+
+def test_cov_full_size_config3(oracle):
+    """200,000 x 200,000: 512 sampled points of each cloud bit-exact against
+    the oracle (sigma and the inverse-covariance and g handles), and the
+    updated context's energy and pose gradient against the FP64 oracle with
+    the oracle's covariances on a subsample."""
+    d = W.config3()
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        sT, sS, un = dg.update_cov_knn(h, dg.SIGMA_TARGET | dg.SIGMA_SOURCE, 8, 0.5, 0.002)
+        hXi, hgX, hYi, hgY = (h.read_handle(w) for w in (1, 4, 3, 5))
+    finally:
+        dg.fr_m(h)
+    assert un == 0
+    rng = np.random.default_rng(5)
+    for s, P, Ci, gg in ((sT, d["X"], hXi, hgX), (sS, d["Y"], hYi, hgY)):
+        q = rng.choice(P.shape[1], 512, replace=False)
+        ref_s, ref_C = oracle.knn_cov(P.T, 8, 0.5, 0.002, queries=q)
+        np.testing.assert_array_equal(s[q], ref_s)
+        inv, g, bad = oracle.prep_inverse(ref_C.astype(np.float64), 3)
+        assert bad == 0
+        np.testing.assert_array_equal(Ci.reshape(-1, 9)[q], inv)
+        np.testing.assert_array_equal(gg[q], g)
+
+
+def test_cov_argument_errors():
+    d = W.config1(n=60, m=50)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        for which, k, sc, fl in ((0, 8, 0.5, 0.0), (1, 1, 0.5, 0.0), (1, 33, 0.5, 0.0), (1, 8, -1.0, 0.0),
+                                 (1, 8, 0.0, 0.0)):
+            with pytest.raises(dg.DugmaError):
+                dg.update_cov_knn(h, which, k, sc, fl)
+    finally:
+        dg.fr_m(h)

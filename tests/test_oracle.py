@@ -570,3 +570,23 @@ def test_dugma_loop_reproduces_published_kinect_outcomes(oracle, k):
     assert W.rotation_error_deg(R, Rp) < 0.01
     assert np.linalg.norm(t - tp) < 1e-4 * np.linalg.norm(d["t_G"])
 
+
+
+def test_knn_cov_against_numpy_and_plane(oracle):
+    """dor_knn_cov: the same neighbours as dor_knn, their FP64 sample
+    covariance (numpy) to FP32 rounding, and on a planar patch the normal
+    direction holds exactly the floor: C_zz = floor sigma^2, C_xz = C_yz = 0."""
+    rng = np.random.default_rng(7)
+    P = rng.normal(size=(400, 3)).astype(np.float32)
+    sig, C = oracle.knn_cov(P, 8, 0.5, 0.002)
+    idx, d2 = oracle.knn(P, P, 8, self_exclude=True)
+    np.testing.assert_array_equal(sig, oracle.knn_sigma(P, 8))
+    nb = P.astype(np.float64)[idx]
+    mu = nb.mean(1, keepdims=True)
+    ref = 0.5 * np.einsum("nka,nkb->nab", nb - mu, nb - mu) / 8 + (0.002 * sig.astype(np.float64) ** 2)[:, None, None] * np.eye(3)
+    np.testing.assert_allclose(C, ref, rtol=2e-5, atol=1e-6 * np.abs(ref).max())
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), indexing="ij"), -1).reshape(-1, 2) * 0.25
+    Q = np.concatenate([g + rng.uniform(-0.05, 0.05, size=g.shape), np.full((len(g), 1), 2.0)], 1).astype(np.float32)
+    s2, C2 = oracle.knn_cov(Q, 8, 0.5, 0.002)
+    np.testing.assert_array_equal(C2[:, 2, 2], np.float32(0.002) * (s2 * s2))
+    assert (C2[:, 0, 2] == 0).all() and (C2[:, 1, 2] == 0).all()
