@@ -273,6 +273,7 @@ static void free_ctx(dugma_ctx *c) {
   if (c->hparts) (void)hipHostFree(c->hparts);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evpts) (void)hipEventDestroy(c->evpts);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -764,17 +764,23 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
   rc = c->hstage.ensure(sizeof(double) * (nT + nS) * nacc);
   if (rc) return rc;
   const double *hT = (const double *)c->hstage.ptr, *hS = hT + nT * nacc;
+  rc = refresh_host_sources(c);  // the host copy of the sources, before the passes (it may sync)
+  if (rc) return rc;
+  if (!c->evpts) DUGMA_HIP(hipEventCreateWithFlags(&c->evpts, hipEventDisableTiming));
+  // target pass, its sums on their way to the host, then the source pass:
+  // the host converts the target sums while the source pass runs
   rc = point_pass(c, c->dTrecF, c->nT, c->dSrecF, c->nS, 1, acc);
-  if (!rc) rc = point_pass(c, c->dSrecF, c->nS, c->dTrecF, c->nT, 2, acc + nT * nacc);
   hipError_t e = hipSuccess;
-  if (!rc)
-    e = hipMemcpyAsync(c->hstage.ptr, acc, sizeof(double) * (nT + nS) * nacc, hipMemcpyDeviceToHost, c->stream);
+  if (!rc) e = hipMemcpyAsync(c->hstage.ptr, acc, sizeof(double) * nT * nacc, hipMemcpyDeviceToHost, c->stream);
+  if (!rc && e == hipSuccess) e = hipEventRecord(c->evpts, c->stream);
+  if (!rc && e == hipSuccess) rc = point_pass(c, c->dSrecF, c->nS, c->dTrecF, c->nT, 2, acc + nT * nacc);
+  if (!rc && e == hipSuccess)
+    e = hipMemcpyAsync((double *)c->hstage.ptr + nT * nacc, acc + nT * nacc, sizeof(double) * nS * nacc,
+                       hipMemcpyDeviceToHost, c->stream);
   clear_points();  // while the passes run
   if (rc) return rc;
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->evpts);
   if (e != hipSuccess) return hip_fail(e, "dugma_energy_grad_points: download");
-  rc = refresh_host_sources(c);
-  if (rc) return rc;
 
   // Everything below is in scaled units until the final unscaling:
   // positions x s, covariances x s^2, energy / s^D.
@@ -895,11 +901,11 @@ extern "C" int dugma_energy_grad_points(dugma_ctx *ctx, const double *R,
   const size_t ncT = (nT + kHostChunk - 1) / kHostChunk, ncS = (nS + kHostChunk - 1) / kHostChunk;
   std::vector<double> Epart(ncT);
   std::vector<PoseSums> Ppart(ncS);
-  parallel_for(ncT + ncS, [&](size_t q) {
-    if (q < ncT)
-      target_chunk(q * kHostChunk, std::min(nT, (q + 1) * kHostChunk), Epart[q]);
-    else
-      source_chunk((q - ncT) * kHostChunk, std::min(nS, (q - ncT + 1) * kHostChunk), Ppart[q - ncT]);
+  parallel_for(ncT, [&](size_t q) { target_chunk(q * kHostChunk, std::min(nT, (q + 1) * kHostChunk), Epart[q]); });
+  e = hipStreamSynchronize(c->stream);  // the source pass and its sums
+  if (e != hipSuccess) return hip_fail(e, "dugma_energy_grad_points: download");
+  parallel_for(ncS, [&](size_t q) {
+    source_chunk(q * kHostChunk, std::min(nS, (q + 1) * kHostChunk), Ppart[q]);
   });
   double Etot = 0.0;
   for (double e : Epart) Etot += e;

@@ -238,6 +238,7 @@ struct dugma_ctx {
   // optional kernel timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evpts = nullptr;  // per-point gradients: the target sums have reached the host
   double pair_ms = 0.0;
   long long pair_launches = 0;
 
