@@ -392,14 +392,16 @@ class Batch:
         return np.ascontiguousarray(np.transpose(R0, (0, 2, 1))), np.ascontiguousarray(t0)  # column-major R
 
     def autotune(self, R0=None, t0=None, sigma_update=True):
-        """dugma_batch_autotune: returns the chosen launch shape and its time
-        per iteration."""
+        """dugma_batch_autotune: returns the chosen launch shape, its time per
+        isotropic iteration and (with covariances set) per refinement pass."""
         R0c, t0c = self._poses(R0, t0)
-        w, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        w, g, ms, rms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         ch = (ctypes.c_int * 4)()
         _lib.check(_lib.load().dugma_batch_autotune(self._ptr, _dp(R0c), _dp(t0c), int(bool(sigma_update)),
-                                                    ctypes.byref(w), ch, ctypes.byref(g), ctypes.byref(ms)))
-        return dict(waves=w.value, chunks=list(ch), grid_waves=g.value, ms_per_iteration=ms.value)
+                                                    ctypes.byref(w), ch, ctypes.byref(g), ctypes.byref(ms),
+                                                    ctypes.byref(rms)))
+        return dict(waves=w.value, chunks=list(ch), grid_waves=g.value, ms_per_iteration=ms.value,
+                    ms_per_refine_pass=rms.value)
 
     def set_shape(self, waves, chunks, grid_waves):
         ch = (ctypes.c_int * 4)(*[int(c) for c in chunks])
@@ -407,13 +409,14 @@ class Batch:
 
     def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
                  tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
-                 check_every=10, grid=0, sigma_update=True, relax=1.0, coarse_iters=0, refine_iters=0):
+                 check_every=10, grid=0, sigma_update=True, relax=1.0, coarse_iters=0, refine_iters=0,
+                 refine_coarse_iters=0):
         K = self.ntrials
         R0c, t0c = self._poses(R0, t0)
         o = _lib.BatchOpts(int(max_iter), int(bool(anneal)), float(anneal_decay), float(tol_rot),
                            float(tol_trans), int(bool(use_graph)), int(bool(one_trial_per_launch)),
                            int(check_every), int(grid), int(bool(sigma_update)), float(relax),
-                           int(coarse_iters), int(refine_iters))
+                           int(coarse_iters), int(refine_iters), int(refine_coarse_iters))
         R = np.zeros((K, 9))
         t = np.zeros((K, 3))
         iters = np.zeros(K, np.int32)

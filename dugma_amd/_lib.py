@@ -23,7 +23,8 @@ class BatchOpts(ctypes.Structure):
                 ("tol_trans", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int), ("relax", ctypes.c_double),
-                ("coarse_iters", ctypes.c_int), ("refine_iters", ctypes.c_int)]
+                ("coarse_iters", ctypes.c_int), ("refine_iters", ctypes.c_int),
+                ("refine_coarse_iters", ctypes.c_int)]
 
 
 class RegisterOpts(ctypes.Structure):
@@ -67,7 +68,7 @@ SIGNATURES = {
     "dugma_batch_register": (_ci, [_vp, _d, _d, ctypes.POINTER(BatchOpts), _d, _d, _i, _i, _d]),
     "dugma_batch_info": (_ci, [_vp, _i, _i, ctypes.POINTER(ctypes.c_longlong)]),
     "dugma_batch_free": (_ci, [_vp]),
-    "dugma_batch_autotune": (_ci, [_vp, _d, _d, _ci, _i, _i, _i, _d]),
+    "dugma_batch_autotune": (_ci, [_vp, _d, _d, _ci, _i, _i, _i, _d, _d]),
     "dugma_batch_set_shape": (_ci, [_vp, _ci, _i, _ci]),
     "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
     "dugma_read_handle": (_ci, [_vp, _ci, _f]),

@@ -295,6 +295,11 @@ typedef struct {
                                 on -log E with the full covariances of
                                 dugma_batch_set_covariances (one fused energy + pose-gradient
                                 pass per trial pose; stops at tol_rot / tol_trans) */
+  int refine_coarse_iters;   /* with refine_iters > 0: first at most this many BFGS steps on
+                                every 4th target and source point, with their covariances
+                                times 4 (the sampling density is a quarter, so a surface
+                                covariance spans twice the distance), then the full clouds
+                                from the pose reached */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */
@@ -327,16 +332,21 @@ int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
                          int *iters, int *status, double *E);
 int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
                      long long *pairs_per_iteration);
-/* Launch shape of the persistent pair kernel: waves per workgroup (1, 2, 4;
+/* Launch shape of the persistent pair kernels: waves per workgroup (1, 2, 4;
  * the target tile is 64 x waves points), source chunks per target tile for
  * each of 4 size classes of trials (source count < 1000, < 2000, < 4000,
- * more), and resident waves.  dugma_batch_autotune times whole iterations
- * from the poses R0, t0 on the batch's own data, waves first, then each
- * present size class, then the resident waves, keeps the fastest shape and
- * reports it with its time per iteration (ms); any output may be NULL. */
+ * more), and resident waves.  dugma_batch_autotune times whole isotropic
+ * iterations -- plus, once dugma_batch_set_covariances has run, whole
+ * refinement passes, their sum being the figure minimised -- from the poses
+ * R0, t0 on the batch's own data, waves first (4 kept unless fewer are
+ * faster by more than 3%), then each present size class, then the resident
+ * waves, keeps the fastest shape and reports it
+ * with the chosen shape's time per isotropic iteration (ms) and per
+ * refinement pass (refine_ms, 0 without covariances); any output may be
+ * NULL. */
 int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
                          int sigma_update, int *waves, int *chunks,
-                         int *grid_waves, double *ms);
+                         int *grid_waves, double *ms, double *refine_ms);
 int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunks,
                           int grid_waves);
 int dugma_batch_free(dugma_batch *b);

@@ -95,7 +95,7 @@ def test_batch_argument_errors_before_any_device_call():
     assert e.value.code == -1 and "empty trial" in str(e.value)
     L = _lib.load()
     assert L.dugma_batch_register(None, None, None, None, None, None, None, None, None) == -1
-    assert L.dugma_batch_autotune(None, None, None, 1, None, None, None, None) == -1
+    assert L.dugma_batch_autotune(None, None, None, 1, None, None, None, None, None) == -1
     assert L.dugma_batch_free(None) == -1
     ch = (ctypes.c_int * 4)(1, 1, 1, 1)
     assert L.dugma_batch_set_shape(None, 2, ch, 8192) == -1

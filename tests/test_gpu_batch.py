@@ -331,12 +331,13 @@ def test_refine_energy_matches_oracle(c4_cov_trials, oracle):
         assert abs(out["E"][k] / ref - 1) <= 1e-5, (C4_REFINE[k], out["E"][k], ref)
 
 
-def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials):
+@pytest.mark.parametrize("refine_coarse", [0, 8])
+def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials, refine_coarse):
     trials = [c4_cov_trials[k] for k in C4_REFINE]
     b = dg.Batch(trials)
     try:
         kw = dict(max_iter=40, anneal=True, sigma_update=True, coarse_iters=10, refine_iters=20,
-                  tol_rot=1e-4, tol_trans=1e-4)
+                  refine_coarse_iters=refine_coarse, tol_rot=1e-4, tol_trans=1e-4)
         g = b.register(use_graph=True, check_every=7, **kw)
         p = b.register(use_graph=False, **kw)
         o = b.register(one_trial_per_launch=True, **kw)
@@ -349,17 +350,22 @@ def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials):
         np.testing.assert_array_equal(g["status"], a["status"])
 
 
-def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle):
-    """Bench settings then the refinement: every trial ends where the FP64
-    oracle's pose gradient (surface covariances) vanishes next to its size
-    one degree / 0.01 units away, and within 2 degrees of the ground truth
-    in at least 80% of these trials (500-8,000 points, up to 30% outliers,
-    0-90 degree starts)."""
+@pytest.mark.parametrize("schedule", ["full", "coarse"])
+def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedule):
+    """Then the refinement -- after 300 full-cloud isotropic iterations, or
+    (the bench's schedule) straight from the coarse isotropic phase through
+    40 coarse refinement steps: every trial ends where the FP64 oracle's pose
+    gradient (surface covariances, all points) vanishes next to its size one
+    degree / 0.01 units away, and within 2 degrees of the ground truth in at
+    least 80% of these trials (500-8,000 points, up to 30% outliers, 0-90
+    degree starts)."""
     trials = [c4_cov_trials[k] for k in C4_REFINE]
+    sched = (dict(max_iter=460, refine_coarse_iters=0) if schedule == "full"
+             else dict(max_iter=160, refine_coarse_iters=40))
     b = dg.Batch(trials)
     try:
-        out = b.register(max_iter=460, anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9,
-                         coarse_iters=160, tol_rot=1e-5, tol_trans=1e-5, refine_iters=200)
+        out = b.register(anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9, coarse_iters=160,
+                         tol_rot=1e-5, tol_trans=1e-5, refine_iters=200, **sched)
     finally:
         b.free()
     errs = []
@@ -393,5 +399,7 @@ def test_refine_argument_errors():
         b.set_covariances([C], [C])
         with pytest.raises(dg.DugmaError, match="refine_iters"):
             b.register(max_iter=3, refine_iters=-1)
+        with pytest.raises(dg.DugmaError, match="refine_coarse_iters"):
+            b.register(max_iter=3, refine_iters=2, refine_coarse_iters=-1)
     finally:
         b.free()
