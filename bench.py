@@ -340,21 +340,24 @@ def bench_contours_2d(hip, dg, W, steps=200):
 # the refinement on the energy with surface covariances (half the 8-NN
 # sample covariance plus 0.002 sigma^2 I), one fused energy + pose-gradient
 # pass per trial pose, BFGS per trial: at most 60 steps on the same coarse
-# points (covariances times 4), then at most 100 on the full clouds, until
-# the rotation step is below 2e-4 rad and the translation step below 2e-4.
-# The refinement converges from tens of degrees, so the isotropic phase only
-# has to bring the trials into its basin.
+# points (covariances times 4), then at most 100 on the full clouds starting
+# from the coarse level's inverse Hessian, until the rotation step is below
+# 1e-4 rad and the translation step below 1e-4 (as the reference's own
+# driver stops, main_kinect_real_application.m:14-17).  The refinement
+# converges from tens of degrees, so the isotropic phase only has to bring
+# the trials into its basin.
 # Measured on the 4,096 trials (success < 2 deg, registrations/s; coarse
 # isotropic / full isotropic iterations, coarse / full refinement steps, tol):
 #   160 / 300 / 0 / 100, 1e-4: 89.2%, 1,375/s;  160 / 40 / 0 / 100: 86.7%,
 #   2,221-2,355/s;  160 / 0 / 0 / 100: 85.2%, 2,745-2,919/s;
-#   160 / 0 / 40 / 100: 88.1%, 3,663-3,735/s;  the same at tol 3e-4: 87.8%,
-#   4,243-4,424/s, at 1e-3: 84.0%, 5,412/s;  120 / 0 / 40 / 100, 3e-4: 87.5%,
-#   4,452/s;  160 / 0 / 60 / 100, 2e-4: 88.1%, 3,966/s.
+#   160 / 0 / 40 / 100: 88.1%, 3,663-3,735/s (fine level started afresh);
+#   with the coarse inverse Hessian carried over: 160 / 0 / 60 / 100, 1e-4:
+#   88.4%, 5,432/s;  the same at 2e-4: 88.3%, 5,562/s;  120 / 0 / 60 / 100,
+#   2e-4: 88.1%, 5,797/s.
 BATCH_COARSE_ITERS = 160
 BATCH_REFINE_ITERS = 100
 BATCH_REFINE_COARSE_ITERS = 60
-BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=2e-4, tol_trans=2e-4, relax=1.9,
+BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=1e-4, tol_trans=1e-4, relax=1.9,
                   coarse_iters=BATCH_COARSE_ITERS, refine_iters=BATCH_REFINE_ITERS,
                   refine_coarse_iters=BATCH_REFINE_COARSE_ITERS)
 BATCH_MAX_ITER = BATCH_COARSE_ITERS  # no full-cloud isotropic iterations

@@ -501,7 +501,8 @@ struct BRef {
   int bt;               // halvings in the current line search
   int it;               // accepted steps
   int status;           // 0 running, 1 converged, 2 step limit, 3 no weight
-  int pad[3];
+  int warm;             // 1: H (and scaled) carried over from the coarse level
+  int pad[2];
 };
 
 struct RParams {
@@ -709,8 +710,10 @@ __global__ __launch_bounds__(256) void k_batchf_update(
       r.f = -log(E);
       r.E = E;
       for (int a = 0; a < 6; ++a) r.gq[a] = gq[a];
-      for (int a = 0; a < 36; ++a) r.H[a] = (a % 7 == 0) ? 1.0 : 0.0;
-      r.scaled = 0;
+      if (!r.warm) {
+        for (int a = 0; a < 36; ++a) r.H[a] = (a % 7 == 0) ? 1.0 : 0.0;
+        r.scaled = 0;
+      }
       r.state = 1;
       if (rp.max_iter < 1 || !ref_direction(r)) {
         r.status = rp.max_iter < 1 ? 2 : 1;
@@ -1265,8 +1268,13 @@ static int launch_refine(dugma_batch *b, int l, int k0, int k1, const RParams &r
 // max_steps accepted BFGS steps (each line search needs one pass when its
 // first step is accepted).
 // Optimiser states for a refinement on level l from the poses in dtrials,
-// every done flag cleared.
-static int init_refine(dugma_batch *b, int l) {
+// every done flag cleared.  carry (optional): the coarse level's final
+// states; a trial whose inverse Hessian was scaled there starts from it
+// divided by kCoarseStep (the coarse covariances are kCoarseStep times the
+// fine ones, so the curvature of -log E is about that much lower), so that
+// its first step on the fine level is a quasi-Newton step rather than a
+// kernel width.
+static int init_refine(dugma_batch *b, int l, const std::vector<BRef> *carry = nullptr) {
   const int K = b->ntrials;
   std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
@@ -1281,6 +1289,11 @@ static int init_refine(dugma_batch *b, int l) {
     // the first step is a kernel width: twice as wide on the coarse level
     r.w0 = b->refW0[k] * (l == 1 ? std::sqrt((double)kCoarseStep) : 1.0);
     r.smax = 0.25 * r.L;
+    if (carry && (*carry)[k].state == 1 && (*carry)[k].scaled) {
+      for (int a = 0; a < 36; ++a) r.H[a] = (*carry)[k].H[a] / kCoarseStep;
+      r.scaled = 1;
+      r.warm = 1;
+    }
   }
   DUGMA_HIP(hipMemcpyAsync(b->dref, ref.data(), sizeof(BRef) * K, hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_batch_fill, dim3((K + 255) / 256), dim3(256), 0, b->stream, (unsigned *)b->ddone, K, 0u);
@@ -1289,9 +1302,10 @@ static int init_refine(dugma_batch *b, int l) {
   return DUGMA_OK;
 }
 
-static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int max_steps) {
+static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int max_steps,
+                      const std::vector<BRef> *carry = nullptr) {
   const int K = b->ntrials;
-  int rc = init_refine(b, l);
+  int rc = init_refine(b, l, carry);
   if (rc) return rc;
   RParams rp;
   memset(&rp, 0, sizeof(rp));
@@ -1359,11 +1373,11 @@ static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int m
 
 // After a coarse refinement: each trial whose start pose was evaluated
 // continues from the accepted pose, its steps counted with the isotropic
-// ones.
-static int fold_refined_poses(dugma_batch *b) {
+// ones; ref receives the coarse level's final optimiser states.
+static int fold_refined_poses(dugma_batch *b, std::vector<BRef> &ref) {
   const int K = b->ntrials;
   std::vector<BTrial> tr(K);
-  std::vector<BRef> ref(K);
+  ref.assign(K, BRef());
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
   DUGMA_HIP(hipMemcpyAsync(ref.data(), b->dref, sizeof(BRef) * K, hipMemcpyDeviceToHost, b->stream));
   DUGMA_HIP(hipStreamSynchronize(b->stream));
@@ -1487,11 +1501,13 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
     }
   }
   if (rc) return rc;
+  std::vector<BRef> coarse_ref;
   if (opts->refine_iters > 0 && opts->refine_coarse_iters > 0) {
     rc = run_refine(b, opts, 1, opts->refine_coarse_iters);
-    if (!rc) rc = fold_refined_poses(b);
+    if (!rc) rc = fold_refined_poses(b, coarse_ref);
   }
-  if (!rc && opts->refine_iters > 0) rc = run_refine(b, opts, 0, opts->refine_iters);
+  if (!rc && opts->refine_iters > 0)
+    rc = run_refine(b, opts, 0, opts->refine_iters, coarse_ref.empty() ? nullptr : &coarse_ref);
   if (rc) return rc;
   std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
