@@ -385,6 +385,35 @@ def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedul
     assert np.mean(np.array(errs) < 2.0) >= 0.8, errs
 
 
+def test_tail_queue_keeps_results():
+    """Once at most a sixteenth of the trials run, passes use a queue of the
+    running trials' items only (rebuilt as that count halves); 48 trials of
+    uneven sizes that finish at different passes must give exactly the
+    poses, step counts and status of one trial per launch, in the isotropic
+    phase (full clouds) and in both refinement levels."""
+    rng = np.random.default_rng(7)
+    trials = []
+    for s in range(48):
+        tr = small_trial(200 + s, int(rng.integers(150, 700)), int(rng.integers(150, 700)),
+                         deg=float(rng.uniform(2.0, 40.0)))
+        trials.append(tr)
+    b = dg.Batch(trials)
+    try:
+        b.set_covariances([np.tile(np.eye(3), (tr["X"].shape[1], 1, 1)) * tr["sX"][:, None, None] for tr in trials],
+                          [np.tile(np.eye(3), (tr["Y"].shape[1], 1, 1)) * tr["sY"][:, None, None] for tr in trials])
+        kw = dict(max_iter=150, anneal=False, sigma_update=True, tol_rot=1e-6, tol_trans=1e-6,
+                  refine_iters=40, refine_coarse_iters=10)
+        g = b.register(use_graph=True, check_every=3, **kw)
+        o = b.register(one_trial_per_launch=True, **kw)
+    finally:
+        b.free()
+    assert len(set(g["iters"].tolist())) > 8  # the trials finish at many different passes
+    np.testing.assert_array_equal(g["R"], o["R"])
+    np.testing.assert_array_equal(g["t"], o["t"])
+    np.testing.assert_array_equal(g["iters"], o["iters"])
+    np.testing.assert_array_equal(g["status"], o["status"])
+
+
 def test_refine_argument_errors():
     tr = small_trial(50, 300, 300)
     b = dg.Batch([tr])
