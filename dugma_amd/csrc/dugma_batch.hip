@@ -97,7 +97,7 @@ struct BTrial {
   double c[3];  // target centroid (targets are stored relative to it)
   double a;     // annealing variance
   double a_stop;
-  double E;     // energy at the last iteration (without the (2 pi)^-3/2 factor)
+  double E;     // energy at the returned pose, from the energy pass (without the (2 pi)^-3/2 factor)
   double dang, dtr;  // last rotation / translation step
   double g;     // variance scale: sig / sbar with the sigma update, else 1
   double sig;   // mean squared nearest distance / 3 at the last pass
@@ -201,8 +201,11 @@ __global__ __launch_bounds__(256) void k_batch_resig(BTrial *__restrict__ trials
 // values).  The source stream is software-pipelined two deep: record j + 2
 // is requested while record j is used (index clamped to the item), so the
 // scalar-load latency overlaps the arithmetic.
+// EN: also sum E_i.  The pose step does not need it, so the iterations
+// leave it out (one VALU operation per pair fewer) and one pass with EN
+// after the iterations measures the energy at the returned poses.
 constexpr int kItemStat = 17;
-template <int W, bool SIG>
+template <int W, bool SIG, bool EN>
 __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     const Rec3I *__restrict__ T, const BZ *__restrict__ Z, const BSrc *__restrict__ S,
     const BItem *__restrict__ items, const int *__restrict__ order, int nitems,
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
         const float inv = rs * rs;
         const float e = fexp2(-(d2 * inv)) * (rs * inv);
         const float w = e * inv;
-        fE += e;
+        if (EN) fE += e;
         fW += w;
         fD0 = fmaf(w, dx, fD0);
         fD1 = fmaf(w, dy, fD1);
@@ -394,7 +397,6 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
         st[8 + 3 * a + b] = YB[3 * a] * tr.R[b] + YB[3 * a + 1] * tr.R[b + 3] + YB[3 * a + 2] * tr.R[b + 6] +
                             st[2 + a] * tc[b];
   }
-  tr.E = st[0] * kLn4Pow1p5;  // e was computed with variances times ln 4
   tr.iters += 1;
   if (!(W > 0.0) || !isfinite(W)) {
     tr.status = 3;  // no pair has any weight: nothing to align
@@ -478,6 +480,19 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
     done[k] = 1;
   }
   trials[k] = tr;
+}
+
+// After the isotropic iterations: each trial's energy at its pose, from the
+// item sums of one energy pass (in item order); the queue is re-armed.
+__global__ __launch_bounds__(256) void k_batch_energy(const double *__restrict__ part,
+                                                      const int *__restrict__ titem, int K,
+                                                      BTrial *__restrict__ trials, unsigned *__restrict__ queue) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  double e = 0.0;
+  for (int i = titem[k]; i < titem[k + 1]; ++i) e += part[(size_t)i * kStat];
+  trials[k].E = e * kLn4Pow1p5;  // e was computed with variances times ln 4
 }
 
 // ---------------------------------------------------------------------------
@@ -1157,13 +1172,17 @@ extern "C" int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
 }
 
 template <int W>
-static void launch_pairs(dugma_batch *b, const BLevel &L, int grid, const int *order, int ni, bool sigma) {
-  if (sigma)
-    hipLaunchKernelGGL((k_batch_pairs<W, true>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ, L.dS,
-                       L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
+static void launch_pairs(dugma_batch *b, const BLevel &L, int grid, const int *order, int ni, bool sigma,
+                         bool energy = false) {
+  if (energy)  // the energy pass after the iterations (no sigma measurement)
+    hipLaunchKernelGGL((k_batch_pairs<W, false, true>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ,
+                       L.dS, L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
+  else if (sigma)
+    hipLaunchKernelGGL((k_batch_pairs<W, true, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ,
+                       L.dS, L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
   else
-    hipLaunchKernelGGL((k_batch_pairs<W, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ, L.dS,
-                       L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
+    hipLaunchKernelGGL((k_batch_pairs<W, false, false>), dim3(grid), dim3(64 * W), 0, b->stream, L.dT, L.dZ,
+                       L.dS, L.ditems, order, ni, b->dqueue, b->ddone, L.dnn, L.dpart);
 }
 
 // One iteration of trials [k0, k1) on level l.  grid: workgroups of the
@@ -1231,6 +1250,30 @@ struct TailQueue {
     return DUGMA_OK;
   }
 };
+
+// One pass over every trial at its pose after the isotropic iterations that
+// sums the energy (the iterations leave it out), on the full clouds with
+// the trial's last variance scale; the done flags are cleared for it.
+static int energy_pass(dugma_batch *b) {
+  const BLevel &L = b->lv[0];
+  const int K = b->ntrials;
+  hipLaunchKernelGGL(k_batch_fill, dim3((K + 255) / 256), dim3(256), 0, b->stream, (unsigned *)b->ddone, K, 0u);
+  if (L.ntgt > 0)
+    hipLaunchKernelGGL(k_batch_transform, dim3((L.ntgt + 255) / 256), dim3(256), 0, b->stream, L.dT,
+                       L.dtgt_trial, 0, L.ntgt, b->dtrials, b->ddone, L.dZ);
+  if (L.nitems > 0) {
+    const int grid = std::min(std::max(1, b->grid_waves / b->waves), L.nitems);
+    switch (b->waves) {
+      case 1: launch_pairs<1>(b, L, grid, L.dorder, L.nitems, false, true); break;
+      case 2: launch_pairs<2>(b, L, grid, L.dorder, L.nitems, false, true); break;
+      default: launch_pairs<4>(b, L, grid, L.dorder, L.nitems, false, true); break;
+    }
+  }
+  hipLaunchKernelGGL(k_batch_energy, dim3((K + 255) / 256), dim3(256), 0, b->stream, L.dpart, L.dtitem, K,
+                     b->dtrials, b->dqueue);
+  DUGMA_HIP(hipGetLastError());
+  return DUGMA_OK;
+}
 
 // Trial states at the start of a run, uploaded with every done flag clear
 // and every nearest distance at +inf.
@@ -1551,6 +1594,9 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
       rc = launch_iteration(b, level_of(it), 0, K, params(level_of(it)), false, grid);
     }
   }
+  if (rc) return rc;
+  // without a refinement, E is the isotropic energy at the returned pose
+  if (opts->refine_iters == 0) rc = energy_pass(b);
   if (rc) return rc;
   std::vector<BRef> coarse_ref;
   if (opts->refine_iters > 0 && opts->refine_coarse_iters > 0) {

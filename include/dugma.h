@@ -325,7 +325,9 @@ int dugma_batch_set_covariances(dugma_batch *b, const double *const *X_C,
                                 const double *const *Y_C);
 /* R0 / R: ntrials x 9 column-major, t0 / t: ntrials x 3; iters, status
  * (1 converged, 2 iteration limit, 3 no weight), E: ntrials each (any output
- * may be NULL).  With refine_iters > 0, R, t, status and E are those of the
+ * may be NULL).  E is the energy at the returned pose (isotropic: with the
+ * variances of the last iteration, from one extra pass after the
+ * iterations).  With refine_iters > 0, R, t, status and E are those of the
  * refinement and iters counts both phases' steps. */
 int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
                          const dugma_batch_opts *opts, double *R, double *t,

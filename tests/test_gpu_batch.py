@@ -85,6 +85,26 @@ def test_one_step_matches_procrustes(sigma):
         assert out["iters"][k] == 1
 
 
+def test_isotropic_energy_at_returned_pose(oracle):
+    """Without a refinement, E is the isotropic energy at the returned pose
+    (one pass after the iterations): against the FP64 oracle's with the
+    trials' own variances (no sigma update, no annealing: the variance scale
+    stays 1)."""
+    trials = [small_trial(s, n, m) for s, n, m in ((11, 700, 500), (12, 300, 900), (13, 1500, 1200))]
+    b = dg.Batch(trials)
+    try:
+        out = b.register(max_iter=25, anneal=False, sigma_update=False, coarse_iters=8)
+    finally:
+        b.free()
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    for k, tr in enumerate(trials):
+        Y, X = f32(tr["X"].T), f32(tr["Y"].T)
+        SY = f32(tr["sX"])[:, None, None] * np.eye(3)
+        SX = f32(tr["sY"])[:, None, None] * np.eye(3)
+        ref = oracle.energy(Y, SY, X, SX, out["R"][k], out["t"][k])["E"]
+        assert abs(out["E"][k] / ref - 1) <= 1e-5, (k, out["E"][k], ref)
+
+
 @pytest.mark.parametrize("coarse", [0, 9])
 def test_modes_give_identical_poses(coarse):
     """Graph replay, plain launches and one trial per launch run the same
