@@ -16,9 +16,10 @@
  *     about 40, plus 20 FMA feature accumulations.
  *   Batched registration pair (isotropic, sigma-update): S 2 (one FMA),
  *     d 3, |d|^2 5, rsq 1, exponent and weights 6 (with the exp),
- *     accumulations 8 (E, W and three FMA), nearest-distance min 1.  (The
- *     difference is counted as 3, as everywhere in this table: the FP32
- *     hi/lo compensation some kernels add to it is not algorithmic work.)
+ *     accumulations 7 (W and three FMA; the energy is summed in one pass
+ *     after the iterations), nearest-distance min 1.  (The difference is
+ *     counted as 3, as everywhere in this table: the FP32 hi/lo compensation
+ *     some kernels add to it is not algorithmic work.)
  * The reference's own compi spends about 4.7 K operations per pair; it is
  * reported as a ratio, not against this table. */
 #ifndef DUGMA_ROOFLINE_H
@@ -32,7 +33,7 @@
 #define DUGMA_FLOP_2D_FULL_FUSED 50
 #define DUGMA_FLOP_KNN 8
 #define DUGMA_FLOP_COMPI 80
-#define DUGMA_FLOP_BATCH_PAIR 26
+#define DUGMA_FLOP_BATCH_PAIR 25
 
 /* FP32 vector peak of one MI355X, TFLOP/s (AMD's public figure:
  * 256 CU x 2.4 GHz x 256 FLOP per clock per CU). */
