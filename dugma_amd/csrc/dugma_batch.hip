@@ -819,6 +819,9 @@ struct BLevel {
   std::vector<int> titem;
   std::vector<int> horder, hitem_trial;  // host copies: queue order; trial of each item
   long long pairs = 0;              // target x source pairs of one full iteration
+  // launch shape of this level's persistent pair kernels
+  int waves = 4;                        // waves per workgroup: target tile = 64 * waves
+  int chunks[kClasses] = {1, 1, 2, 4};  // source chunks per target tile, per size class
 };
 
 constexpr int kCoarseStep = 4;
@@ -845,9 +848,8 @@ struct dugma_batch {
   std::vector<BSrc> hsrc0;                  // host copy of the level-0 sources
   std::vector<double> refL, refW0;          // per trial: RMS source radius, kernel width
   // launch shape
-  int waves = 4;                            // waves per workgroup: target tile = 64 * waves
-  int chunks[kClasses] = {1, 1, 2, 4};      // source chunks per target tile, per size class
-  int grid_waves = 8192;                    // resident waves of the persistent pair kernel
+  int grid_waves = 8192;                    // resident waves of the persistent pair kernels
+                                            // (tile shapes: per level, BLevel::waves / chunks)
 };
 
 static void free_level(BLevel &L) {
@@ -885,14 +887,14 @@ static int bmalloc(T **p, size_t n) {
 // class; queue order longest first.  (Re)allocates the item arrays when they
 // grow.
 static int make_items(dugma_batch *b, BLevel &L) {
-  const int K = b->ntrials, tile_t = 64 * b->waves;
+  const int K = b->ntrials, tile_t = 64 * L.waves;
   std::vector<BItem> items;
   L.titem.assign(K + 1, 0);
   for (int k = 0; k < K; ++k) {
     L.titem[k] = (int)items.size();
     const int nt = L.nt[k], ns = L.ns[k];
     if (nt == 0 || ns == 0) continue;
-    const int nch = std::max(1, std::min(b->chunks[size_class(ns)], ns));
+    const int nch = std::max(1, std::min(L.chunks[size_class(ns)], ns));
     const int tile_s = (ns + nch - 1) / nch;
     for (int t0 = 0; t0 < nt; t0 += tile_t)
       for (int s0 = 0; s0 < ns; s0 += tile_s) {
@@ -1205,9 +1207,9 @@ static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams
     hipLaunchKernelGGL(k_batch_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT,
                        L.dtgt_trial, t0, t1, b->dtrials, b->ddone, L.dZ);
   if (ni > 0) {
-    if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
+    if (grid <= 0) grid = std::max(1, b->grid_waves / L.waves);
     grid = std::min(grid, ni);
-    switch (b->waves) {
+    switch (L.waves) {
       case 1: launch_pairs<1>(b, L, grid, order, ni, bp.sigma != 0); break;
       case 2: launch_pairs<2>(b, L, grid, order, ni, bp.sigma != 0); break;
       default: launch_pairs<4>(b, L, grid, order, ni, bp.sigma != 0); break;
@@ -1262,8 +1264,8 @@ static int energy_pass(dugma_batch *b) {
     hipLaunchKernelGGL(k_batch_transform, dim3((L.ntgt + 255) / 256), dim3(256), 0, b->stream, L.dT,
                        L.dtgt_trial, 0, L.ntgt, b->dtrials, b->ddone, L.dZ);
   if (L.nitems > 0) {
-    const int grid = std::min(std::max(1, b->grid_waves / b->waves), L.nitems);
-    switch (b->waves) {
+    const int grid = std::min(std::max(1, b->grid_waves / L.waves), L.nitems);
+    switch (L.waves) {
       case 1: launch_pairs<1>(b, L, grid, L.dorder, L.nitems, false, true); break;
       case 2: launch_pairs<2>(b, L, grid, L.dorder, L.nitems, false, true); break;
       default: launch_pairs<4>(b, L, grid, L.dorder, L.nitems, false, true); break;
@@ -1331,9 +1333,9 @@ static int launch_refine(dugma_batch *b, int l, int k0, int k1, const RParams &r
     hipLaunchKernelGGL(k_batchf_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT, b->dTC[l],
                        L.dtgt_trial, t0, t1, b->dtrials, b->dref, b->ddone, b->dZF[l]);
   if (ni > 0) {
-    if (grid <= 0) grid = std::max(1, b->grid_waves / b->waves);
+    if (grid <= 0) grid = std::max(1, b->grid_waves / L.waves);
     grid = std::min(grid, ni);
-    switch (b->waves) {
+    switch (L.waves) {
       case 1: launch_pairsf<1>(b, l, grid, order, ni); break;
       case 2: launch_pairsf<2>(b, l, grid, order, ni); break;
       default: launch_pairsf<4>(b, l, grid, order, ni); break;
@@ -1732,10 +1734,10 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
   return DUGMA_OK;
 }
 
-// Average time of one all-trial iteration (ms) on the fine level under the
-// current launch shape, from the given start poses: one untimed pass, then
-// `reps` timed.
-static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bool sigma, int reps,
+// Average time of one all-trial isotropic iteration (ms) on level l under
+// the current launch shape, from the given start poses: one untimed pass,
+// then `reps` timed.
+static int time_iteration(dugma_batch *b, int l, const double *R0, const double *t0, bool sigma, int reps,
                           hipEvent_t ev0, hipEvent_t ev1, double *ms) {
   int rc = start_trials(b, R0, t0, false, sigma);
   if (rc) return rc;
@@ -1747,10 +1749,10 @@ static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bo
   bp.max_iter = 1 << 30;
   bp.relax = 1.0;
   bp.sigma = sigma ? 1 : 0;
-  if ((rc = launch_iteration(b, 0, 0, b->ntrials, bp, false, 0))) return rc;
+  if ((rc = launch_iteration(b, l, 0, b->ntrials, bp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev0, b->stream));
   for (int r = 0; r < reps; ++r)
-    if ((rc = launch_iteration(b, 0, 0, b->ntrials, bp, false, 0))) return rc;
+    if ((rc = launch_iteration(b, l, 0, b->ntrials, bp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev1, b->stream));
   DUGMA_HIP(hipEventSynchronize(ev1));
   float f = 0.0f;
@@ -1759,23 +1761,23 @@ static int time_iteration(dugma_batch *b, const double *R0, const double *t0, bo
   return DUGMA_OK;
 }
 
-// Average time of one all-trial refinement pass (ms) on the fine level under
-// the current launch shape, from the given start poses: one untimed pass
-// (the start poses), then `reps` timed (candidate poses; nothing converges).
-static int time_refine_pass(dugma_batch *b, const double *R0, const double *t0, int reps, hipEvent_t ev0,
-                            hipEvent_t ev1, double *ms) {
+// Average time of one all-trial refinement pass (ms) on level l under the
+// current launch shape, from the given start poses: one untimed pass (the
+// start poses), then `reps` timed (candidate poses; nothing converges).
+static int time_refine_pass(dugma_batch *b, int l, const double *R0, const double *t0, int reps,
+                            hipEvent_t ev0, hipEvent_t ev1, double *ms) {
   int rc = start_trials(b, R0, t0, false, false);
-  if (!rc) rc = init_refine(b, 0);
+  if (!rc) rc = init_refine(b, l);
   if (rc) return rc;
   RParams rp;
   memset(&rp, 0, sizeof(rp));
   rp.tol_rot = -1.0;
   rp.tol_trans = -1.0;
   rp.max_iter = 1 << 30;
-  if ((rc = launch_refine(b, 0, 0, b->ntrials, rp, false, 0))) return rc;
+  if ((rc = launch_refine(b, l, 0, b->ntrials, rp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev0, b->stream));
   for (int r = 0; r < reps; ++r)
-    if ((rc = launch_refine(b, 0, 0, b->ntrials, rp, false, 0))) return rc;
+    if ((rc = launch_refine(b, l, 0, b->ntrials, rp, false, 0))) return rc;
   DUGMA_HIP(hipEventRecord(ev1, b->stream));
   DUGMA_HIP(hipEventSynchronize(ev1));
   float f = 0.0f;
@@ -1796,16 +1798,17 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     (void)hipEventDestroy(ev0);
     return hip_fail(e, "hipEventCreate");
   }
-  const BLevel &L = b->lv[0];
   int rc = DUGMA_OK;
   double best = 0.0, cur = 0.0;
-  // the figure of merit: one isotropic iteration, plus (once covariances
-  // are set) one refinement pass, both on the fine level
+  // the figure of merit on level l: one isotropic iteration, plus (once
+  // covariances are set) one refinement pass
   double cur_it = 0.0, cur_ref = 0.0, best_it = 0.0, best_ref = 0.0;
+  int lvl = 0;
   auto timed = [&](double *out) {
     cur_ref = 0.0;
-    int r = time_iteration(b, R0, t0, sigma, 2, ev0, ev1, &cur_it);
-    if (!r && b->dTC[0]) r = time_refine_pass(b, R0, t0, 2, ev0, ev1, &cur_ref);
+    const int reps = lvl == 0 ? 2 : 6;  // a coarse pass is ~1/10 of a fine one
+    int r = time_iteration(b, lvl, R0, t0, sigma, reps, ev0, ev1, &cur_it);
+    if (!r && b->dTC[lvl]) r = time_refine_pass(b, lvl, R0, t0, reps, ev0, ev1, &cur_ref);
     *out = cur_it + cur_ref;
     return r;
   };
@@ -1814,71 +1817,85 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     best_it = cur_it;
     best_ref = cur_ref;
   };
-  bool present[kClasses] = {false, false, false, false};
-  for (int k = 0; k < b->ntrials; ++k)
-    if (L.nt[k] > 0 && L.ns[k] > 0) present[size_class(L.ns[k])] = true;
-  // 1. waves per workgroup (the target tile).  The largest tile is kept
-  // unless a smaller one is faster by more than 3%: with all trials active
-  // (as timed here) the shapes are close, and once trials converge the
-  // larger tile's fewer work items cost less to skip (measured on 4,096
-  // trials: the refinement's tail 13% faster with 4 waves than with 2).
-  const int wave_opts[] = {4, 2, 1};
-  int best_w = b->waves;
-  for (int w : wave_opts) {
-    b->waves = w;
-    if ((rc = make_items(b, b->lv[0])) || (rc = timed(&cur))) break;
-    if (best == 0.0 || cur < 0.97 * best) {
-      keep_best();
-      best_w = w;
-    }
-  }
-  b->waves = best_w;
-  // 2. source chunks per target tile, one size class at a time
-  const int chunk_opts[] = {1, 2, 4, 8};
-  for (int c = 0; c < kClasses && !rc; ++c) {
-    if (!present[c]) continue;
-    const int keep = b->chunks[c];
-    int best_n = keep;
-    for (int n : chunk_opts) {
-      if (n == keep) continue;
-      b->chunks[c] = n;
-      if ((rc = make_items(b, b->lv[0])) || (rc = timed(&cur))) break;
-      if (cur < best) {
+  double fine_it = 0.0, fine_ref = 0.0;
+  for (lvl = 0; lvl < 2 && !rc; ++lvl) {
+    BLevel &L = b->lv[lvl];
+    if (L.nitems == 0) continue;
+    best = 0.0;
+    bool present[kClasses] = {false, false, false, false};
+    for (int k = 0; k < b->ntrials; ++k)
+      if (L.nt[k] > 0 && L.ns[k] > 0) present[size_class(L.ns[k])] = true;
+    // 1. waves per workgroup (the target tile).  On the full clouds the
+    // largest tile is kept unless a smaller one is faster by more than 3%:
+    // with all trials active (as timed here) the shapes are close, and once
+    // trials converge the larger tile's fewer work items cost less to skip
+    // (measured on 4,096 trials: the refinement's tail 13% faster with 4
+    // waves than with 2).  The coarse level's iterations never converge,
+    // and its small trials fill a large tile poorly: plain fastest.
+    const double margin = lvl == 0 ? 0.97 : 1.0;
+    const int wave_opts[] = {4, 2, 1};
+    int best_w = L.waves;
+    for (int w : wave_opts) {
+      L.waves = w;
+      if ((rc = make_items(b, L)) || (rc = timed(&cur))) break;
+      if (best == 0.0 || cur < margin * best) {
         keep_best();
-        best_n = n;
+        best_w = w;
       }
     }
-    b->chunks[c] = best_n;
-  }
-  // 3. resident waves of the persistent kernel
-  if (!rc) rc = make_items(b, b->lv[0]);
-  const int grid_opts[] = {4096, 8192, 16384, 32768};
-  const int keep_g = b->grid_waves;
-  int best_g = keep_g;
-  for (int gw : grid_opts) {
-    if (rc || gw == keep_g) continue;
-    b->grid_waves = gw;
-    if ((rc = timed(&cur))) break;
-    if (cur < best) {
-      keep_best();
-      best_g = gw;
+    L.waves = best_w;
+    // 2. source chunks per target tile, one size class at a time
+    const int chunk_opts[] = {1, 2, 4, 8};
+    for (int c = 0; c < kClasses && !rc; ++c) {
+      if (!present[c]) continue;
+      const int keep = L.chunks[c];
+      int best_n = keep;
+      for (int n : chunk_opts) {
+        if (n == keep) continue;
+        L.chunks[c] = n;
+        if ((rc = make_items(b, L)) || (rc = timed(&cur))) break;
+        if (cur < best) {
+          keep_best();
+          best_n = n;
+        }
+      }
+      L.chunks[c] = best_n;
+    }
+    if (!rc) rc = make_items(b, L);
+    // 3. resident waves of the persistent kernels (shared by the levels:
+    // tuned on the full clouds)
+    if (lvl == 0) {
+      const int grid_opts[] = {4096, 8192, 16384, 32768};
+      const int keep_g = b->grid_waves;
+      int best_g = keep_g;
+      for (int gw : grid_opts) {
+        if (rc || gw == keep_g) continue;
+        b->grid_waves = gw;
+        if ((rc = timed(&cur))) break;
+        if (cur < best) {
+          keep_best();
+          best_g = gw;
+        }
+      }
+      b->grid_waves = best_g;
+      fine_it = best_it;
+      fine_ref = best_ref;
     }
   }
-  b->grid_waves = best_g;
   {
-    // both levels follow the kept shape, also after a failed timing run
+    // both levels follow the kept shapes, also after a failed timing run
     const int rc2 = make_all_items(b);
     if (!rc) rc = rc2;
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   if (rc) return rc;
-  if (waves) *waves = b->waves;
+  if (waves) *waves = b->lv[0].waves;
   if (chunks)
-    for (int c = 0; c < kClasses; ++c) chunks[c] = b->chunks[c];
+    for (int c = 0; c < kClasses; ++c) chunks[c] = b->lv[0].chunks[c];
   if (grid_waves) *grid_waves = b->grid_waves;
-  if (ms) *ms = best_it;
-  if (refine_ms) *refine_ms = best_ref;
+  if (ms) *ms = fine_it;
+  if (refine_ms) *refine_ms = fine_ref;
   return DUGMA_OK;
 }
 
@@ -1887,8 +1904,21 @@ extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunk
     return fail(DUGMA_ERR_ARG, "dugma_batch_set_shape: waves must be 1, 2 or 4, chunks >= 1, grid_waves >= 1");
   for (int c = 0; c < kClasses; ++c)
     if (chunks[c] < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_set_shape: chunks must be >= 1");
-  b->waves = waves;
-  for (int c = 0; c < kClasses; ++c) b->chunks[c] = chunks[c];
+  for (BLevel &L : b->lv) {
+    L.waves = waves;
+    for (int c = 0; c < kClasses; ++c) L.chunks[c] = chunks[c];
+  }
   b->grid_waves = grid_waves;
   return make_all_items(b);
+}
+
+extern "C" int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves, int *chunks, int *grid_waves) {
+  if (!b || (level != 0 && level != 1))
+    return fail(DUGMA_ERR_ARG, "dugma_batch_get_shape: null batch or level not 0 / 1");
+  const BLevel &L = b->lv[level];
+  if (waves) *waves = L.waves;
+  if (chunks)
+    for (int c = 0; c < kClasses; ++c) chunks[c] = L.chunks[c];
+  if (grid_waves) *grid_waves = b->grid_waves;
+  return DUGMA_OK;
 }

@@ -335,22 +335,26 @@ int dugma_batch_register(dugma_batch *b, const double *R0, const double *t0,
 int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
                      long long *pairs_per_iteration);
 /* Launch shape of the persistent pair kernels: waves per workgroup (1, 2, 4;
- * the target tile is 64 x waves points), source chunks per target tile for
- * each of 4 size classes of trials (source count < 1000, < 2000, < 4000,
- * more), and resident waves.  dugma_batch_autotune times whole isotropic
- * iterations -- plus, once dugma_batch_set_covariances has run, whole
- * refinement passes, their sum being the figure minimised -- from the poses
- * R0, t0 on the batch's own data, waves first (4 kept unless fewer are
- * faster by more than 3%), then each present size class, then the resident
- * waves, keeps the fastest shape and reports it
- * with the chosen shape's time per isotropic iteration (ms) and per
- * refinement pass (refine_ms, 0 without covariances); any output may be
- * NULL. */
+ * the target tile is 64 x waves points) and source chunks per target tile
+ * for each of 4 size classes of trials (source count < 1000, < 2000, < 4000,
+ * more), per level (0: full clouds, 1: every 4th point), and resident waves.
+ * dugma_batch_autotune times whole isotropic iterations -- plus, once
+ * dugma_batch_set_covariances has run, whole refinement passes, their sum
+ * being the figure minimised -- from the poses R0, t0 on the batch's own
+ * data, on the full clouds first (waves, 4 kept unless fewer are faster by
+ * more than 3%; then each present size class; then the resident waves),
+ * then on the coarse level (waves, size classes), keeps the fastest shapes
+ * and reports the full clouds' one with its time per isotropic iteration
+ * (ms) and per refinement pass (refine_ms, 0 without covariances); any
+ * output may be NULL.  dugma_batch_set_shape gives both levels the same
+ * tile shape; dugma_batch_get_shape reports a level's (chunks: 4 ints). */
 int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
                          int sigma_update, int *waves, int *chunks,
                          int *grid_waves, double *ms, double *refine_ms);
 int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunks,
                           int grid_waves);
+int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves,
+                          int *chunks, int *grid_waves);
 int dugma_batch_free(dugma_batch *b);
 
 /* ---- the six reference handles ------------------------------------------ */
