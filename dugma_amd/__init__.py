@@ -400,11 +400,12 @@ class Batch:
         _lib.check(_lib.load().dugma_batch_autotune(self._ptr, _dp(R0c), _dp(t0c), int(bool(sigma_update)),
                                                     ctypes.byref(w), ch, ctypes.byref(g), ctypes.byref(ms),
                                                     ctypes.byref(rms)))
-        cw = ctypes.c_int()
+        cw, cg = ctypes.c_int(), ctypes.c_int()
         cch = (ctypes.c_int * 4)()
-        _lib.check(_lib.load().dugma_batch_get_shape(self._ptr, 1, ctypes.byref(cw), cch, None))
+        _lib.check(_lib.load().dugma_batch_get_shape(self._ptr, 1, ctypes.byref(cw), cch, ctypes.byref(cg)))
         return dict(waves=w.value, chunks=list(ch), grid_waves=g.value, ms_per_iteration=ms.value,
-                    ms_per_refine_pass=rms.value, coarse_waves=cw.value, coarse_chunks=list(cch))
+                    ms_per_refine_pass=rms.value, coarse_waves=cw.value, coarse_chunks=list(cch),
+                    coarse_grid_waves=cg.value)
 
     def set_shape(self, waves, chunks, grid_waves):
         ch = (ctypes.c_int * 4)(*[int(c) for c in chunks])

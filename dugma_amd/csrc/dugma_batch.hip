@@ -822,6 +822,7 @@ struct BLevel {
   // launch shape of this level's persistent pair kernels
   int waves = 4;                        // waves per workgroup: target tile = 64 * waves
   int chunks[kClasses] = {1, 1, 2, 4};  // source chunks per target tile, per size class
+  int grid_waves = 8192;                // resident waves
 };
 
 constexpr int kCoarseStep = 4;
@@ -848,8 +849,7 @@ struct dugma_batch {
   std::vector<BSrc> hsrc0;                  // host copy of the level-0 sources
   std::vector<double> refL, refW0;          // per trial: RMS source radius, kernel width
   // launch shape
-  int grid_waves = 8192;                    // resident waves of the persistent pair kernels
-                                            // (tile shapes: per level, BLevel::waves / chunks)
+  // (launch shapes: per level, BLevel::waves / chunks / grid_waves)
 };
 
 static void free_level(BLevel &L) {
@@ -1207,7 +1207,7 @@ static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams
     hipLaunchKernelGGL(k_batch_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT,
                        L.dtgt_trial, t0, t1, b->dtrials, b->ddone, L.dZ);
   if (ni > 0) {
-    if (grid <= 0) grid = std::max(1, b->grid_waves / L.waves);
+    if (grid <= 0) grid = std::max(1, L.grid_waves / L.waves);
     grid = std::min(grid, ni);
     switch (L.waves) {
       case 1: launch_pairs<1>(b, L, grid, order, ni, bp.sigma != 0); break;
@@ -1264,7 +1264,7 @@ static int energy_pass(dugma_batch *b) {
     hipLaunchKernelGGL(k_batch_transform, dim3((L.ntgt + 255) / 256), dim3(256), 0, b->stream, L.dT,
                        L.dtgt_trial, 0, L.ntgt, b->dtrials, b->ddone, L.dZ);
   if (L.nitems > 0) {
-    const int grid = std::min(std::max(1, b->grid_waves / L.waves), L.nitems);
+    const int grid = std::min(std::max(1, L.grid_waves / L.waves), L.nitems);
     switch (L.waves) {
       case 1: launch_pairs<1>(b, L, grid, L.dorder, L.nitems, false, true); break;
       case 2: launch_pairs<2>(b, L, grid, L.dorder, L.nitems, false, true); break;
@@ -1333,7 +1333,7 @@ static int launch_refine(dugma_batch *b, int l, int k0, int k1, const RParams &r
     hipLaunchKernelGGL(k_batchf_transform, dim3((t1 - t0 + 255) / 256), dim3(256), 0, b->stream, L.dT, b->dTC[l],
                        L.dtgt_trial, t0, t1, b->dtrials, b->dref, b->ddone, b->dZF[l]);
   if (ni > 0) {
-    if (grid <= 0) grid = std::max(1, b->grid_waves / L.waves);
+    if (grid <= 0) grid = std::max(1, L.grid_waves / L.waves);
     grid = std::min(grid, ni);
     switch (L.waves) {
       case 1: launch_pairsf<1>(b, l, grid, order, ni); break;
@@ -1862,22 +1862,21 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
       L.chunks[c] = best_n;
     }
     if (!rc) rc = make_items(b, L);
-    // 3. resident waves of the persistent kernels (shared by the levels:
-    // tuned on the full clouds)
-    if (lvl == 0) {
-      const int grid_opts[] = {4096, 8192, 16384, 32768};
-      const int keep_g = b->grid_waves;
-      int best_g = keep_g;
-      for (int gw : grid_opts) {
-        if (rc || gw == keep_g) continue;
-        b->grid_waves = gw;
-        if ((rc = timed(&cur))) break;
-        if (cur < best) {
-          keep_best();
-          best_g = gw;
-        }
+    // 3. resident waves of the persistent kernels
+    const int grid_opts[] = {4096, 8192, 16384, 32768};
+    const int keep_g = L.grid_waves;
+    int best_g = keep_g;
+    for (int gw : grid_opts) {
+      if (rc || gw == keep_g) continue;
+      L.grid_waves = gw;
+      if ((rc = timed(&cur))) break;
+      if (cur < best) {
+        keep_best();
+        best_g = gw;
       }
-      b->grid_waves = best_g;
+    }
+    L.grid_waves = best_g;
+    if (lvl == 0) {
       fine_it = best_it;
       fine_ref = best_ref;
     }
@@ -1893,7 +1892,7 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
   if (waves) *waves = b->lv[0].waves;
   if (chunks)
     for (int c = 0; c < kClasses; ++c) chunks[c] = b->lv[0].chunks[c];
-  if (grid_waves) *grid_waves = b->grid_waves;
+  if (grid_waves) *grid_waves = b->lv[0].grid_waves;
   if (ms) *ms = fine_it;
   if (refine_ms) *refine_ms = fine_ref;
   return DUGMA_OK;
@@ -1907,8 +1906,8 @@ extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunk
   for (BLevel &L : b->lv) {
     L.waves = waves;
     for (int c = 0; c < kClasses; ++c) L.chunks[c] = chunks[c];
+    L.grid_waves = grid_waves;
   }
-  b->grid_waves = grid_waves;
   return make_all_items(b);
 }
 
@@ -1919,6 +1918,6 @@ extern "C" int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves
   if (waves) *waves = L.waves;
   if (chunks)
     for (int c = 0; c < kClasses; ++c) chunks[c] = L.chunks[c];
-  if (grid_waves) *grid_waves = b->grid_waves;
+  if (grid_waves) *grid_waves = L.grid_waves;
   return DUGMA_OK;
 }

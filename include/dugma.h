@@ -337,17 +337,18 @@ int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
 /* Launch shape of the persistent pair kernels: waves per workgroup (1, 2, 4;
  * the target tile is 64 x waves points) and source chunks per target tile
  * for each of 4 size classes of trials (source count < 1000, < 2000, < 4000,
- * more), per level (0: full clouds, 1: every 4th point), and resident waves.
+ * more), and resident waves, per level (0: full clouds, 1: every 4th point).
  * dugma_batch_autotune times whole isotropic iterations -- plus, once
  * dugma_batch_set_covariances has run, whole refinement passes, their sum
  * being the figure minimised -- from the poses R0, t0 on the batch's own
  * data, on the full clouds first (waves, 4 kept unless fewer are faster by
  * more than 3%; then each present size class; then the resident waves),
- * then on the coarse level (waves, size classes), keeps the fastest shapes
+ * then on the coarse level (waves, size classes, resident waves), keeps the
+ * fastest shapes
  * and reports the full clouds' one with its time per isotropic iteration
  * (ms) and per refinement pass (refine_ms, 0 without covariances); any
  * output may be NULL.  dugma_batch_set_shape gives both levels the same
- * tile shape; dugma_batch_get_shape reports a level's (chunks: 4 ints). */
+ * shape; dugma_batch_get_shape reports a level's (chunks: 4 ints). */
 int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
                          int sigma_update, int *waves, int *chunks,
                          int *grid_waves, double *ms, double *refine_ms);
