@@ -340,8 +340,9 @@ def bench_contours_2d(hip, dg, W, steps=200):
 # the refinement on the energy with surface covariances (half the 8-NN
 # sample covariance plus 0.002 sigma^2 I), one fused energy + pose-gradient
 # pass per trial pose, BFGS per trial: at most 60 steps on the same coarse
-# points (covariances times 4), then at most 100 on the full clouds starting
-# from the coarse level's inverse Hessian, until the rotation step is below
+# points (covariances times 4), at most 20 on every 2nd point (covariances
+# times 2), then at most 100 on the full clouds, each level starting from the
+# previous one's pose and scaled inverse Hessian, until the rotation step is below
 # 1e-4 rad and the translation step below 1e-4 (as the reference's own
 # driver stops, main_kinect_real_application.m:14-17).  The refinement
 # converges from tens of degrees, so the isotropic phase only has to bring
@@ -353,13 +354,18 @@ def bench_contours_2d(hip, dg, W, steps=200):
 #   160 / 0 / 40 / 100: 88.1%, 3,663-3,735/s (fine level started afresh);
 #   with the coarse inverse Hessian carried over: 160 / 0 / 60 / 100, 1e-4:
 #   88.4%, 5,432/s;  the same at 2e-4: 88.3%, 5,562/s;  120 / 0 / 60 / 100,
-#   2e-4: 88.1%, 5,797/s.
+#   2e-4: 88.1%, 5,797/s.  With a launch shape per level and the middle level
+#   (coarse isotropic / coarse / middle refinement steps, tol 1e-4): 160 / 60 /
+#   0: 88.4%, 6,376/s;  160 / 60 / 20: 88.8%, 6,220/s;  160 / 60 / 40: 88.9%,
+#   6,165/s;  100 / 60 / 30: 88.7%, 6,318/s;  80 / 60 / 30: 88.6%, 6,310/s
+#   (run-to-run spread of the rate about 4%).
 BATCH_COARSE_ITERS = 160
 BATCH_REFINE_ITERS = 100
 BATCH_REFINE_COARSE_ITERS = 60
+BATCH_REFINE_MID_ITERS = 20
 BATCH_MODE = dict(anneal=True, anneal_decay=0.95, sigma_update=True, tol_rot=1e-4, tol_trans=1e-4, relax=1.9,
                   coarse_iters=BATCH_COARSE_ITERS, refine_iters=BATCH_REFINE_ITERS,
-                  refine_coarse_iters=BATCH_REFINE_COARSE_ITERS)
+                  refine_coarse_iters=BATCH_REFINE_COARSE_ITERS, refine_mid_iters=BATCH_REFINE_MID_ITERS)
 BATCH_MAX_ITER = BATCH_COARSE_ITERS  # no full-cloud isotropic iterations
 # the isotropic phase alone, for comparison: up to 300 full-cloud iterations
 BATCH_ISO_MAX_ITER = 300 + BATCH_COARSE_ITERS
@@ -389,7 +395,8 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
         R_G = np.array([tr["R_G"] for tr in trials])
         t_G = np.array([tr["t_G"] for tr in trials])
         outg = b.register(R_G, t_G, max_iter=max(1, max_iter - BATCH_COARSE_ITERS), use_graph=True,
-                          **{**BATCH_MODE, "anneal": False, "coarse_iters": 0, "refine_coarse_iters": 0})
+                          **{**BATCH_MODE, "anneal": False, "coarse_iters": 0, "refine_coarse_iters": 0,
+                             "refine_mid_iters": 0})
         # the isotropic iterations alone (no refinement), for comparison
         t3 = time.perf_counter()
         outi = b.register(max_iter=BATCH_ISO_MAX_ITER, use_graph=True, check_every=10,
@@ -443,10 +450,10 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
             "success_by_min_points": bands,
             "refine": "BFGS per trial on the energy with surface covariances (half the 8-NN sample covariance "
                       "+ 0.002 sigma^2 I), one fused energy + pose-gradient pass per trial pose: at most %d steps "
-                      "on every 4th point (covariances x 4), then at most %d on the full clouds; stop at rotation "
-                      "step < %g rad and translation step < %g"
-                      % (BATCH_MODE["refine_coarse_iters"], BATCH_MODE["refine_iters"], BATCH_MODE["tol_rot"],
-                         BATCH_MODE["tol_trans"]),
+                      "on every 4th point (covariances x 4), %d on every 2nd (x 2), then at most %d on the full "
+                      "clouds; stop at rotation step < %g rad and translation step < %g"
+                      % (BATCH_MODE["refine_coarse_iters"], BATCH_MODE["refine_mid_iters"],
+                         BATCH_MODE["refine_iters"], BATCH_MODE["tol_rot"], BATCH_MODE["tol_trans"]),
             "without_refinement": {"value": round(ntrials / wall_iso, 2), "unit": "registrations/s",
                                    "max_iter": BATCH_ISO_MAX_ITER,
                                    "success_rate": round(float((errs_i < 2.0).mean()), 4),

@@ -414,13 +414,14 @@ class Batch:
     def register(self, R0=None, t0=None, max_iter=80, anneal=True, anneal_decay=0.85,
                  tol_rot=1e-7, tol_trans=1e-7, use_graph=True, one_trial_per_launch=False,
                  check_every=10, grid=0, sigma_update=True, relax=1.0, coarse_iters=0, refine_iters=0,
-                 refine_coarse_iters=0):
+                 refine_coarse_iters=0, refine_mid_iters=0):
         K = self.ntrials
         R0c, t0c = self._poses(R0, t0)
         o = _lib.BatchOpts(int(max_iter), int(bool(anneal)), float(anneal_decay), float(tol_rot),
                            float(tol_trans), int(bool(use_graph)), int(bool(one_trial_per_launch)),
                            int(check_every), int(grid), int(bool(sigma_update)), float(relax),
-                           int(coarse_iters), int(refine_iters), int(refine_coarse_iters))
+                           int(coarse_iters), int(refine_iters), int(refine_coarse_iters),
+                           int(refine_mid_iters))
         R = np.zeros((K, 9))
         t = np.zeros((K, 3))
         iters = np.zeros(K, np.int32)

@@ -24,7 +24,7 @@ class BatchOpts(ctypes.Structure):
                 ("one_trial_per_launch", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("grid", ctypes.c_int), ("sigma_update", ctypes.c_int), ("relax", ctypes.c_double),
                 ("coarse_iters", ctypes.c_int), ("refine_iters", ctypes.c_int),
-                ("refine_coarse_iters", ctypes.c_int)]
+                ("refine_coarse_iters", ctypes.c_int), ("refine_mid_iters", ctypes.c_int)]
 
 
 class RegisterOpts(ctypes.Structure):

@@ -55,10 +55,12 @@
 //                       by the target-side identity, in the source frame
 //   k_batchf_update     sums the items, rotates the gradient back, Armijo
 //                       test, BFGS update, next candidate pose
-// Optionally (refine_coarse_iters) the refinement runs first on the coarse
-// level's points, with their covariances times kCoarseStep (a quarter of
-// the sampling density: a surface covariance spans twice the distance),
-// then restarts BFGS on the full clouds from the pose reached.
+// Optionally the refinement runs first on sparser levels' points, with
+// their covariances times the level's point step: the coarse level
+// (refine_coarse_iters; a quarter of the sampling density, so a surface
+// covariance spans twice the distance), then every 2nd point
+// (refine_mid_iters), each restarting BFGS from the previous level's pose
+// and scaled inverse Hessian, before the full clouds.
 //
 // Launch shape: waves per workgroup (target tile = 64 x waves), the number
 // of source chunks per target tile for each size class of trials (by source
@@ -799,7 +801,10 @@ __global__ __launch_bounds__(256) void k_batchf_update(
 
 // One resolution of every trial's clouds.  Level 0 holds all kept points;
 // level 1 every kCoarseStep-th target and source of each trial (same
-// centroid, same per-point variances), for the early, broad iterations.
+// centroid, same per-point variances), for the early, broad iterations;
+// level 2 every 2nd point, between the two in the refinement only.
+constexpr int kLevels = 3;
+constexpr int kLevelStep[kLevels] = {1, 4, 2};
 struct BLevel {
   int ntgt = 0, nsrc = 0, nitems = 0;
   Rec3I *dT = nullptr;       // targets (all trials), relative to each trial's centroid
@@ -825,7 +830,7 @@ struct BLevel {
   int grid_waves = 8192;                // resident waves
 };
 
-constexpr int kCoarseStep = 4;
+constexpr int kCoarseStep = kLevelStep[1];
 
 }  // namespace dugma
 
@@ -834,16 +839,16 @@ using namespace dugma;
 struct dugma_batch {
   int ntrials = 0;
   hipStream_t stream = nullptr;
-  BLevel lv[2];
+  BLevel lv[kLevels];
   BTrial *dtrials = nullptr;
   int *ddone = nullptr;
   unsigned *dqueue = nullptr;
   std::vector<double> a0, astop, cent, sbar;  // per trial
-  // refinement with full covariances, per level (0: every point, 1: the
-  // coarse level's points, covariances times kCoarseStep)
-  BCov *dTC[2] = {nullptr, nullptr};        // per target: covariance
-  Rec3F *dSF[2] = {nullptr, nullptr};       // per source: position and covariance (static)
-  Rec3F *dZF[2] = {nullptr, nullptr};       // per target at each pass: in the trial's source frame
+  // refinement with full covariances, per level (the level's points, their
+  // covariances times the level's point step)
+  BCov *dTC[kLevels] = {};                  // per target: covariance
+  Rec3F *dSF[kLevels] = {};                 // per source: position and covariance (static)
+  Rec3F *dZF[kLevels] = {};                 // per target at each pass: in the trial's source frame
   BRef *dref = nullptr;                     // per trial: optimiser state
   std::vector<int> keepT, keepS;            // kept input indices per trial (level-0 order)
   std::vector<BSrc> hsrc0;                  // host copy of the level-0 sources
@@ -861,10 +866,13 @@ static void free_level(BLevel &L) {
 
 static void free_batch(dugma_batch *b) {
   if (!b) return;
-  free_level(b->lv[0]);
-  free_level(b->lv[1]);
-  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue, b->dTC[0], b->dSF[0], b->dZF[0],
-                  b->dTC[1], b->dSF[1], b->dZF[1], b->dref};
+  for (int l = 0; l < kLevels; ++l) {
+    free_level(b->lv[l]);
+    void *lp[] = {b->dTC[l], b->dSF[l], b->dZF[l]};
+    for (void *p : lp)
+      if (p) (void)hipFree(p);
+  }
+  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue, b->dref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -944,8 +952,8 @@ static int make_items(dugma_batch *b, BLevel &L) {
 }
 
 static int make_all_items(dugma_batch *b) {
-  int rc = make_items(b, b->lv[0]);
-  if (!rc) rc = make_items(b, b->lv[1]);
+  int rc = DUGMA_OK;
+  for (int l = 0; l < kLevels && !rc; ++l) rc = make_items(b, b->lv[l]);
   return rc;
 }
 
@@ -1014,8 +1022,8 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
                        dugma_batch **out) {
   dugma_batch *b = new dugma_batch();
   b->ntrials = ntrials;
-  LevelHost H[2];
-  for (int l = 0; l < 2; ++l) {
+  LevelHost H[kLevels];
+  for (int l = 0; l < kLevels; ++l) {
     BLevel &L = b->lv[l];
     L.tb.assign(ntrials, 0);
     L.nt.assign(ntrials, 0);
@@ -1065,8 +1073,8 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
     if (nt > 0)
       for (int a = 0; a < 3; ++a) c[a] /= nt;
     double r2 = 0.0, vt = 0.0, vs = 0.0;
-    for (int l = 0; l < 2; ++l) {
-      const int step = l == 0 ? 1 : kCoarseStep;
+    for (int l = 0; l < kLevels; ++l) {
+      const int step = kLevelStep[l];
       BLevel &L = b->lv[l];
       LevelHost &h = H[l];
       L.tb[k] = (int)h.tg.size();
@@ -1112,8 +1120,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
     if (e != hipSuccess) rc = hip_fail(e, "hipStreamCreateWithFlags");
   }
   b->hsrc0 = H[0].sr;
-  if (!rc) rc = upload_level(b, b->lv[0], H[0]);
-  if (!rc) rc = upload_level(b, b->lv[1], H[1]);
+  for (int l = 0; l < kLevels && !rc; ++l) rc = upload_level(b, b->lv[l], H[l]);
   if (!rc) rc = bmalloc(&b->dtrials, (size_t)ntrials);
   if (!rc) rc = bmalloc(&b->ddone, (size_t)ntrials);
   if (!rc) rc = bmalloc(&b->dqueue, 1);
@@ -1298,7 +1305,7 @@ static int start_trials(dugma_batch *b, const double *R0, const double *t0, bool
   std::vector<int> done(K, 0);
   DUGMA_HIP(hipMemcpyAsync(b->dtrials, tr.data(), sizeof(BTrial) * K, hipMemcpyHostToDevice, b->stream));
   DUGMA_HIP(hipMemcpyAsync(b->ddone, done.data(), sizeof(int) * K, hipMemcpyHostToDevice, b->stream));
-  for (int l = 0; l < 2; ++l)
+  for (int l = 0; l < kLevels; ++l)
     if (b->lv[l].ntgt > 0)
       hipLaunchKernelGGL(k_batch_fill, dim3((b->lv[l].ntgt + 255) / 256), dim3(256), 0, b->stream, b->lv[l].dnn,
                          b->lv[l].ntgt, kInfBits);
@@ -1352,13 +1359,13 @@ static int launch_refine(dugma_batch *b, int l, int k0, int k1, const RParams &r
 // max_steps accepted BFGS steps (each line search needs one pass when its
 // first step is accepted).
 // Optimiser states for a refinement on level l from the poses in dtrials,
-// every done flag cleared.  carry (optional): the coarse level's final
-// states; a trial whose inverse Hessian was scaled there starts from it
-// divided by kCoarseStep (the coarse covariances are kCoarseStep times the
-// fine ones, so the curvature of -log E is about that much lower), so that
-// its first step on the fine level is a quasi-Newton step rather than a
+// every done flag cleared.  carry (optional): the final states of the
+// previous level, carry_level; a trial whose inverse Hessian was scaled
+// there starts from it times kLevelStep[l] / kLevelStep[carry_level] (the
+// covariances scale with the point step, and the curvature of -log E
+// inversely), so that its first step is a quasi-Newton step rather than a
 // kernel width.
-static int init_refine(dugma_batch *b, int l, const std::vector<BRef> *carry = nullptr) {
+static int init_refine(dugma_batch *b, int l, const std::vector<BRef> *carry = nullptr, int carry_level = 0) {
   const int K = b->ntrials;
   std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
@@ -1370,11 +1377,12 @@ static int init_refine(dugma_batch *b, int l, const std::vector<BRef> *carry = n
     for (int a = 0; a < 9; ++a) r.cR[a] = r.R[a] = tr[k].R[a];
     for (int a = 0; a < 3; ++a) r.ct[a] = r.t[a] = tr[k].t[a];
     r.L = b->refL[k];
-    // the first step is a kernel width: twice as wide on the coarse level
-    r.w0 = b->refW0[k] * (l == 1 ? std::sqrt((double)kCoarseStep) : 1.0);
+    // the first step is a kernel width: wider on the sparser levels
+    r.w0 = b->refW0[k] * std::sqrt((double)kLevelStep[l]);
     r.smax = 0.25 * r.L;
     if (carry && (*carry)[k].state == 1 && (*carry)[k].scaled) {
-      for (int a = 0; a < 36; ++a) r.H[a] = (*carry)[k].H[a] / kCoarseStep;
+      const double f = (double)kLevelStep[l] / kLevelStep[carry_level];
+      for (int a = 0; a < 36; ++a) r.H[a] = (*carry)[k].H[a] * f;
       r.scaled = 1;
       r.warm = 1;
     }
@@ -1387,9 +1395,9 @@ static int init_refine(dugma_batch *b, int l, const std::vector<BRef> *carry = n
 }
 
 static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int max_steps,
-                      const std::vector<BRef> *carry = nullptr) {
+                      const std::vector<BRef> *carry = nullptr, int carry_level = 0) {
   const int K = b->ntrials;
-  int rc = init_refine(b, l, carry);
+  int rc = init_refine(b, l, carry, carry_level);
   if (rc) return rc;
   RParams rp;
   memset(&rp, 0, sizeof(rp));
@@ -1459,9 +1467,9 @@ static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int m
   return rc;
 }
 
-// After a coarse refinement: each trial whose start pose was evaluated
-// continues from the accepted pose, its steps counted with the isotropic
-// ones; ref receives the coarse level's final optimiser states.
+// After a refinement on a sparser level: each trial whose start pose was
+// evaluated continues from the accepted pose, its steps counted with the
+// isotropic ones; ref receives that level's final optimiser states.
 static int fold_refined_poses(dugma_batch *b, std::vector<BRef> &ref) {
   const int K = b->ntrials;
   std::vector<BTrial> tr(K);
@@ -1490,6 +1498,8 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   if (opts->refine_iters < 0) return fail(DUGMA_ERR_ARG, "dugma_batch_register: refine_iters must be >= 0");
   if (opts->refine_coarse_iters < 0)
     return fail(DUGMA_ERR_ARG, "dugma_batch_register: refine_coarse_iters must be >= 0");
+  if (opts->refine_mid_iters < 0)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_register: refine_mid_iters must be >= 0");
   if (opts->refine_iters > 0 && !b->dref)
     return fail(DUGMA_ERR_STATE, "dugma_batch_register: refine_iters needs dugma_batch_set_covariances first");
   const int K = b->ntrials;
@@ -1600,13 +1610,24 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   // without a refinement, E is the isotropic energy at the returned pose
   if (opts->refine_iters == 0) rc = energy_pass(b);
   if (rc) return rc;
-  std::vector<BRef> coarse_ref;
-  if (opts->refine_iters > 0 && opts->refine_coarse_iters > 0) {
-    rc = run_refine(b, opts, 1, opts->refine_coarse_iters);
-    if (!rc) rc = fold_refined_poses(b, coarse_ref);
+  // the refinement levels in order: coarse (every 4th point), middle (every
+  // 2nd), full clouds; each starts from the previous one's pose and inverse
+  // Hessian
+  if (opts->refine_iters > 0) {
+    const int steps[kLevels] = {opts->refine_iters, opts->refine_coarse_iters, opts->refine_mid_iters};
+    const int seq[3] = {1, 2, 0};
+    std::vector<BRef> prev;
+    int prev_level = 0;
+    for (int q = 0; q < 3 && !rc; ++q) {
+      const int l = seq[q];
+      if (steps[l] <= 0) continue;
+      rc = run_refine(b, opts, l, steps[l], prev.empty() ? nullptr : &prev, prev_level);
+      if (!rc && l != 0) {
+        rc = fold_refined_poses(b, prev);
+        prev_level = l;
+      }
+    }
   }
-  if (!rc && opts->refine_iters > 0)
-    rc = run_refine(b, opts, 0, opts->refine_iters, coarse_ref.empty() ? nullptr : &coarse_ref);
   if (rc) return rc;
   std::vector<BTrial> tr(K);
   DUGMA_HIP(hipMemcpyAsync(tr.data(), b->dtrials, sizeof(BTrial) * K, hipMemcpyDeviceToHost, b->stream));
@@ -1693,42 +1714,49 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
     refL[k] = r2 > 0 ? std::sqrt(r2) : 1.0;
     refW0[k] = var > 0 ? std::sqrt(2.0 * var) : 1e-3 * refL[k];
   }
-  // the coarse level's points are every kCoarseStep-th of each trial's
-  // (in level-0 order); their covariances times kCoarseStep (exact)
-  const BLevel &L1 = b->lv[1];
-  std::vector<BCov> tc1((size_t)L1.ntgt);
-  std::vector<Rec3F> sf1((size_t)L1.nsrc);
-  const float cs = (float)kCoarseStep;
-  for (int k = 0; k < K; ++k) {
-    for (int q = 0; q < L1.nt[k]; ++q) {
-      BCov c = tc[L.tb[k] + (size_t)q * kCoarseStep];
-      c.s11 *= cs; c.s12 *= cs; c.s13 *= cs; c.s22 *= cs; c.s23 *= cs; c.s33 *= cs;
-      tc1[L1.tb[k] + q] = c;
-    }
-    for (int q = 0; q < L1.ns[k]; ++q) {
-      Rec3F r = sf[L.sb[k] + (size_t)q * kCoarseStep];
-      r.s11 *= cs; r.s12 *= cs; r.s13 *= cs; r.s22 *= cs; r.s23 *= cs; r.s33 *= cs;
-      sf1[L1.sb[k] + q] = r;
+  // the sparser levels' points are every kLevelStep[l]-th of each trial's
+  // (in level-0 order); their covariances times kLevelStep[l] (exact: a
+  // power of two)
+  std::vector<BCov> tcl[kLevels];
+  std::vector<Rec3F> sfl[kLevels];
+  for (int l = 1; l < kLevels; ++l) {
+    const BLevel &Ll = b->lv[l];
+    const int step = kLevelStep[l];
+    const float cs = (float)step;
+    tcl[l].resize((size_t)Ll.ntgt);
+    sfl[l].resize((size_t)Ll.nsrc);
+    for (int k = 0; k < K; ++k) {
+      for (int q = 0; q < Ll.nt[k]; ++q) {
+        BCov c = tc[L.tb[k] + (size_t)q * step];
+        c.s11 *= cs; c.s12 *= cs; c.s13 *= cs; c.s22 *= cs; c.s23 *= cs; c.s33 *= cs;
+        tcl[l][Ll.tb[k] + q] = c;
+      }
+      for (int q = 0; q < Ll.ns[k]; ++q) {
+        Rec3F r = sf[L.sb[k] + (size_t)q * step];
+        r.s11 *= cs; r.s12 *= cs; r.s13 *= cs; r.s22 *= cs; r.s23 *= cs; r.s33 *= cs;
+        sfl[l][Ll.sb[k] + q] = r;
+      }
     }
   }
-  if (!b->dTC[0]) {
-    int rc;
-    if ((rc = bmalloc(&b->dTC[0], tc.size())) || (rc = bmalloc(&b->dSF[0], sf.size())) ||
-        (rc = bmalloc(&b->dZF[0], tc.size())) || (rc = bmalloc(&b->dTC[1], tc1.size())) ||
-        (rc = bmalloc(&b->dSF[1], sf1.size())) || (rc = bmalloc(&b->dZF[1], tc1.size())) ||
-        (rc = bmalloc(&b->dref, (size_t)K)))
-      return rc;
+  tcl[0].swap(tc);
+  sfl[0].swap(sf);
+  {
+    // allocated once (sizes are fixed by the batch); a buffer is allocated
+    // only where a previous call left none
+    int rc = b->dref ? DUGMA_OK : bmalloc(&b->dref, (size_t)K);
+    for (int l = 0; l < kLevels && !rc; ++l) {
+      if (!b->dTC[l] && (rc = bmalloc(&b->dTC[l], tcl[l].size()))) break;
+      if (!b->dSF[l] && (rc = bmalloc(&b->dSF[l], sfl[l].size()))) break;
+      if (!b->dZF[l] && (rc = bmalloc(&b->dZF[l], tcl[l].size()))) break;
+    }
+    if (rc) return rc;
   }
-  struct Up {
-    void *d;
-    const void *h;
-    size_t n;
-  } ups[] = {{b->dTC[0], tc.data(), tc.size() * sizeof(BCov)},
-             {b->dSF[0], sf.data(), sf.size() * sizeof(Rec3F)},
-             {b->dTC[1], tc1.data(), tc1.size() * sizeof(BCov)},
-             {b->dSF[1], sf1.data(), sf1.size() * sizeof(Rec3F)}};
-  for (const Up &u : ups)
-    if (u.n) DUGMA_HIP(hipMemcpy(u.d, u.h, u.n, hipMemcpyHostToDevice));
+  for (int l = 0; l < kLevels; ++l) {
+    if (!tcl[l].empty())
+      DUGMA_HIP(hipMemcpy(b->dTC[l], tcl[l].data(), tcl[l].size() * sizeof(BCov), hipMemcpyHostToDevice));
+    if (!sfl[l].empty())
+      DUGMA_HIP(hipMemcpy(b->dSF[l], sfl[l].data(), sfl[l].size() * sizeof(Rec3F), hipMemcpyHostToDevice));
+  }
   b->refL = refL;
   b->refW0 = refW0;
   return DUGMA_OK;
@@ -1805,9 +1833,10 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
   double cur_it = 0.0, cur_ref = 0.0, best_it = 0.0, best_ref = 0.0;
   int lvl = 0;
   auto timed = [&](double *out) {
-    cur_ref = 0.0;
+    cur_it = cur_ref = 0.0;
     const int reps = lvl == 0 ? 2 : 6;  // a coarse pass is ~1/10 of a fine one
-    int r = time_iteration(b, lvl, R0, t0, sigma, reps, ev0, ev1, &cur_it);
+    int r = DUGMA_OK;
+    if (lvl != 2) r = time_iteration(b, lvl, R0, t0, sigma, reps, ev0, ev1, &cur_it);  // no iterations on level 2
     if (!r && b->dTC[lvl]) r = time_refine_pass(b, lvl, R0, t0, reps, ev0, ev1, &cur_ref);
     *out = cur_it + cur_ref;
     return r;
@@ -1818,9 +1847,15 @@ extern "C" int dugma_batch_autotune(dugma_batch *b, const double *R0, const doub
     best_ref = cur_ref;
   };
   double fine_it = 0.0, fine_ref = 0.0;
-  for (lvl = 0; lvl < 2 && !rc; ++lvl) {
+  for (lvl = 0; lvl < kLevels && !rc; ++lvl) {
     BLevel &L = b->lv[lvl];
     if (L.nitems == 0) continue;
+    if (lvl == 2 && !b->dTC[2]) {  // refinement only: keep the full clouds' shape
+      L.waves = b->lv[0].waves;
+      for (int c = 0; c < kClasses; ++c) L.chunks[c] = b->lv[0].chunks[c];
+      L.grid_waves = b->lv[0].grid_waves;
+      continue;
+    }
     best = 0.0;
     bool present[kClasses] = {false, false, false, false};
     for (int k = 0; k < b->ntrials; ++k)
@@ -1912,8 +1947,8 @@ extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunk
 }
 
 extern "C" int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves, int *chunks, int *grid_waves) {
-  if (!b || (level != 0 && level != 1))
-    return fail(DUGMA_ERR_ARG, "dugma_batch_get_shape: null batch or level not 0 / 1");
+  if (!b || level < 0 || level >= kLevels)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_get_shape: null batch or level not 0, 1 or 2");
   const BLevel &L = b->lv[level];
   if (waves) *waves = L.waves;
   if (chunks)

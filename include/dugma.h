@@ -299,7 +299,9 @@ typedef struct {
                                 every 4th target and source point, with their covariances
                                 times 4 (the sampling density is a quarter, so a surface
                                 covariance spans twice the distance), then the full clouds
-                                from the pose reached */
+                                from the pose reached and the inverse Hessian / 4 */
+  int refine_mid_iters;      /* with refine_iters > 0: after the coarse level (if any), at most
+                                this many steps on every 2nd point, covariances times 2 */
 } dugma_batch_opts;
 
 /* Arrays of ntrials pointers, each in the layout of dugma_init. */

@@ -351,13 +351,13 @@ def test_refine_energy_matches_oracle(c4_cov_trials, oracle):
         assert abs(out["E"][k] / ref - 1) <= 1e-5, (C4_REFINE[k], out["E"][k], ref)
 
 
-@pytest.mark.parametrize("refine_coarse", [0, 8])
-def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials, refine_coarse):
+@pytest.mark.parametrize("refine_coarse, refine_mid", [(0, 0), (8, 0), (8, 6), (0, 6)])
+def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials, refine_coarse, refine_mid):
     trials = [c4_cov_trials[k] for k in C4_REFINE]
     b = dg.Batch(trials)
     try:
         kw = dict(max_iter=40, anneal=True, sigma_update=True, coarse_iters=10, refine_iters=20,
-                  refine_coarse_iters=refine_coarse, tol_rot=1e-4, tol_trans=1e-4)
+                  refine_coarse_iters=refine_coarse, refine_mid_iters=refine_mid, tol_rot=1e-4, tol_trans=1e-4)
         g = b.register(use_graph=True, check_every=7, **kw)
         p = b.register(use_graph=False, **kw)
         o = b.register(one_trial_per_launch=True, **kw)
@@ -374,14 +374,14 @@ def test_refine_graph_equals_one_trial_per_launch(c4_cov_trials, refine_coarse):
 def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedule):
     """Then the refinement -- after 300 full-cloud isotropic iterations, or
     (the bench's schedule) straight from the coarse isotropic phase through
-    40 coarse refinement steps: every trial ends where the FP64 oracle's pose
-    gradient (surface covariances, all points) vanishes next to its size one
-    degree / 0.01 units away, and within 2 degrees of the ground truth in at
-    least 80% of these trials (500-8,000 points, up to 30% outliers, 0-90
-    degree starts)."""
+    60 coarse and 20 middle-level refinement steps: every trial ends where
+    the FP64 oracle's pose gradient (surface covariances, all points)
+    vanishes next to its size one degree / 0.01 units away, and within 2
+    degrees of the ground truth in at least 80% of these trials (500-8,000
+    points, up to 30% outliers, 0-90 degree starts)."""
     trials = [c4_cov_trials[k] for k in C4_REFINE]
     sched = (dict(max_iter=460, refine_coarse_iters=0) if schedule == "full"
-             else dict(max_iter=160, refine_coarse_iters=40))
+             else dict(max_iter=160, refine_coarse_iters=60, refine_mid_iters=20))
     b = dg.Batch(trials)
     try:
         out = b.register(anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9, coarse_iters=160,
@@ -450,5 +450,7 @@ def test_refine_argument_errors():
             b.register(max_iter=3, refine_iters=-1)
         with pytest.raises(dg.DugmaError, match="refine_coarse_iters"):
             b.register(max_iter=3, refine_iters=2, refine_coarse_iters=-1)
+        with pytest.raises(dg.DugmaError, match="refine_mid_iters"):
+            b.register(max_iter=3, refine_iters=2, refine_mid_iters=-1)
     finally:
         b.free()
