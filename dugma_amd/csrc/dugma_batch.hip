@@ -694,8 +694,24 @@ __device__ bool ref_direction(BRef &r) {
   return true;
 }
 
+// The step ref_direction just proposed, from an inverse Hessian scaled by a
+// curvature pair (here or on the previous level), is already below the
+// tolerances: the trial has converged at its accepted pose, without a pass
+// to evaluate that step.
+__device__ bool next_step_converged(const BRef &r, const RParams &rp) {
+  if (!r.scaled) return false;
+  double pa = 0.0, pt = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    pa += r.d[a] * r.d[a];
+    pt += r.d[3 + a] * r.d[3 + a];
+  }
+  return r.alpha * sqrt(pa) / r.L < rp.tol_rot && r.alpha * sqrt(pt) < rp.tol_trans;
+}
+
 // One thread per trial: the pass's energy and gradient at the candidate
-// pose, then the optimiser's next move (dugma_register_energy's logic).
+// pose, then the optimiser's next move (dugma_register_energy's logic, and
+// a trial also stops when its next quasi-Newton step is below the
+// tolerances).
 __global__ __launch_bounds__(256) void k_batchf_update(
     const double *__restrict__ part, const int *__restrict__ titem, int k0, int k1,
     BRef *__restrict__ ref, int *__restrict__ done, unsigned *__restrict__ queue, RParams rp) {
@@ -734,6 +750,9 @@ __global__ __launch_bounds__(256) void k_batchf_update(
       r.state = 1;
       if (rp.max_iter < 1 || !ref_direction(r)) {
         r.status = rp.max_iter < 1 ? 2 : 1;
+        finished = true;
+      } else if (next_step_converged(r, rp)) {  // a carried-over H already says so
+        r.status = 1;
         finished = true;
       }
     }
@@ -779,6 +798,9 @@ __global__ __launch_bounds__(256) void k_batchf_update(
       r.status = 2;
       finished = true;
     } else if (!ref_direction(r)) {
+      r.status = 1;
+      finished = true;
+    } else if (next_step_converged(r, rp)) {
       r.status = 1;
       finished = true;
     }
