@@ -538,6 +538,19 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
     dn = std::sqrt(dn);
     double alpha = scaled ? 1.0 : w0 / dn;
     if (alpha * dn > step_max) alpha = step_max / dn;
+    if (scaled) {
+      // the quasi-Newton step is already below the tolerances: converged at
+      // the current pose, without a pass to evaluate that step
+      double pa = 0.0, pt = 0.0;
+      for (int a = 0; a < np; ++a) {
+        const double v = alpha * d[a];
+        if (a < nr)
+          pa += (v / L) * (v / L);
+        else
+          pt += v * v;
+      }
+      if (std::sqrt(pa) < op.tol_rot && pt < op.tol_trans) break;
+    }
     const double f = -std::log(E);
     double s[6], Rn[9], tn[3], En = 0.0, gn[6];
     bool ok = false;

@@ -149,9 +149,11 @@ int dugma_energy_grad_noalloc(dugma_ctx *ctx, const double *R, const double *t,
  * compi surrogate, DUGMA.m:57-123).  Rotation parameters are scaled by the
  * RMS radius of the source cloud so that both blocks are lengths; the first
  * step moves by about one kernel width.  Stops when a step rotates by less
- * than tol_rot (rad) and translates by less than sqrt(tol_trans), when no
- * decrease is found along the search direction, or after max_iter accepted
- * steps.  iterations = accepted steps; E = energy at the returned pose.
+ * than tol_rot (rad) and translates by less than sqrt(tol_trans) -- an
+ * accepted step, or the next quasi-Newton step once the inverse Hessian has
+ * been scaled (then without evaluating it) -- when no decrease is found
+ * along the search direction, or after max_iter accepted steps.
+ * iterations = accepted steps; E = energy at the returned pose.
  * opts may be NULL (max_iter 200, tol_rot 1e-7, tol_trans 1e-14, AUTO). */
 typedef struct {
   int max_iter;
@@ -294,7 +296,9 @@ typedef struct {
   int refine_iters;          /* > 0: then at most this many accepted BFGS steps per trial
                                 on -log E with the full covariances of
                                 dugma_batch_set_covariances (one fused energy + pose-gradient
-                                pass per trial pose; stops at tol_rot / tol_trans) */
+                                pass per trial pose; stops when an accepted step, or the next
+                                quasi-Newton step once the inverse Hessian is scaled, is below
+                                tol_rot / tol_trans) */
   int refine_coarse_iters;   /* with refine_iters > 0: first at most this many BFGS steps on
                                 every 4th target and source point, with their covariances
                                 times 4 (the sampling density is a quarter, so a surface
