@@ -418,7 +418,8 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
     sub = trials[:baseline_trials]
     b1 = dg.Batch(sub)
     try:
-        b1.set_shape(shape["waves"], shape["chunks"], shape["grid_waves"])  # same sums as the batched run
+        for lvl, sh in enumerate(shape["levels"]):  # same sums as the batched run
+            b1.set_level_shape(lvl, sh["waves"], sh["chunks"], sh["grid_waves"])
         b1.register(max_iter=2, one_trial_per_launch=True, **BATCH_MODE)
         t2 = time.perf_counter()
         out1 = b1.register(max_iter=max_iter, one_trial_per_launch=True, **BATCH_MODE)

@@ -400,12 +400,21 @@ class Batch:
         _lib.check(_lib.load().dugma_batch_autotune(self._ptr, _dp(R0c), _dp(t0c), int(bool(sigma_update)),
                                                     ctypes.byref(w), ch, ctypes.byref(g), ctypes.byref(ms),
                                                     ctypes.byref(rms)))
-        cw, cg = ctypes.c_int(), ctypes.c_int()
-        cch = (ctypes.c_int * 4)()
-        _lib.check(_lib.load().dugma_batch_get_shape(self._ptr, 1, ctypes.byref(cw), cch, ctypes.byref(cg)))
+        levels = [self.get_shape(l) for l in range(3)]
         return dict(waves=w.value, chunks=list(ch), grid_waves=g.value, ms_per_iteration=ms.value,
-                    ms_per_refine_pass=rms.value, coarse_waves=cw.value, coarse_chunks=list(cch),
-                    coarse_grid_waves=cg.value)
+                    ms_per_refine_pass=rms.value, levels=levels)
+
+    def get_shape(self, level):
+        """dugma_batch_get_shape: one level's launch shape (0 full clouds,
+        1 every 4th point, 2 every 2nd point)."""
+        w, g = ctypes.c_int(), ctypes.c_int()
+        ch = (ctypes.c_int * 4)()
+        _lib.check(_lib.load().dugma_batch_get_shape(self._ptr, int(level), ctypes.byref(w), ch, ctypes.byref(g)))
+        return dict(waves=w.value, chunks=list(ch), grid_waves=g.value)
+
+    def set_level_shape(self, level, waves, chunks, grid_waves):
+        ch = (ctypes.c_int * 4)(*[int(c) for c in chunks])
+        _lib.check(_lib.load().dugma_batch_set_level_shape(self._ptr, int(level), int(waves), ch, int(grid_waves)))
 
     def set_shape(self, waves, chunks, grid_waves):
         ch = (ctypes.c_int * 4)(*[int(c) for c in chunks])

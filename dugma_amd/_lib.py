@@ -70,6 +70,7 @@ SIGNATURES = {
     "dugma_batch_free": (_ci, [_vp]),
     "dugma_batch_autotune": (_ci, [_vp, _d, _d, _ci, _i, _i, _i, _d, _d]),
     "dugma_batch_get_shape": (_ci, [_vp, _ci, _i, _i, _i]),
+    "dugma_batch_set_level_shape": (_ci, [_vp, _ci, _ci, _i, _ci]),
     "dugma_batch_set_shape": (_ci, [_vp, _ci, _i, _ci]),
     "dugma_ctx_handles": (_ci, [_vp, ctypes.POINTER(_vp)]),
     "dugma_read_handle": (_ci, [_vp, _ci, _f]),

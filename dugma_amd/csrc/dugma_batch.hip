@@ -1968,6 +1968,21 @@ extern "C" int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunk
   return make_all_items(b);
 }
 
+extern "C" int dugma_batch_set_level_shape(dugma_batch *b, int level, int waves, const int *chunks,
+                                           int grid_waves) {
+  if (!b || level < 0 || level >= kLevels || (waves != 1 && waves != 2 && waves != 4) || !chunks ||
+      grid_waves < 1)
+    return fail(DUGMA_ERR_ARG, "dugma_batch_set_level_shape: level 0, 1 or 2, waves 1, 2 or 4, chunks >= 1, "
+                               "grid_waves >= 1");
+  for (int c = 0; c < kClasses; ++c)
+    if (chunks[c] < 1) return fail(DUGMA_ERR_ARG, "dugma_batch_set_level_shape: chunks must be >= 1");
+  BLevel &L = b->lv[level];
+  L.waves = waves;
+  for (int c = 0; c < kClasses; ++c) L.chunks[c] = chunks[c];
+  L.grid_waves = grid_waves;
+  return make_items(b, L);
+}
+
 extern "C" int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves, int *chunks, int *grid_waves) {
   if (!b || level < 0 || level >= kLevels)
     return fail(DUGMA_ERR_ARG, "dugma_batch_get_shape: null batch or level not 0, 1 or 2");

@@ -353,13 +353,18 @@ int dugma_batch_info(const dugma_batch *b, int *ntrials, int *nitems,
  * fastest shapes
  * and reports the full clouds' one with its time per isotropic iteration
  * (ms) and per refinement pass (refine_ms, 0 without covariances); any
- * output may be NULL.  dugma_batch_set_shape gives both levels the same
- * shape; dugma_batch_get_shape reports a level's (chunks: 4 ints). */
+ * output may be NULL.  dugma_batch_set_shape gives every level the same
+ * shape, dugma_batch_set_level_shape one level (0, 1, or 2: every 2nd
+ * point, refinement only) its own; dugma_batch_get_shape reports a level's
+ * (chunks: 4 ints).  The shapes change how the pair sums are grouped, so
+ * two batches give bit-identical results only with the same shapes. */
 int dugma_batch_autotune(dugma_batch *b, const double *R0, const double *t0,
                          int sigma_update, int *waves, int *chunks,
                          int *grid_waves, double *ms, double *refine_ms);
 int dugma_batch_set_shape(dugma_batch *b, int waves, const int *chunks,
                           int grid_waves);
+int dugma_batch_set_level_shape(dugma_batch *b, int level, int waves,
+                                const int *chunks, int grid_waves);
 int dugma_batch_get_shape(const dugma_batch *b, int level, int *waves,
                           int *chunks, int *grid_waves);
 int dugma_batch_free(dugma_batch *b);

@@ -100,6 +100,7 @@ def test_batch_argument_errors_before_any_device_call():
     ch = (ctypes.c_int * 4)(1, 1, 1, 1)
     assert L.dugma_batch_set_shape(None, 2, ch, 8192) == -1
     assert L.dugma_batch_get_shape(None, 0, None, None, None) == -1
+    assert L.dugma_batch_set_level_shape(None, 0, 2, ch, 8192) == -1
 
 
 def test_option_structs_match_the_header(tmp_path):

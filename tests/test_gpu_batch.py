@@ -405,6 +405,34 @@ def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedul
     assert np.mean(np.array(errs) < 2.0) >= 0.8, errs
 
 
+def test_level_shapes_copied_give_identical_results(c4_cov_trials):
+    """Each level has its own launch shape; a second batch of the same trials
+    given the same per-level shapes (set_level_shape) reproduces the first
+    one's results bit for bit, one trial per launch against the graph."""
+    trials = [c4_cov_trials[k] for k in C4_REFINE[:10]]
+    kw = dict(max_iter=30, anneal=True, sigma_update=True, coarse_iters=20, refine_iters=15,
+              refine_coarse_iters=6, refine_mid_iters=4, tol_rot=1e-4, tol_trans=1e-4)
+    b = dg.Batch(trials)
+    try:
+        b.set_level_shape(1, 2, [1, 2, 2, 4], 4096)
+        b.set_level_shape(2, 1, [2, 1, 1, 1], 8192)
+        assert b.get_shape(1) == dict(waves=2, chunks=[1, 2, 2, 4], grid_waves=4096)
+        assert b.get_shape(2) == dict(waves=1, chunks=[2, 1, 1, 1], grid_waves=8192)
+        shapes = [b.get_shape(l) for l in range(3)]
+        g = b.register(use_graph=True, check_every=4, **kw)
+    finally:
+        b.free()
+    b1 = dg.Batch(trials)
+    try:
+        for l, sh in enumerate(shapes):
+            b1.set_level_shape(l, sh["waves"], sh["chunks"], sh["grid_waves"])
+        o = b1.register(one_trial_per_launch=True, **kw)
+    finally:
+        b1.free()
+    for key in ("R", "t", "iters", "status"):
+        np.testing.assert_array_equal(g[key], o[key])
+
+
 def test_tail_queue_keeps_results():
     """Once at most a sixteenth of the trials run, passes use a queue of the
     running trials' items only (rebuilt as that count halves); 48 trials of
