@@ -58,7 +58,7 @@ FLOP_3D_ISO_FWD = int(_RC["FLOP_3D_ISO_FWD"])
 # 200,000, the autotuned launch): rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
 # each scaled by what the counter reads for a copy kernel of known size in the
 # same process (profiles/k_pairs_traffic_200000x200000_iso.txt).
-PROFILED_TRAFFIC_200K_ISO = 566933787
+PROFILED_TRAFFIC_200K_ISO = 117246982
 TRAFFIC_PLAN = {"lane_points": 1, "target_waves": 524288}  # the launch those counters measured
 TRAFFIC_PROFILE = "profiles/k_pairs_traffic_200000x200000_iso.txt"
 

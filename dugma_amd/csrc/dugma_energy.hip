@@ -44,17 +44,50 @@ struct PoseArg {
 // With one stationary point per lane every variant fits in 64 VGPRs (no
 // scratch), which allows 8 waves per SIMD instead of 6; two per lane need
 // more registers and keep the compiler's default.
+//
+// Tiles are (stationary block, streamed chunk) pairs.  With xcd_map set
+// they lie on a 1-D grid mapped XCD-aware: workgroups are dispatched to the 8
+// XCDs round-robin, so XCD x runs workgroups x, x + 8, ...; each XCD takes a
+// contiguous eighth of the stationary blocks (bpx of them) and walks every
+// streamed chunk across them, chunk by chunk.  A chunk is then fetched into
+// an XCD's L2 once and used by its bpx blocks, and the XCD's stationary
+// records (an eighth of them) stay in that L2 for every chunk.  The grid is
+// padded to 8 x bpx x qchunks workgroups; the padding exits at once.
+// Without xcd_map (clouds that fit one L2 anyway) the grid is 2-D:
+// block blockIdx.x of chunk blockIdx.y.
+constexpr int kXcds = 8;
+constexpr size_t kXcdMapBytes = size_t(4) << 20;  // both clouds' records beyond one XCD's L2 (4 MiB)
+__device__ __forceinline__ bool xcd_tile(int pblocks, int qchunks, int xcd_map, int *sb, int *qc) {
+  if (!xcd_map) {
+    *sb = blockIdx.x;
+    *qc = blockIdx.y;
+    return true;
+  }
+  const int g = blockIdx.x;
+  const int bpx = (pblocks + kXcds - 1) / kXcds;
+  const int xcd = g % kXcds, local = g / kXcds;
+  *sb = xcd * bpx + local % bpx;
+  *qc = local / bpx;
+  return *sb < pblocks && *qc < qchunks;
+}
+static inline dim3 tile_grid(int pblocks, int qchunks, int xcd_map) {
+  return xcd_map ? dim3(kXcds * ((pblocks + kXcds - 1) / kXcds) * qchunks) : dim3(pblocks, qchunks);
+}
+
 template <class Rec, int MODE, int T, int COMP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ? 8 : 1, 8))) void k_pairs(
     const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
-    int qchunk, double *__restrict__ out, double t0, double t1, double t2) {
+    int qchunk, int pblocks, int qchunks, int xcd_map, double *__restrict__ out, double t0, double t1,
+    double t2) {
   using Tr = RecTraits<Rec>;
   constexpr bool GRAD = MODE != kModeEnergy;
   constexpr int NA = GRAD ? Tr::NACC : 1;
+  int sb, qc;
+  if (!xcd_tile(pblocks, qchunks, xcd_map, &sb, &qc)) return;  // grid padding (whole workgroup)
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int pbase = blockIdx.x * (kBlock * T) + wave * (64 * T) + lane;
-  const int q0 = blockIdx.y * qchunk;
+  const int pbase = sb * (kBlock * T) + wave * (64 * T) + lane;
+  const int q0 = qc * qchunk;
   const int q1 = min(q0 + qchunk, nQ);
 
   Rec p[T];
@@ -114,7 +147,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
     for (int k = 0; k < T; ++k) {
       const int pi = pbase + 64 * k;
       if (pi < nP) {
-        double *o = out + ((size_t)blockIdx.y * nP + pi) * NA;
+        double *o = out + ((size_t)qc * nP + pi) * NA;
 #pragma unroll
         for (int a = 0; a < NA; ++a) o[a] = dacc[k][a];
       }
@@ -135,7 +168,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
         r[0] += dacc[k][0];
     }
   }
-  double *o = out + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPosePart;
+  double *o = out + ((size_t)qc * pblocks + sb) * kPosePart;
   if (GRAD) {
     // Workgroup sums of the kPosePart values in a fixed order: every lane's
     // values go to LDS; thread (a, k) = (tid / 32, tid % 32) adds entries
@@ -334,6 +367,9 @@ static Plan make_plan(int nP, int nQ, int mode, int T, int target_waves, int max
   int qc = (target_waves + waves_p - 1) / waves_p;
   int maxc = (nQ + kMinChunk - 1) / kMinChunk;
   if (mode == kModePoint) maxc = std::min(maxc, max_chunks);
+  // grid limits: 65,535 chunks on the 2-D grid, 2^31 - 1 threads on the
+  // padded 1-D (XCD-mapped) one
+  maxc = std::min({maxc, 65535, (int)(INT32_MAX / kBlock / (pl.pblocks + kXcds))});
   qc = std::max(1, std::min(qc, maxc));
   pl.qchunk = (nQ + qc - 1) / qc;
   pl.qchunks = (nQ + pl.qchunk - 1) / pl.qchunk;
@@ -355,14 +391,15 @@ static Plan ctx_plan(const dugma_ctx *c, int nP, int nQ, int mode, bool iso) {
 template <class Rec, int MODE, int COMP>
 static void launch_pairs_c(const Plan &pl, const Rec *P, int nP, const Rec *Q, int nQ,
                            double *out, const double *t, hipStream_t s) {
-  dim3 grid(pl.pblocks, pl.qchunks), block(kBlock);
+  const int xcd_map = (size_t)(nP + nQ) * sizeof(Rec) > kXcdMapBytes ? 1 : 0;
+  const dim3 grid = tile_grid(pl.pblocks, pl.qchunks, xcd_map), block(kBlock);
   const double t0 = t[0], t1 = t[1], t2 = (RecTraits<Rec>::D == 3) ? t[2] : 0.0;
   if (pl.T == 2)
     hipLaunchKernelGGL((k_pairs<Rec, MODE, 2, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
-                       out, t0, t1, t2);
+                       pl.pblocks, pl.qchunks, xcd_map, out, t0, t1, t2);
   else
     hipLaunchKernelGGL((k_pairs<Rec, MODE, 1, COMP>), grid, block, 0, s, P, nP, Q, nQ, pl.qchunk,
-                       out, t0, t1, t2);
+                       pl.pblocks, pl.qchunks, xcd_map, out, t0, t1, t2);
 }
 
 // comp: 0 none, 1 streamed side carries the lo part, 2 stationary side does.
