@@ -215,7 +215,7 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     unsigned *__restrict__ nn, double *__restrict__ part) {
   __shared__ int s_p;
   __shared__ double red[W][kItemStat];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (;;) {
     if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
     __syncthreads();
@@ -232,8 +232,11 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
     double dE = 0.0, dW = 0.0, dD[3] = {0.0, 0.0, 0.0};
     float dmin = __builtin_huge_valf();
     const int s_end = it.s0 + it.sn;
+    // a wave past the tile's last target skips the pair loop (its sums stay
+    // zero, as its masked lanes would have made them)
+    const int s_run = wave * 64 < it.tn ? s_end : it.s0;
     BSrc qa = S[it.s0], qb = S[min(it.s0 + 1, s_end - 1)];  // it.sn >= 1
-    for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
+    for (int j0 = it.s0; j0 < s_run; j0 += kBSub) {
       float fE = 0.0f, fW = 0.0f, fD0 = 0.0f, fD1 = 0.0f, fD2 = 0.0f;
       const int j1 = min(j0 + kBSub, s_end);
 #pragma unroll 2
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(64 * W) void k_batchf_pairs(
     double *__restrict__ part) {
   __shared__ int s_p;
   __shared__ double red[W][7];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (;;) {
     if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
     __syncthreads();
@@ -607,8 +610,9 @@ __global__ __launch_bounds__(64 * W) void k_batchf_pairs(
 #pragma unroll
     for (int a = 0; a < 10; ++a) dacc[a] = 0.0;
     const int s_end = it.s0 + it.sn;
+    const int s_run = wave * 64 < it.tn ? s_end : it.s0;  // idle waves skip the pairs
     Rec3F qa = S[it.s0], qb = S[min(it.s0 + 1, s_end - 1)];  // it.sn >= 1
-    for (int j0 = it.s0; j0 < s_end; j0 += kBSub) {
+    for (int j0 = it.s0; j0 < s_run; j0 += kBSub) {
       float acc[10];
 #pragma unroll
       for (int a = 0; a < 10; ++a) acc[a] = 0.0f;
