@@ -20,7 +20,7 @@
 // calls update_sig before its loop.  Optionally an extra variance a -> 0 is
 // annealed on top.
 //
-// One iteration for all trials is three kernels:
+// One iteration for all trials is four kernels:
 //   k_batch_transform  the targets in each trial's source frame,
 //                      z_i = R_k^T (y_i - (t_k - c_k)) (FP64, rounded to
 //                      FP32), and g s_i + a; the sources stay as given (FP32)
@@ -28,10 +28,11 @@
 //                      chunk) items from a queue ordered longest first and
 //                      write FP64 sufficient statistics per item, plus the
 //                      per-target nearest squared distance
-//   k_batch_update     one workgroup per trial: its nearest-distance sum,
-//                      a fixed-order sum of its items, Horn's quaternion
-//                      solution (4x4 Jacobi, FP64), pose, sigma, annealing,
-//                      convergence; re-arms the queue
+//   k_batch_fold       one workgroup per trial: its nearest-distance sum
+//                      and a fixed-order sum of its items; re-arms the queue
+//   k_batch_update     one thread per trial: Horn's quaternion solution
+//                      (4x4 Jacobi, FP64), pose, sigma, annealing,
+//                      convergence
 // captured once into a hipGraph and replayed.  The same kernels run one
 // trial per launch for the baseline.
 //
@@ -79,7 +80,7 @@ namespace dugma {
 
 constexpr int kBSub = 64;
 constexpr int kStat = 20;         // stride of the per-item statistics (doubles; 17 used)
-constexpr int kUpd = 256;         // threads of the update kernel (one workgroup per trial)
+constexpr int kUpd = 256;         // threads of the fold kernel (one workgroup per trial)
 constexpr int kClasses = 4;       // size classes by source count: < 1000, < 2000, < 4000, more
 constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
 constexpr double kGMin = 1e-6;    // smallest variance scale g
@@ -300,34 +301,45 @@ __global__ __launch_bounds__(64 * W) void k_batch_pairs(
 }
 
 // Largest-eigenvalue eigenvector of a symmetric 4x4 (cyclic Jacobi, FP64).
+// Every index is a compile-time constant (loops unrolled), so A and V stay
+// in registers.
 __device__ void eig4_max(double A[4][4], double q[4]) {
   double V[4][4];
+#pragma unroll
   for (int a = 0; a < 4; ++a)
+#pragma unroll
     for (int b = 0; b < 4; ++b) V[a][b] = a == b ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, tot = 0.0;
+#pragma unroll
     for (int a = 0; a < 4; ++a)
+#pragma unroll
       for (int b = 0; b < 4; ++b) {
         tot += A[a][b] * A[a][b];
         if (a != b) off += A[a][b] * A[a][b];
       }
     if (off <= 1e-30 * tot) break;
+#pragma unroll
     for (int p = 0; p < 3; ++p)
+#pragma unroll
       for (int r = p + 1; r < 4; ++r) {
         if (A[p][r] == 0.0) continue;
         const double th = 0.5 * (A[r][r] - A[p][p]) / A[p][r];
         const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
         const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {  // A <- A J
           const double akp = A[k][p], akr = A[k][r];
           A[k][p] = c * akp - s * akr;
           A[k][r] = s * akp + c * akr;
         }
+#pragma unroll
         for (int k = 0; k < 4; ++k) {  // A <- J^T A
           const double apk = A[p][k], ark = A[r][k];
           A[p][k] = c * apk - s * ark;
           A[r][k] = s * apk + c * ark;
         }
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
           const double vkp = V[k][p], vkr = V[k][r];
           V[k][p] = c * vkp - s * vkr;
@@ -335,25 +347,36 @@ __device__ void eig4_max(double A[4][4], double q[4]) {
         }
       }
   }
-  int best = 0;
-  for (int a = 1; a < 4; ++a)
-    if (A[a][a] > A[best][best]) best = a;
-  for (int a = 0; a < 4; ++a) q[a] = V[a][best];
+  double best = A[0][0];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) q[a] = V[a][0];
+#pragma unroll
+  for (int b = 1; b < 4; ++b)
+    if (A[b][b] > best) {
+      best = A[b][b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) q[a] = V[a][b];
+    }
 }
 
 // One workgroup per trial.  All threads fold the trial's nearest distances
-// (and reset them for the next pass); thread 0 does the rest.
-__global__ __launch_bounds__(kUpd) void k_batch_update(
-    const double *__restrict__ part, const int *__restrict__ titem,
-    const int *__restrict__ ttgt, int k0, int k1, BTrial *__restrict__ trials,
-    int *__restrict__ done, unsigned *__restrict__ queue, unsigned *__restrict__ nn,
-    BParams bp) {
+// (and reset them for the next pass) into sig = mean / 3 (-1: none), and
+// stage the item statistics in LDS a block of items at a time, from which
+// thread a < 17 sums statistic a in item order.  fold[k] receives the 17
+// sums and, at index 17, sig.
+constexpr int kUpdItems = 64;  // items staged per block
+__global__ __launch_bounds__(kUpd) void k_batch_fold(
+    const double *__restrict__ part, const int *__restrict__ titem, const int *__restrict__ ttgt, int k0,
+    int k1, const BTrial *__restrict__ trials, const int *__restrict__ done, unsigned *__restrict__ queue,
+    unsigned *__restrict__ nn, double *__restrict__ fold, int sigma) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
   const int k = k0 + blockIdx.x;
   if (k >= k1 || done[k]) return;
-  __shared__ double s_nn[kUpd / 64];
-  if (bp.sigma) {
+  double *f = fold + (size_t)k * kStat;
+  if (sigma) {
+    __shared__ double s_nn[kUpd / 64];
     double snn = 0.0;
+#pragma unroll 4
     for (int i = ttgt[k] + threadIdx.x; i < ttgt[k + 1]; i += kUpd) {
       snn += (double)__uint_as_float(nn[i]);
       nn[i] = kInfBits;
@@ -362,29 +385,49 @@ __global__ __launch_bounds__(kUpd) void k_batch_update(
     for (int off = 32; off > 0; off >>= 1) snn += __shfl_xor(snn, off);
     if ((threadIdx.x & 63) == 0) s_nn[threadIdx.x >> 6] = snn;
     __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  BTrial tr = trials[k];
-  double sig = -1.0;  // < 0: not measured
-  if (bp.sigma) {
-    double s = 0.0;
-    for (int w = 0; w < kUpd / 64; ++w) s += s_nn[w];
-    const int nt = ttgt[k + 1] - ttgt[k];
-    if (nt > 0 && isfinite(s)) sig = s / (3.0 * nt);
-    if (!tr.sig_ready) {  // the pass before the first iteration only measures sig
-      if (sig >= 0.0) {
-        tr.sig = sig;
-        tr.g = fmax(sig / tr.sbar, kGMin);
-      }
-      tr.sig_ready = 1;
-      trials[k] = tr;
-      return;
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kUpd / 64; ++w) s += s_nn[w];
+      const int nt = ttgt[k + 1] - ttgt[k];
+      f[kItemStat] = (nt > 0 && isfinite(s)) ? s / (3.0 * nt) : -1.0;
     }
   }
+  if (sigma && !trials[k].sig_ready) return;  // this pass only measures sig
+  __shared__ double s_part[kUpdItems * kStat];
+  const int i0 = titem[k], i1 = titem[k + 1];
+  double acc = 0.0;
+  for (int ib = i0; ib < i1; ib += kUpdItems) {
+    const int nb = min(kUpdItems, i1 - ib);
+    for (int e = threadIdx.x; e < nb * kStat; e += kUpd) s_part[e] = part[(size_t)ib * kStat + e];
+    __syncthreads();
+    if (threadIdx.x < kItemStat)
+      for (int i = 0; i < nb; ++i) acc += s_part[i * kStat + threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x < kItemStat) f[threadIdx.x] = acc;
+}
+
+// One thread per trial, after k_batch_fold: the pose step from the folded
+// sums, the sigma update, annealing and convergence.
+__global__ __launch_bounds__(64) void k_batch_update(const double *__restrict__ fold, int k0, int k1,
+                                                     BTrial *__restrict__ trials, int *__restrict__ done,
+                                                     BParams bp) {
+  const int k = k0 + blockIdx.x * 64 + threadIdx.x;
+  if (k >= k1 || done[k]) return;
+  const double *f = fold + (size_t)k * kStat;
+  BTrial tr = trials[k];
+  const double sig = bp.sigma ? f[kItemStat] : -1.0;  // < 0: not measured
+  if (bp.sigma && !tr.sig_ready) {  // the pass before the first iteration only measures sig
+    if (sig >= 0.0) {
+      tr.sig = sig;
+      tr.g = fmax(sig / tr.sbar, kGMin);
+    }
+    tr.sig_ready = 1;
+    trials[k] = tr;
+    return;
+  }
   double st[kStat];
-  for (int a = 0; a < kStat; ++a) st[a] = 0.0;
-  for (int i = titem[k]; i < titem[k + 1]; ++i)
-    for (int a = 0; a < kItemStat; ++a) st[a] += part[(size_t)i * kStat + a];
+  for (int a = 0; a < kStat; ++a) st[a] = a < kItemStat ? f[a] : 0.0;
   const double W = st[1];
   {
     // B' parts from the source frame to the target frame (relative to c):
@@ -868,6 +911,7 @@ struct dugma_batch {
   BLevel lv[kLevels];
   BTrial *dtrials = nullptr;
   int *ddone = nullptr;
+  double *dfold = nullptr;  // per trial: folded item sums and sig of the last pass (kStat doubles)
   unsigned *dqueue = nullptr;
   std::vector<double> a0, astop, cent, sbar;  // per trial
   // refinement with full covariances, per level (the level's points, their
@@ -898,7 +942,7 @@ static void free_batch(dugma_batch *b) {
     for (void *p : lp)
       if (p) (void)hipFree(p);
   }
-  void *ptrs[] = {b->dtrials, b->ddone, b->dqueue, b->dref};
+  void *ptrs[] = {b->dtrials, b->ddone, b->dfold, b->dqueue, b->dref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1149,6 +1193,7 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
   for (int l = 0; l < kLevels && !rc; ++l) rc = upload_level(b, b->lv[l], H[l]);
   if (!rc) rc = bmalloc(&b->dtrials, (size_t)ntrials);
   if (!rc) rc = bmalloc(&b->ddone, (size_t)ntrials);
+  if (!rc) rc = bmalloc(&b->dfold, (size_t)ntrials * kStat);
   if (!rc) rc = bmalloc(&b->dqueue, 1);
   if (!rc) {
     hipError_t e = hipMemset(b->dqueue, 0, sizeof(unsigned));
@@ -1248,8 +1293,10 @@ static int launch_iteration(dugma_batch *b, int l, int k0, int k1, const BParams
       default: launch_pairs<4>(b, L, grid, order, ni, bp.sigma != 0); break;
     }
   }
-  hipLaunchKernelGGL(k_batch_update, dim3(k1 - k0), dim3(kUpd), 0, b->stream, L.dpart, L.dtitem, L.dttgt, k0,
-                     k1, b->dtrials, b->ddone, b->dqueue, L.dnn, bp);
+  hipLaunchKernelGGL(k_batch_fold, dim3(k1 - k0), dim3(kUpd), 0, b->stream, L.dpart, L.dtitem, L.dttgt, k0, k1,
+                     b->dtrials, b->ddone, b->dqueue, L.dnn, b->dfold, bp.sigma);
+  hipLaunchKernelGGL(k_batch_update, dim3((k1 - k0 + 63) / 64), dim3(64), 0, b->stream, b->dfold, k0, k1,
+                     b->dtrials, b->ddone, bp);
   DUGMA_HIP(hipGetLastError());
   return DUGMA_OK;
 }
