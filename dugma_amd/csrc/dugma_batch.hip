@@ -69,6 +69,7 @@
 // dugma_batch_autotune, which times whole iterations on the batch's own data.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -90,8 +91,11 @@ constexpr double kGMin = 1e-6;    // smallest variance scale g
 
 static inline int size_class(int ns) { return ns < 1000 ? 0 : ns < 2000 ? 1 : ns < 4000 ? 2 : 3; }
 
+// A work item: targets [t0, t0 + tn) of a trial against its sources
+// [s0, s0 + sn); s0 sits a multiple of 64 sources into the trial's, and blk
+// is the index of its first 64-source block in the level's block bounds.
 struct BItem {
-  int trial, t0, tn, s0, sn, pad0, pad1, pad2;
+  int trial, t0, tn, s0, sn, blk, pad1, pad2;
 };
 
 struct BTrial {
@@ -547,8 +551,26 @@ __global__ __launch_bounds__(256) void k_batch_energy(const double *__restrict__
 // Refinement: BFGS on -log E with full covariances, one pass per trial pose
 // ---------------------------------------------------------------------------
 struct alignas(16) BCov {  // a target's covariance (FP32, symmetric)
-  float s11, s12, s13, s22, s23, s33, pad0, pad1;
+  float s11, s12, s13, s22, s23, s33;
+  float lam;  // upper bound of its largest eigenvalue
+  float pad;
 };
+
+// A block of 64 consecutive sources of one trial (the last one may be
+// shorter): their bounding box and the largest eigenvalue bound of their
+// covariances.
+struct alignas(16) BBound {
+  float lo[3], lam;
+  float hi[3], pad;
+};
+
+// A wave skips a block of sources when, for every pair, the exponent
+// d^T S^-1 d >= |d|^2 / lambda_max(S) >= gap^2 / (lam_target + lam_source)
+// exceeds kFarExp: each skipped term is below exp(-kFarExp / 2) = 7e-13 of
+// that pair's own peak, far below the FP32 rounding of the terms summed.
+// DUGMA_FAR_SKIP=0 in the environment when the covariances are set turns
+// the skip off for that batch (for comparisons).
+constexpr float kFarExp = 56.0f;
 
 // Per-trial optimiser state (all FP64).  The pose parameters are
 // p = [w L, dt] for R <- exp([w]x) R, t <- t + dt (L: the trial's RMS source
@@ -618,7 +640,8 @@ __global__ __launch_bounds__(256) void k_batchf_transform(
   o.s12 = (float)(0.5 * (Sp[3] + Sp[1]));
   o.s13 = (float)(0.5 * (Sp[6] + Sp[2]));
   o.s23 = (float)(0.5 * (Sp[7] + Sp[5]));
-  o.pad0 = o.pad1 = o.pad2 = o.pad3 = 0.0f;
+  o.pad0 = cv.lam;  // the largest eigenvalue does not change with the rotation
+  o.pad1 = o.pad2 = o.pad3 = 0.0f;
   Z[i] = o;
 }
 
@@ -629,11 +652,22 @@ __global__ __launch_bounds__(256) void k_batchf_transform(
 // gradient follows from these target-side sums (pose_contrib: rotation
 // about the origin of the frame, i.e. about t, then translation).  Item
 // totals: [E, rotation (3), translation (3)], in the source frame.
+__device__ __forceinline__ float wave_min_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 template <int W>
 __global__ __launch_bounds__(64 * W) void k_batchf_pairs(
-    const Rec3F *__restrict__ Z, const Rec3F *__restrict__ S, const BItem *__restrict__ items,
-    const int *__restrict__ order, int nitems, unsigned *__restrict__ queue, const int *__restrict__ done,
-    double *__restrict__ part) {
+    const Rec3F *__restrict__ Z, const Rec3F *__restrict__ S, const BBound *__restrict__ SB, float far_exp,
+    const BItem *__restrict__ items, const int *__restrict__ order, int nitems, unsigned *__restrict__ queue,
+    const int *__restrict__ done, double *__restrict__ part) {
   __shared__ int s_p;
   __shared__ double red[W][7];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -654,8 +688,28 @@ __global__ __launch_bounds__(64 * W) void k_batchf_pairs(
     for (int a = 0; a < 10; ++a) dacc[a] = 0.0;
     const int s_end = it.s0 + it.sn;
     const int s_run = wave * 64 < it.tn ? s_end : it.s0;  // idle waves skip the pairs
+    // the wave's targets: bounding box and largest eigenvalue bound (lanes
+    // past the tile hold the tile's first target)
+    const float zlo0 = wave_min_f(pz.hx), zlo1 = wave_min_f(pz.hy), zlo2 = wave_min_f(pz.hz);
+    const float zhi0 = wave_max_f(pz.hx), zhi1 = wave_max_f(pz.hy), zhi2 = wave_max_f(pz.hz);
+    const float zlam = wave_max_f(pz.pad0);
     Rec3F qa = S[it.s0], qb = S[min(it.s0 + 1, s_end - 1)];  // it.sn >= 1
-    for (int j0 = it.s0; j0 < s_run; j0 += kBSub) {
+    bool primed = true;  // qa, qb hold records j0 and j0 + 1
+    for (int j0 = it.s0, blk = it.blk; j0 < s_run; j0 += kBSub, ++blk) {
+      const BBound bd = SB[blk];
+      const float g0 = fmaxf(0.0f, fmaxf(bd.lo[0] - zhi0, zlo0 - bd.hi[0]));
+      const float g1 = fmaxf(0.0f, fmaxf(bd.lo[1] - zhi1, zlo1 - bd.hi[1]));
+      const float g2 = fmaxf(0.0f, fmaxf(bd.lo[2] - zhi2, zlo2 - bd.hi[2]));
+      const float gap2 = fmaf(g0, g0, fmaf(g1, g1, g2 * g2));
+      if (__builtin_amdgcn_readfirstlane(gap2 > far_exp * (zlam + bd.lam) ? 1 : 0)) {
+        primed = false;  // far from every target of the wave
+        continue;
+      }
+      if (!primed) {
+        qa = S[j0];
+        qb = S[min(j0 + 1, s_end - 1)];
+        primed = true;
+      }
       float acc[10];
 #pragma unroll
       for (int a = 0; a < 10; ++a) acc[a] = 0.0f;
@@ -890,6 +944,8 @@ struct BLevel {
   double *dpart = nullptr;   // per-item statistics
   size_t items_cap = 0;      // capacity of ditems / dorder / dident / dpart
   std::vector<int> tb, nt, sb, ns;  // per trial: target / source base and count
+  std::vector<int> bb;              // per trial: first 64-source block (prefix of ceil(ns / 64))
+  int nblocks = 0;                  // 64-source blocks of the level
   std::vector<int> titem;
   std::vector<int> horder, hitem_trial;  // host copies: queue order; trial of each item
   long long pairs = 0;              // target x source pairs of one full iteration
@@ -919,6 +975,8 @@ struct dugma_batch {
   BCov *dTC[kLevels] = {};                  // per target: covariance
   Rec3F *dSF[kLevels] = {};                 // per source: position and covariance (static)
   Rec3F *dZF[kLevels] = {};                 // per target at each pass: in the trial's source frame
+  BBound *dSB[kLevels] = {};                // per 64-source block: box and eigenvalue bound
+  float far_exp = kFarExp;                  // skip threshold (infinite: no skip)
   BRef *dref = nullptr;                     // per trial: optimiser state
   std::vector<int> keepT, keepS;            // kept input indices per trial (level-0 order)
   std::vector<BSrc> hsrc0;                  // host copy of the level-0 sources
@@ -938,7 +996,7 @@ static void free_batch(dugma_batch *b) {
   if (!b) return;
   for (int l = 0; l < kLevels; ++l) {
     free_level(b->lv[l]);
-    void *lp[] = {b->dTC[l], b->dSF[l], b->dZF[l]};
+    void *lp[] = {b->dTC[l], b->dSF[l], b->dZF[l], b->dSB[l]};
     for (void *p : lp)
       if (p) (void)hipFree(p);
   }
@@ -968,12 +1026,17 @@ static int make_items(dugma_batch *b, BLevel &L) {
   const int K = b->ntrials, tile_t = 64 * L.waves;
   std::vector<BItem> items;
   L.titem.assign(K + 1, 0);
+  L.bb.assign(K, 0);
+  L.nblocks = 0;
   for (int k = 0; k < K; ++k) {
     L.titem[k] = (int)items.size();
+    L.bb[k] = L.nblocks;
     const int nt = L.nt[k], ns = L.ns[k];
+    L.nblocks += (ns + kBSub - 1) / kBSub;
     if (nt == 0 || ns == 0) continue;
     const int nch = std::max(1, std::min(L.chunks[size_class(ns)], ns));
-    const int tile_s = (ns + nch - 1) / nch;
+    // chunks of whole 64-source blocks
+    const int tile_s = ((ns + nch - 1) / nch + kBSub - 1) / kBSub * kBSub;
     for (int t0 = 0; t0 < nt; t0 += tile_t)
       for (int s0 = 0; s0 < ns; s0 += tile_s) {
         BItem it;
@@ -983,6 +1046,7 @@ static int make_items(dugma_batch *b, BLevel &L) {
         it.tn = std::min(tile_t, nt - t0);
         it.s0 = L.sb[k] + s0;
         it.sn = std::min(tile_s, ns - s0);
+        it.blk = L.bb[k] + s0 / kBSub;
         items.push_back(it);
       }
   }
@@ -1087,6 +1151,48 @@ static double iso_var(const void *c, int k, int which, int i) {
   return (std::isfinite(s) && s32 > 0) ? (double)s32 : -1.0;
 }
 
+// Order of one trial's points for the refinement's far-field skip: points
+// whose variance exceeds 4x the trial's median (outliers, whose wide kernels
+// would widen every bound they share a wave with) after the rest, each group
+// along a 30-bit Morton curve over the trial's bounding box, so that 64
+// consecutive points (a wave's targets, a sub-tile of sources) lie close
+// together -- and so do every 2nd and every 4th, the sparser levels.
+// p: n x stride doubles (x, y, z first), v: n variances.
+static std::vector<int> spatial_order(const double *p, int stride, const double *v, int n) {
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  if (n < 2) return ord;
+  std::vector<double> vs(v, v + n);
+  std::nth_element(vs.begin(), vs.begin() + n / 2, vs.end());
+  const double big = 4.0 * vs[n / 2];
+  double lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+  for (int i = 1; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], p[(size_t)i * stride + a]);
+      hi[a] = std::max(hi[a], p[(size_t)i * stride + a]);
+    }
+  auto spread = [](uint64_t x) {  // 10 bits -> every 3rd bit
+    x &= 0x3ff;
+    x = (x | (x << 16)) & 0x30000ffull;
+    x = (x | (x << 8)) & 0x300f00full;
+    x = (x | (x << 4)) & 0x30c30c3ull;
+    x = (x | (x << 2)) & 0x9249249ull;
+    return x;
+  };
+  std::vector<uint64_t> key(n);
+  for (int i = 0; i < n; ++i) {
+    uint64_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double w = hi[a] - lo[a];
+      const double u = w > 0 ? (p[(size_t)i * stride + a] - lo[a]) / w : 0.0;
+      code |= spread((uint64_t)std::min(1023.0, std::max(0.0, u * 1023.0))) << a;
+    }
+    key[i] = ((v[i] > big ? 1ull : 0ull) << 40) | code;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int u, int w) { return key[u] < key[w]; });
+  return ord;
+}
+
 static int build_batch(int ntrials, const int *N, const int *M, const double *const *X,
                        const double *const *Y, VarFn var, const void *vctx,
                        dugma_batch **out) {
@@ -1140,6 +1246,29 @@ static int build_batch(int ntrials, const int *N, const int *M, const double *co
       b->keepS.push_back(j);
     }
     const int nt = (int)tv.size(), ns = (int)(sy.size() / 4);
+    {
+      // the kept points in spatial order (keepT / keepS follow)
+      const std::vector<int> ot = spatial_order(ty.data(), 3, tv.data(), nt);
+      std::vector<double> ty2(ty.size()), tv2(tv.size());
+      std::vector<int> kt(b->keepT.end() - nt, b->keepT.end());
+      for (int i = 0; i < nt; ++i) {
+        for (int a = 0; a < 3; ++a) ty2[3 * (size_t)i + a] = ty[3 * (size_t)ot[i] + a];
+        tv2[i] = tv[ot[i]];
+        b->keepT[b->keepT.size() - nt + i] = kt[ot[i]];
+      }
+      ty.swap(ty2);
+      tv.swap(tv2);
+      std::vector<double> svar(ns);
+      for (int j = 0; j < ns; ++j) svar[j] = sy[4 * (size_t)j + 3];
+      const std::vector<int> os = spatial_order(sy.data(), 4, svar.data(), ns);
+      std::vector<double> sy2(sy.size());
+      std::vector<int> ks(b->keepS.end() - ns, b->keepS.end());
+      for (int j = 0; j < ns; ++j) {
+        for (int a = 0; a < 4; ++a) sy2[4 * (size_t)j + a] = sy[4 * (size_t)os[j] + a];
+        b->keepS[b->keepS.size() - ns + j] = ks[os[j]];
+      }
+      sy.swap(sy2);
+    }
     if (nt > 0)
       for (int a = 0; a < 3; ++a) c[a] /= nt;
     double r2 = 0.0, vt = 0.0, vs = 0.0;
@@ -1390,8 +1519,8 @@ static int start_trials(dugma_batch *b, const double *R0, const double *t0, bool
 template <int W>
 static void launch_pairsf(dugma_batch *b, int l, int grid, const int *order, int ni) {
   const BLevel &L = b->lv[l];
-  hipLaunchKernelGGL((k_batchf_pairs<W>), dim3(grid), dim3(64 * W), 0, b->stream, b->dZF[l], b->dSF[l], L.ditems,
-                     order, ni, b->dqueue, b->ddone, L.dpart);
+  hipLaunchKernelGGL((k_batchf_pairs<W>), dim3(grid), dim3(64 * W), 0, b->stream, b->dZF[l], b->dSF[l],
+                     b->dSB[l], b->far_exp, L.ditems, order, ni, b->dqueue, b->ddone, L.dpart);
 }
 
 // One refinement pass of trials [k0, k1) on level l: transform at the
@@ -1729,6 +1858,31 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
 // covariance.  Also sets each trial's length scales: L, the RMS radius of
 // its sources about their centroid, and w0, the per-axis kernel width
 // sqrt(2 x mean source variance).
+// An upper bound of the largest eigenvalue of the symmetric 3x3 [s11 s12
+// s13; . s22 s23; . . s33]: the closed form (trigonometric solution of the
+// characteristic cubic, O. K. Smith, Comm. ACM 4(4):168, 1961) in FP64,
+// raised by 1e-4 of the spectral radius bound so that its rounding errs
+// upward.
+static float lam_max_bound(float s11, float s12, float s13, float s22, float s23, float s33) {
+  const double a11 = s11, a12 = s12, a13 = s13, a22 = s22, a23 = s23, a33 = s33;
+  const double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+  const double q = (a11 + a22 + a33) / 3.0;
+  double lam;
+  if (p1 == 0.0) {
+    lam = std::max(a11, std::max(a22, a33));
+  } else {
+    const double p2 = (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + (a33 - q) * (a33 - q) + 2.0 * p1;
+    const double pp = std::sqrt(p2 / 6.0);
+    const double b11 = (a11 - q) / pp, b22 = (a22 - q) / pp, b33 = (a33 - q) / pp;
+    const double b12 = a12 / pp, b13 = a13 / pp, b23 = a23 / pp;
+    const double det = b11 * (b22 * b33 - b23 * b23) - b12 * (b12 * b33 - b23 * b13) + b13 * (b12 * b23 - b22 * b13);
+    const double r = std::min(1.0, std::max(-1.0, 0.5 * det));
+    lam = q + 2.0 * pp * std::cos(std::acos(r) / 3.0);
+  }
+  const double bound = std::fabs(a11) + std::fabs(a22) + std::fabs(a33) + 2.0 * std::sqrt(3.0 * p1);
+  return (float)(lam + 1e-4 * bound) * (1.0f + 1e-6f);
+}
+
 extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *X_C, const double *const *Y_C) {
   if (!b || !X_C || !Y_C) return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: null argument");
   const int K = b->ntrials;
@@ -1758,7 +1912,7 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
         return fail(DUGMA_ERR_ARG, "dugma_batch_set_covariances: a target covariance is not positive definite");
       BCov &c = tc[L.tb[k] + q];
       c.s11 = o[0]; c.s12 = o[1]; c.s13 = o[2]; c.s22 = o[3]; c.s23 = o[4]; c.s33 = o[5];
-      c.pad0 = c.pad1 = 0.0f;
+      c.lam = c.pad = 0.0f;
     }
     double cen[3] = {0, 0, 0}, var = 0.0, r2 = 0.0;
     const int ns = L.ns[k];
@@ -1813,6 +1967,33 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
   }
   tcl[0].swap(tc);
   sfl[0].swap(sf);
+  // eigenvalue bounds per target, and per 64-source block the box and the
+  // largest bound
+  std::vector<BBound> sbl[kLevels];
+  for (int l = 0; l < kLevels; ++l) {
+    const BLevel &Ll = b->lv[l];
+    for (BCov &c : tcl[l]) c.lam = lam_max_bound(c.s11, c.s12, c.s13, c.s22, c.s23, c.s33);
+    sbl[l].resize((size_t)std::max(Ll.nblocks, 1));
+    for (int k = 0; k < K; ++k)
+      for (int q0 = 0, m = Ll.bb[k]; q0 < Ll.ns[k]; q0 += kBSub, ++m) {
+        BBound bd;
+        memset(&bd, 0, sizeof(bd));
+        for (int a = 0; a < 3; ++a) {
+          bd.lo[a] = __builtin_huge_valf();
+          bd.hi[a] = -__builtin_huge_valf();
+        }
+        for (int q = q0; q < std::min(q0 + kBSub, Ll.ns[k]); ++q) {
+          const Rec3F &r = sfl[l][Ll.sb[k] + q];
+          const float x[3] = {r.hx, r.hy, r.hz};
+          for (int a = 0; a < 3; ++a) {
+            bd.lo[a] = std::min(bd.lo[a], x[a]);
+            bd.hi[a] = std::max(bd.hi[a], x[a]);
+          }
+          bd.lam = std::max(bd.lam, lam_max_bound(r.s11, r.s12, r.s13, r.s22, r.s23, r.s33));
+        }
+        sbl[l][m] = bd;
+      }
+  }
   {
     // allocated once (sizes are fixed by the batch); a buffer is allocated
     // only where a previous call left none
@@ -1821,6 +2002,7 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
       if (!b->dTC[l] && (rc = bmalloc(&b->dTC[l], tcl[l].size()))) break;
       if (!b->dSF[l] && (rc = bmalloc(&b->dSF[l], sfl[l].size()))) break;
       if (!b->dZF[l] && (rc = bmalloc(&b->dZF[l], tcl[l].size()))) break;
+      if (!b->dSB[l] && (rc = bmalloc(&b->dSB[l], sbl[l].size()))) break;
     }
     if (rc) return rc;
   }
@@ -1829,9 +2011,12 @@ extern "C" int dugma_batch_set_covariances(dugma_batch *b, const double *const *
       DUGMA_HIP(hipMemcpy(b->dTC[l], tcl[l].data(), tcl[l].size() * sizeof(BCov), hipMemcpyHostToDevice));
     if (!sfl[l].empty())
       DUGMA_HIP(hipMemcpy(b->dSF[l], sfl[l].data(), sfl[l].size() * sizeof(Rec3F), hipMemcpyHostToDevice));
+    DUGMA_HIP(hipMemcpy(b->dSB[l], sbl[l].data(), sbl[l].size() * sizeof(BBound), hipMemcpyHostToDevice));
   }
   b->refL = refL;
   b->refW0 = refW0;
+  const char *fs = getenv("DUGMA_FAR_SKIP");
+  b->far_exp = (fs && fs[0] == '0') ? __builtin_huge_valf() : kFarExp;
   return DUGMA_OK;
 }
 

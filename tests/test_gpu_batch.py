@@ -405,6 +405,32 @@ def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedul
     assert np.mean(np.array(errs) < 2.0) >= 0.8, errs
 
 
+def test_far_field_skip_changes_nothing_measurable(c4_cov_trials, monkeypatch):
+    """The refinement's pair passes let a wave skip a block of 64 sources
+    when a bound (box distance squared over the largest eigenvalues of the
+    two covariances) puts every term below 7e-13 of its own peak.  The same
+    registrations with the skip off (DUGMA_FAR_SKIP=0 when the covariances
+    are set) take the same steps to the same poses and energies, to 1e-9."""
+    trials = [c4_cov_trials[k] for k in C4_REFINE]
+    kw = dict(max_iter=160, anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9, coarse_iters=160,
+              refine_iters=100, refine_coarse_iters=60, refine_mid_iters=20, tol_rot=1e-4, tol_trans=1e-4)
+    out = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("DUGMA_FAR_SKIP", skip)
+        b = dg.Batch(trials)  # the covariances are set here
+        try:
+            out[skip] = b.register(**kw)
+        finally:
+            b.free()
+    a, o = out["1"], out["0"]
+    np.testing.assert_array_equal(a["iters"], o["iters"])
+    np.testing.assert_array_equal(a["status"], o["status"])
+    for k in range(len(trials)):
+        assert np.abs(a["R"][k] - o["R"][k]).max() < 1e-9, k
+        assert np.abs(a["t"][k] - o["t"][k]).max() < 1e-9, k
+        assert abs(a["E"][k] / o["E"][k] - 1) < 1e-9, (k, a["E"][k], o["E"][k])
+
+
 def test_level_shapes_copied_give_identical_results(c4_cov_trials):
     """Each level has its own launch shape; a second batch of the same trials
     given the same per-level shapes (set_level_shape) reproduces the first
