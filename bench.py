@@ -335,8 +335,10 @@ def bench_contours_2d(hip, dg, W, steps=200):
 
 # Batched schedule.  The isotropic phase: DUGMA's sigma update (the kernel
 # variance tracks the mean nearest distance) on top of an annealed extra
-# variance, pose steps over-relaxed by 1.9 once annealing has ended, 160
-# iterations on every 4th target and source point (1/16 of the pairs).  Then
+# variance, pose steps over-relaxed by 1.9 once annealing has ended, 120
+# iterations on every 4th target and source point (1/16 of the pairs).  Each
+# trial's points are in spatial order, so every 4th is an evenly spread
+# subsample.  Then
 # the refinement on the energy with surface covariances (half the 8-NN
 # sample covariance plus 0.002 sigma^2 I), one fused energy + pose-gradient
 # pass per trial pose, BFGS per trial: at most 60 steps on the same coarse
@@ -358,8 +360,12 @@ def bench_contours_2d(hip, dg, W, steps=200):
 #   (coarse isotropic / coarse / middle refinement steps, tol 1e-4): 160 / 60 /
 #   0: 88.4%, 6,376/s;  160 / 60 / 20: 88.8%, 6,220/s;  160 / 60 / 40: 88.9%,
 #   6,165/s;  100 / 60 / 30: 88.7%, 6,318/s;  80 / 60 / 30: 88.6%, 6,310/s
-#   (run-to-run spread of the rate about 4%).
-BATCH_COARSE_ITERS = 160
+#   (run-to-run spread of the rate about 4%).  With the points in spatial
+#   order and the refinement's far-field skip: 160 / 60 / 20: 91.9%, 7,983/s;
+#   120 / 60 / 20: 91.8%, 8,822/s;  100 / 60 / 20: 91.5%, 8,992/s;  80 / 60 /
+#   20: 91.4%, 9,708/s;  120 / 60 / 0: 91.5%, 9,180/s;  120 / 40 / 10: 91.6%,
+#   8,301/s.
+BATCH_COARSE_ITERS = 120
 BATCH_REFINE_ITERS = 100
 BATCH_REFINE_COARSE_ITERS = 60
 BATCH_REFINE_MID_ITERS = 20

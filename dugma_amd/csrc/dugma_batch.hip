@@ -1721,7 +1721,9 @@ extern "C" int dugma_batch_register(dugma_batch *b, const double *R0,
   // first coarse_iters pose steps run on the coarse level, and when the fine
   // level takes over, its first pass measures sig again
   const int coarse = std::min(opts->coarse_iters, opts->max_iter);
-  const int extra = sigma ? (coarse > 0 ? 2 : 1) : 0;
+  // no full-cloud iterations after the coarse ones: every trial has reached
+  // max_iter there, and the full clouds' measuring pass would find none running
+  const int extra = sigma ? (coarse > 0 && coarse < opts->max_iter ? 2 : 1) : 0;
   const int passes = opts->max_iter + extra;
   const int fine_from = coarse > 0 ? coarse + (sigma ? 1 : 0) : 0;  // first fine pass
   auto level_of = [&](int pass) { return pass < fine_from ? 1 : 0; };

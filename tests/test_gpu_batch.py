@@ -381,7 +381,7 @@ def test_refine_reaches_stationary_accurate_poses(c4_cov_trials, oracle, schedul
     points, up to 30% outliers, 0-90 degree starts)."""
     trials = [c4_cov_trials[k] for k in C4_REFINE]
     sched = (dict(max_iter=460, refine_coarse_iters=0) if schedule == "full"
-             else dict(max_iter=160, refine_coarse_iters=60, refine_mid_iters=20))
+             else dict(max_iter=120, refine_coarse_iters=60, refine_mid_iters=20))
     b = dg.Batch(trials)
     try:
         out = b.register(anneal=True, anneal_decay=0.95, sigma_update=True, relax=1.9, coarse_iters=160,
