@@ -431,6 +431,48 @@ def test_far_field_skip_changes_nothing_measurable(c4_cov_trials, monkeypatch):
         assert abs(a["E"][k] / o["E"][k] - 1) < 1e-9, (k, a["E"][k], o["E"][k])
 
 
+def test_far_field_skip_edge_sizes(monkeypatch, oracle):
+    """Trials smaller than one 64-source block, at block boundaries (64,
+    65, 128, 129 points) and with duplicated points (equal spatial keys),
+    isotropic covariances: with and without the skip, the same steps,
+    poses and energies (1e-9), and the energy at the returned pose against
+    the FP64 oracle."""
+    rng = np.random.default_rng(11)
+    sizes = [(40, 50), (64, 64), (65, 129), (128, 65), (129, 300), (300, 128)]
+    trials = []
+    for s0, (n, m) in enumerate(sizes):
+        tr = small_trial(300 + s0, n, m, deg=float(rng.uniform(5.0, 30.0)))
+        if s0 == 5:  # every source twice
+            tr["Y"] = np.concatenate([tr["Y"], tr["Y"]], axis=1)
+            tr["sY"] = np.concatenate([tr["sY"], tr["sY"]])
+        trials.append(tr)
+    kw = dict(max_iter=60, anneal=True, anneal_decay=0.9, sigma_update=True, coarse_iters=30, refine_iters=40,
+              refine_coarse_iters=10, refine_mid_iters=5, tol_rot=1e-6, tol_trans=1e-6)
+    out = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("DUGMA_FAR_SKIP", skip)
+        b = dg.Batch(trials)
+        try:
+            b.set_covariances([np.eye(3) * tr["sX"][:, None, None] for tr in trials],
+                              [np.eye(3) * tr["sY"][:, None, None] for tr in trials])
+            out[skip] = b.register(**kw)
+        finally:
+            b.free()
+    a, o = out["1"], out["0"]
+    np.testing.assert_array_equal(a["iters"], o["iters"])
+    np.testing.assert_array_equal(a["status"], o["status"])
+    np.testing.assert_allclose(a["R"], o["R"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(a["t"], o["t"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(a["E"], o["E"], rtol=1e-9, atol=0)
+    f32 = lambda v: np.asarray(v, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    for k, tr in enumerate(trials):
+        Y, X = f32(tr["X"].T), f32(tr["Y"].T)
+        SY = f32(tr["sX"])[:, None, None] * np.eye(3)
+        SX = f32(tr["sY"])[:, None, None] * np.eye(3)
+        ref = oracle.energy(Y, SY, X, SX, a["R"][k], a["t"][k])["E"]
+        assert abs(a["E"][k] / ref - 1) <= 1e-5, (sizes[k], a["E"][k], ref)
+
+
 def test_level_shapes_copied_give_identical_results(c4_cov_trials):
     """Each level has its own launch shape; a second batch of the same trials
     given the same per-level shapes (set_level_shape) reproduces the first
