@@ -53,7 +53,9 @@
 //                       frame at the candidate pose (FP64 -> FP32 hi/lo)
 //   k_batchf_pairs      the same persistent item queue; per target lane E,
 //                       EW and B (full covariance), then the pose gradient
-//                       by the target-side identity, in the source frame
+//                       by the target-side identity, in the source frame;
+//                       a wave skips each block of 64 sources whose terms
+//                       are all below 7e-13 of their peaks (kFarExp)
 //   k_batchf_update     sums the items, rotates the gradient back, Armijo
 //                       test, BFGS update, next candidate pose
 // Optionally the refinement runs first on sparser levels' points, with
@@ -62,6 +64,10 @@
 // covariance spans twice the distance), then every 2nd point
 // (refine_mid_iters), each restarting BFGS from the previous level's pose
 // and scaled inverse Hessian, before the full clouds.
+//
+// Each trial's points are kept in spatial order (spatial_order): the
+// sparser levels are then evenly spread subsamples, and 64 consecutive
+// points lie close together, which the far-field skip needs.
 //
 // Launch shape: waves per workgroup (target tile = 64 x waves), the number
 // of source chunks per target tile for each size class of trials (by source
