@@ -57,7 +57,10 @@
 //                       a wave skips each block of 64 sources whose terms
 //                       are all below 7e-13 of their peaks (kFarExp)
 //   k_batchf_update     sums the items, rotates the gradient back, Armijo
-//                       test, BFGS update, next candidate pose
+//                       test, BFGS update, next candidate pose; at an
+//                       accepted pose the direction comes from the pass's
+//                       own approximate Hessian (Newton) when that is
+//                       positive definite, else from the BFGS matrix
 // Optionally the refinement runs first on sparser levels' points, with
 // their covariances times the level's point step: the coarse level
 // (refine_coarse_iters; a quarter of the sampling density, so a surface
@@ -86,7 +89,7 @@
 namespace dugma {
 
 constexpr int kBSub = 64;
-constexpr int kStat = 20;         // stride of the per-item statistics (doubles; 17 used)
+constexpr int kStat = 28;         // stride of the per-item statistics (doubles; 17 / 28 used)
 constexpr int kUpd = 256;         // threads of the fold kernel (one workgroup per trial)
 constexpr int kClasses = 4;       // size classes by source count: < 1000, < 2000, < 4000, more
 constexpr unsigned kInfBits = 0x7f800000u;  // +inf as u32: start value of the nearest distance
@@ -599,8 +602,10 @@ struct BRef {
 struct RParams {
   double tol_rot, tol_trans;
   int max_iter;
-  int pad;
+  int newton;  // 1: Newton direction at each level's first pose, 2: at every accepted pose
 };
+
+constexpr int kRefStat = 28;  // per refinement item: E, gradient (6), Hessian (21)
 
 // Candidate pose as every pass sees it: z = cR^T (y - (ct - c)) and
 // S' = cR^T S cR, FP64 -> FP32 (position hi + lo).
@@ -657,7 +662,8 @@ __global__ __launch_bounds__(256) void k_batchf_transform(
 // S^-1), FP32 over sub-tiles of kBSub sources, then FP64; the pose
 // gradient follows from these target-side sums (pose_contrib: rotation
 // about the origin of the frame, i.e. about t, then translation).  Item
-// totals: [E, rotation (3), translation (3)], in the source frame.
+// totals: [E, rotation (3), translation (3), Hessian (21, pose_hessian_entry,
+// FP32 wave sums)], in the source frame.
 __device__ __forceinline__ float wave_min_f(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
@@ -669,13 +675,51 @@ __device__ __forceinline__ float wave_max_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// Second-order pose statistics of one target lane, from the sums the pair
+// loop leaves (a = [E, EW (3), B (6)], EW = sum e S^-1 d, B = sum e (S^-1 d
+// d^T S^-1 - S^-1), d = z - x): the Hessian of E for moving the target by
+// z -> z + theta x z + delta, keeping its covariance's turn only in the
+// gradient.  With J = [-[z]x | I]:  H = J^T B J, plus, from the second-order
+// turn of z, -1/2 (EW z^T + z EW^T) + (EW . z) I in the theta block.  To
+// first order this is also the Hessian for moving the sources the opposite
+// way (the pose step's parameters).  Entry (i, j) of the 6 x 6 (theta,
+// delta) matrix, i <= j, in FP32 (it only shapes the search direction); one
+// entry at a time keeps few registers live.
+__device__ __forceinline__ float pose_hessian_entry(int i, int j, const float *z, const float *a) {
+  auto Bm = [&](int u, int v) -> float {  // B, symmetric, from a[4..9]
+    const int lo = u < v ? u : v, hi = u < v ? v : u;
+    return a[4 + (lo == 0 ? hi : lo == 1 ? 2 + hi : 5)];
+  };
+  auto Mm = [&](int u, int v) -> float {  // [z]x
+    if (u == v) return 0.0f;
+    const int w = 3 - u - v;
+    return ((u + 1) % 3 == v) ? -z[w] : z[w];
+  };
+  if (i >= 3) return Bm(i - 3, j - 3);
+  if (j >= 3) {  // theta delta: M B
+    const int c = j - 3;
+    return Mm(i, 0) * Bm(0, c) + Mm(i, 1) * Bm(1, c) + Mm(i, 2) * Bm(2, c);
+  }
+  float v = 0.0f;  // theta theta: M^T B M
+  for (int u = 0; u < 3; ++u)
+    for (int w = 0; w < 3; ++w) v += Mm(u, i) * Bm(u, w) * Mm(w, j);
+  const float ez = a[1] * z[0] + a[2] * z[1] + a[3] * z[2];
+  return v - 0.5f * (a[1 + i] * z[j] + z[i] * a[1 + j]) + (i == j ? ez : 0.0f);
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 template <int W>
-__global__ __launch_bounds__(64 * W) void k_batchf_pairs(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_batchf_pairs(
     const Rec3F *__restrict__ Z, const Rec3F *__restrict__ S, const BBound *__restrict__ SB, float far_exp,
     const BItem *__restrict__ items, const int *__restrict__ order, int nitems, unsigned *__restrict__ queue,
     const int *__restrict__ done, double *__restrict__ part) {
   __shared__ int s_p;
-  __shared__ double red[W][7];
+  __shared__ double red[W][kRefStat];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (;;) {
     if (threadIdx.x == 0) s_p = (int)atomicAdd(queue, 1u);
@@ -729,16 +773,33 @@ __global__ __launch_bounds__(64 * W) void k_batchf_pairs(
 #pragma unroll
       for (int a = 0; a < 10; ++a) dacc[a] += (double)acc[a];
     }
-    double r[7] = {0, 0, 0, 0, 0, 0, 0};
-    const double zero[3] = {0.0, 0.0, 0.0};
-    if (valid) pose_contrib(pz, dacc, zero, r);
+    {
+      double r[7] = {0, 0, 0, 0, 0, 0, 0};
+      const double zero[3] = {0.0, 0.0, 0.0};
+      if (valid) pose_contrib(pz, dacc, zero, r);
 #pragma unroll
-    for (int a = 0; a < 7; ++a) {
-      const double v = wave_sum(r[a]);
-      if (lane == 0) red[wave][a] = v;
+      for (int a = 0; a < 7; ++a) {
+        const double v = wave_sum(r[a]);
+        if (lane == 0) red[wave][a] = v;
+      }
+    }
+    {
+      const float z[3] = {pz.hx, pz.hy, pz.hz};
+      float af[10];
+#pragma unroll
+      for (int a = 0; a < 10; ++a) af[a] = (float)dacc[a];
+      int q = 7;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+          const float v = wave_sum_f(valid ? pose_hessian_entry(i, j, z, af) : 0.0f);
+          if (lane == 0) red[wave][q] = (double)v;
+          ++q;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < kRefStat) {
       double v = red[0][threadIdx.x];
 #pragma unroll
       for (int w = 1; w < W; ++w) v += red[w][threadIdx.x];
@@ -815,6 +876,71 @@ __device__ bool next_step_converged(const BRef &r, const RParams &rp) {
   return r.alpha * sqrt(pa) / r.L < rp.tol_rot && r.alpha * sqrt(pt) < rp.tol_trans;
 }
 
+// The Newton inverse Hessian of f = -log E in p = [w L, dt], from a pass's
+// sums st = [E, gradient (6), Hessian (21, upper triangle)] in the source
+// frame: both turned to the world frame by cR (g = cR g' per 3-block, H =
+// Q H' Q^T), H_f = -H_E / E + g g^T / E^2, then scaled to p.  False unless
+// H_f is positive definite (Cholesky pivots above 1e-12 of the largest
+// diagonal entry); g: the world gradient of E.
+__device__ bool newton_inverse(const double *st, const double *cR, const double *g, double E, double L,
+                               double *Hinv) {
+  double Hs[6][6];
+  int q = 7;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) Hs[i][j] = Hs[j][i] = st[q++];
+  double Q[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) Q[i][j] = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) Q[a][b] = Q[3 + a][3 + b] = cR[a + 3 * b];
+  double T[6][6], H[6][6];
+  for (int a = 0; a < 6; ++a)
+    for (int j = 0; j < 6; ++j) {
+      double v = 0.0;
+      for (int i = 0; i < 6; ++i) v += Q[a][i] * Hs[i][j];
+      T[a][j] = v;
+    }
+  const double sc[6] = {1.0 / L, 1.0 / L, 1.0 / L, 1.0, 1.0, 1.0};
+  double dmax = 0.0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      double v = 0.0;
+      for (int j = 0; j < 6; ++j) v += T[a][j] * Q[b][j];
+      H[a][b] = (-v / E + g[a] * g[b] / (E * E)) * sc[a] * sc[b];
+    }
+  for (int a = 0; a < 6; ++a) dmax = fmax(dmax, H[a][a]);
+  if (!(dmax > 0.0) || !isfinite(dmax)) return false;
+  // Cholesky H = C C^T (lower C in place)
+  for (int j = 0; j < 6; ++j) {
+    double d = H[j][j];
+    for (int k = 0; k < j; ++k) d -= H[j][k] * H[j][k];
+    if (!(d > 1e-12 * dmax)) return false;
+    d = sqrt(d);
+    H[j][j] = d;
+    for (int i = j + 1; i < 6; ++i) {
+      double v = H[i][j];
+      for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
+      H[i][j] = v / d;
+    }
+  }
+  // H^-1, one unit vector at a time
+  for (int c = 0; c < 6; ++c) {
+    double y[6], x[6];
+    for (int i = 0; i < 6; ++i) {
+      double v = i == c ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= H[i][k] * y[k];
+      y[i] = v / H[i][i];
+    }
+    for (int i = 5; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < 6; ++k) v -= H[k][i] * x[k];
+      x[i] = v / H[i][i];
+    }
+    for (int i = 0; i < 6; ++i) Hinv[6 * i + c] = x[i];
+  }
+  return true;
+}
+
 // One thread per trial: the pass's energy and gradient at the candidate
 // pose, then the optimiser's next move (dugma_register_energy's logic, and
 // a trial also stops when its next quasi-Newton step is below the
@@ -826,9 +952,10 @@ __global__ __launch_bounds__(256) void k_batchf_update(
   const int k = k0 + blockIdx.x * 256 + threadIdx.x;
   if (k >= k1 || done[k]) return;
   BRef r = ref[k];
-  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  double st[kRefStat];
+  for (int a = 0; a < kRefStat; ++a) st[a] = 0.0;
   for (int i = titem[k]; i < titem[k + 1]; ++i)
-    for (int a = 0; a < 7; ++a) st[a] += part[(size_t)i * kStat + a];
+    for (int a = 0; a < kRefStat; ++a) st[a] += part[(size_t)i * kStat + a];
   const double E = st[0];
   // gradient (rotation, translation) back from the source frame: g = cR g'
   double g[6];
@@ -854,6 +981,7 @@ __global__ __launch_bounds__(256) void k_batchf_update(
         for (int a = 0; a < 36; ++a) r.H[a] = (a % 7 == 0) ? 1.0 : 0.0;
         r.scaled = 0;
       }
+      if (rp.newton >= 1 && newton_inverse(st, r.cR, g, E, r.L, r.H)) r.scaled = 1;
       r.state = 1;
       if (rp.max_iter < 1 || !ref_direction(r)) {
         r.status = rp.max_iter < 1 ? 2 : 1;
@@ -891,6 +1019,7 @@ __global__ __launch_bounds__(256) void k_batchf_update(
         for (int b = 0; b < 6; ++b)
           r.H[6 * a + b] += -rr * (Hy[a] * s[b] + s[a] * Hy[b]) + (rr * rr * yHy + rr) * s[a] * s[b];
     }
+    if (rp.newton >= 2 && newton_inverse(st, r.cR, g, E, r.L, r.H)) r.scaled = 1;
     for (int a = 0; a < 9; ++a) r.R[a] = r.cR[a];
     for (int a = 0; a < 3; ++a) r.t[a] = r.ct[a];
     r.f = -log(E);
@@ -1612,6 +1741,12 @@ static int run_refine(dugma_batch *b, const dugma_batch_opts *opts, int l, int m
   rp.tol_rot = opts->tol_rot;
   rp.tol_trans = opts->tol_trans;
   rp.max_iter = max_steps;
+  {
+    // DUGMA_REF_NEWTON=0 or 1 in the environment: plain BFGS, or the Newton
+    // direction at each level's first pose only (for comparisons)
+    const char *nv = getenv("DUGMA_REF_NEWTON");
+    rp.newton = nv && (nv[0] == '0' || nv[0] == '1') ? nv[0] - '0' : 2;
+  }
   // passes: each accepted step takes one pass at least; the halvings of a
   // line search take more, so allow 4 passes per step on average
   const int passes = 1 + 4 * max_steps;
