@@ -341,7 +341,8 @@ def bench_contours_2d(hip, dg, W, steps=200):
 # subsample.  Then
 # the refinement on the energy with surface covariances (half the 8-NN
 # sample covariance plus 0.002 sigma^2 I), one fused energy + pose-gradient
-# pass per trial pose, BFGS per trial: at most 60 steps on the same coarse
+# pass per trial pose, Newton steps from the pass's approximate Hessian (BFGS
+# where it is not positive definite) per trial: at most 60 steps on the same coarse
 # points (covariances times 4), at most 20 on every 2nd point (covariances
 # times 2), then at most 100 on the full clouds, each level starting from the
 # previous one's pose and scaled inverse Hessian, until the rotation step is below
@@ -455,7 +456,7 @@ def bench_batched(dg, W, ntrials, baseline_trials, max_iter=BATCH_MAX_ITER):
             "rotation_error_median_deg": round(float(np.median(errs)), 4),
             "success_rate_from_ground_truth": round(float((errs_g < 2.0).mean()), 4),
             "success_by_min_points": bands,
-            "refine": "BFGS per trial on the energy with surface covariances (half the 8-NN sample covariance "
+            "refine": "Newton / BFGS per trial on the energy with surface covariances (half the 8-NN sample covariance "
                       "+ 0.002 sigma^2 I), one fused energy + pose-gradient pass per trial pose: at most %d steps "
                       "on every 4th point (covariances x 4), %d on every 2nd (x 2), then at most %d on the full "
                       "clouds; stop at rotation step < %g rad and translation step < %g"

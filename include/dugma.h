@@ -293,13 +293,14 @@ typedef struct {
                                 (rotation angle and centroid move times relax) */
   int coarse_iters;          /* first iterations on every 4th target and source point
                                 (no convergence test there); then the full clouds */
-  int refine_iters;          /* > 0: then at most this many accepted BFGS steps per trial
-                                on -log E with the full covariances of
+  int refine_iters;          /* > 0: then at most this many accepted steps per trial on
+                                -log E with the full covariances of
                                 dugma_batch_set_covariances (one fused energy + pose-gradient
-                                pass per trial pose; stops when an accepted step, or the next
-                                quasi-Newton step once the inverse Hessian is scaled, is below
-                                tol_rot / tol_trans) */
-  int refine_coarse_iters;   /* with refine_iters > 0: first at most this many BFGS steps on
+                                pass per trial pose, which also sums an approximate pose
+                                Hessian: Newton directions where it is positive definite, else
+                                BFGS; stops when an accepted step, or the next step once the
+                                inverse Hessian is scaled, is below tol_rot / tol_trans) */
+  int refine_coarse_iters;   /* with refine_iters > 0: first at most this many steps on
                                 every 4th target and source point, with their covariances
                                 times 4 (the sampling density is a quarter, so a surface
                                 covariance spans twice the distance), then the full clouds
