@@ -229,6 +229,19 @@ def energy_grad(h, R, t, cov="auto"):
     return b.E.value, b.g.copy()
 
 
+def energy_grad_hess(h, R, t, cov="auto"):
+    """(E, gpose (6), H (6, 6)): the fused gradient and the pose Hessian of
+    E from one pass (3-D; see include/dugma.h)."""
+    R = _cm(R, (3, 3))
+    t = _cm(np.ravel(t))
+    E = ctypes.c_double()
+    g = np.zeros(6)
+    H = np.zeros(36)
+    _lib.check(_lib.load().dugma_energy_grad_hess(h.ptr, _dp(R), _dp(t), _cov_mode(cov), ctypes.byref(E),
+                                                  _dp(g), _dp(H)))
+    return E.value, g, H.reshape(6, 6)
+
+
 def energy_grad_fd(h, R, t, step=1e-4, cov="auto"):
     R = _cm(R, (h.D, h.D))
     t = _cm(np.ravel(t))
@@ -240,9 +253,10 @@ def energy_grad_fd(h, R, t, step=1e-4, cov="auto"):
 
 
 def register_energy(h, R0, t0, max_iter=200, tol_rot=1e-7, tol_trans=1e-14, cov="auto"):
-    """Registration on the convolved energy (dugma_register_energy): BFGS
-    descent of -log E over left pose increments from (R0, t0).  Returns
-    (R, t, iterations, E)."""
+    """Registration on the convolved energy (dugma_register_energy): descent
+    of -log E over left pose increments from (R0, t0), Newton directions
+    from each pass's pose Hessian where positive definite (3-D), else BFGS.
+    Returns (R, t, iterations, E)."""
     D = h.D
     R0 = _cm(R0, (D, D))
     t0 = _cm(np.ravel(t0))

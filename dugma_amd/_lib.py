@@ -57,6 +57,7 @@ SIGNATURES = {
     "dugma_register_energy": (_ci, [_vp, _d, _d, ctypes.POINTER(RegisterEnergyOpts), _d, _d, _i, _d]),
     "dugma_energy_grad_points": (_ci, [_vp, _d, _d, _ci, _d, _d, _d, _d, _d, _d, _d]),
     "dugma_energy_grad_fd": (_ci, [_vp, _d, _d, _ci, _cd, _d, _d]),
+    "dugma_energy_grad_hess": (_ci, [_vp, _d, _d, _ci, _d, _d, _d]),
     "dugma_knn": (_ci, [_vp, _ci, _ci, _d, _d, _i, _f]),
     "dugma_update_sigma_knn": (_ci, [_vp, _ci, _ci, _f, _f, _i]),
     "dugma_update_cov_knn": (_ci, [_vp, _ci, _ci, ctypes.c_double, ctypes.c_double, _f, _f, _i]),

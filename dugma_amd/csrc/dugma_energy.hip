@@ -31,7 +31,44 @@ constexpr int kBlock = 256;  // 4 waves of 64
 constexpr int kSub = 64;     // FP32 sub-tile length before the FP64 flush
 constexpr int kMinChunk = 32;  // fewest streamed points per workgroup
 
-enum { kModeEnergy = 0, kModePose = 1, kModePoint = 2 };
+// kModeHess: the pose pass plus the sums of an approximate pose Hessian
+// (3-D, transformed sources stationary)
+enum { kModeEnergy = 0, kModePose = 1, kModePoint = 2, kModeHess = 3 };
+constexpr int kHessPart = 32;  // per workgroup: the kPosePart pose values, then 21 Hessian entries
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Entry (i <= j) of the Hessian of E for moving a stationary point p by
+// p - t -> p - t + theta x (p - t) + delta, from its sums a = [E, EW (3), B
+// (6, upper triangle)] with d = p - q: with z = p - t and J = [-[z]x | I],
+// H = J^T B J plus -1/2 (EW z^T + z EW^T) + (EW . z) I in the theta block
+// (the second-order turn of z).  The turn of p's own covariance enters only
+// the gradient.  FP32: the Hessian only shapes the search direction.
+__device__ __forceinline__ float point_hessian_entry(int i, int j, const float *z, const float *a) {
+  auto Bm = [&](int u, int v) -> float {
+    const int lo = u < v ? u : v, hi = u < v ? v : u;
+    return a[4 + (lo == 0 ? hi : lo == 1 ? 2 + hi : 5)];
+  };
+  auto Mm = [&](int u, int v) -> float {  // [z]x
+    if (u == v) return 0.0f;
+    const int w = 3 - u - v;
+    return ((u + 1) % 3 == v) ? -z[w] : z[w];
+  };
+  if (i >= 3) return Bm(i - 3, j - 3);
+  if (j >= 3) {
+    const int c = j - 3;
+    return Mm(i, 0) * Bm(0, c) + Mm(i, 1) * Bm(1, c) + Mm(i, 2) * Bm(2, c);
+  }
+  float v = 0.0f;
+  for (int u = 0; u < 3; ++u)
+    for (int w = 0; w < 3; ++w) v += Mm(u, i) * Bm(u, w) * Mm(w, j);
+  const float ez = a[1] * z[0] + a[2] * z[1] + a[3] * z[2];
+  return v - 0.5f * (a[1 + i] * z[j] + z[i] * a[1 + j]) + (i == j ? ez : 0.0f);
+}
 
 struct PoseArg {
   double R[9];  // column-major
@@ -75,13 +112,15 @@ static inline dim3 tile_grid(int pblocks, int qchunks, int xcd_map) {
 }
 
 template <class Rec, int MODE, int T, int COMP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ? 8 : 1, 8))) void k_pairs(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 && MODE != kModeHess ? 8 : 1, 8))) void k_pairs(
     const Rec *__restrict__ P, int nP, const Rec *__restrict__ Q, int nQ,
     int qchunk, int pblocks, int qchunks, int xcd_map, double *__restrict__ out, double t0, double t1,
     double t2) {
   using Tr = RecTraits<Rec>;
   constexpr bool GRAD = MODE != kModeEnergy;
-  constexpr int NA = GRAD ? Tr::NACC : 1;
+  // the Hessian pass needs B: isotropic records then hold it in two parts
+  constexpr int NA = MODE == kModeHess ? (Tr::FULL ? Tr::NACC : 11) : (GRAD ? Tr::NACC : 1);
+  constexpr int PART = MODE == kModeHess ? kHessPart : kPosePart;
   int sb, qc;
   if (!xcd_tile(pblocks, qchunks, xcd_map, &sb, &qc)) return;  // grid padding (whole workgroup)
   const int lane = threadIdx.x & 63;
@@ -131,7 +170,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
   // gradient is then taken about the sources, whose rotation also turns
   // their covariances.  With EW and B both negated, pose_contrib's
   // (p - t) x EW + vee(S_p B - B S_p) is exactly that gradient.
-  constexpr bool kSwapped = MODE == kModePose && COMP == 2;
+  constexpr bool kSwapped = (MODE == kModePose || MODE == kModeHess) && COMP == 2;
   if (!RecTraits<Rec>::FULL || kSwapped) {
     constexpr double gs = kSwapped ? -RecTraits<Rec>::GSCALE : RecTraits<Rec>::GSCALE;
 #pragma unroll
@@ -168,7 +207,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
         r[0] += dacc[k][0];
     }
   }
-  double *o = out + ((size_t)qc * pblocks + sb) * kPosePart;
+  double *o = out + ((size_t)qc * pblocks + sb) * PART;
   if (GRAD) {
     // Workgroup sums of the kPosePart values in a fixed order: every lane's
     // values go to LDS; thread (a, k) = (tid / 32, tid % 32) adds entries
@@ -190,6 +229,51 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(T == 1 ?
     if (lane == 0) {
       o[2 * wave] = lo;
       o[2 * wave + 1] = hi;
+    }
+    if constexpr (MODE == kModeHess && Tr::D == 3) {
+      // the Hessian entries, one at a time: lane sums over its T points, a
+      // wave sum, then the waves' sums in order.  The block above negated
+      // EW and B (sources stationary); the entries need them as summed.
+      constexpr float sg = kSwapped ? -1.0f : 1.0f;
+      __shared__ float hred[kBlock / 64][21];
+      int q = 0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+          float v = 0.0f;
+#pragma unroll
+          for (int kk = 0; kk < T; ++kk) {
+            if (pbase + 64 * kk >= nP) continue;
+            float av[10];
+            av[0] = 0.0f;
+#pragma unroll
+            for (int u = 1; u < 4; ++u) av[u] = sg * (float)dacc[kk][u];
+            if (Tr::FULL) {
+#pragma unroll
+              for (int u = 4; u < 10; ++u) av[u] = sg * (float)dacc[kk][u];
+            } else {  // B = P - Q I
+              const float qd = (float)dacc[kk][NA - 1];
+#pragma unroll
+              for (int u = 4; u < 10; ++u)
+                av[u] = sg * ((float)dacc[kk][u] - ((u == 4 || u == 7 || u == 9) ? qd : 0.0f));
+            }
+            const float z[3] = {(float)((double)p[kk].hx + (double)p[kk].lx - t0),
+                                (float)((double)p[kk].hy + (double)p[kk].ly - t1),
+                                (float)((double)p[kk].hz + (double)p[kk].lz - t2)};
+            v += point_hessian_entry(i, j, z, av);
+          }
+          v = wave_sum_f(v);
+          if (lane == 0) hred[wave][q] = v;
+          ++q;
+        }
+      __syncthreads();
+      if (threadIdx.x < 21) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) s2 += (double)hred[w][threadIdx.x];
+        o[kPosePart + threadIdx.x] = s2;
+      }
     }
   } else {
     r[0] = wave_sum(r[0]);
@@ -242,6 +326,38 @@ __global__ __launch_bounds__(kBlock) void k_reduce_parts(
   if (lane == 0) {
     res[kPosePart * blockIdx.x + 2 * wave] = lo;
     res[kPosePart * blockIdx.x + 2 * wave + 1] = hi;
+  }
+}
+
+// The Hessian pass's partials (kHessPart doubles per workgroup of the pair
+// kernel, 29 used): workgroup g sums its contiguous range, each thread over
+// whole records, then the 256 threads' values value by value through LDS in
+// a fixed tree order; the host adds the workgroups' sums in order.
+constexpr int kHessGroups = 64;
+constexpr int kHessUsed = kPosePart + 21;
+__global__ __launch_bounds__(kBlock) void k_reduce_hess(const double *__restrict__ parts, int nparts,
+                                                        double *__restrict__ res) {
+  const int len = (nparts + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int q0 = blockIdx.x * len, q1 = min(q0 + len, nparts);
+  double s[kHessUsed];
+#pragma unroll
+  for (int a = 0; a < kHessUsed; ++a) s[a] = 0.0;
+  for (int q = q0 + (int)threadIdx.x; q < q1; q += kBlock) {
+    const double *row = parts + (size_t)q * kHessPart;
+#pragma unroll
+    for (int a = 0; a < kHessUsed; ++a) s[a] += row[a];
+  }
+  __shared__ double stage[kBlock];
+#pragma unroll 1
+  for (int a = 0; a < kHessUsed; ++a) {
+    stage[threadIdx.x] = s[a];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) stage[threadIdx.x] += stage[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) res[kHessPart * blockIdx.x + a] = stage[0];
+    __syncthreads();
   }
 }
 
@@ -564,6 +680,55 @@ static int pose_pass(dugma_ctx *c, const double *R, const double *t, bool iso,
   return DUGMA_OK;
 }
 
+// The fused pose pass with the sums of an approximate Hessian (3-D, the
+// transformed sources stationary): res = [E, rotation (3), translation
+// (3)] as pose_pass gives them, H = the upper triangle, row by row, of the
+// 6 x 6 Hessian of E in (w, dt) for R <- exp([w]x) R, t <- t + dt, the turn
+// of the sources' covariances left out of its second order; caller units.
+static int pose_hess_pass(dugma_ctx *c, const double *R, const double *t, bool iso, double *res, double *H) {
+  for (int a = 0; a < 7; ++a) res[a] = 0.0;
+  for (int a = 0; a < 21; ++a) H[a] = 0.0;
+  if (c->D != 3) return fail(DUGMA_ERR_ARG, "the pose Hessian is 3-D only");
+  if (c->nT == 0 || c->nS == 0) return DUGMA_OK;
+  PoseArg ps = make_pose(c, R, t);
+  const Plan pl = ctx_plan(c, c->nS, c->nT, kModePose, iso);
+  const size_t nparts = (size_t)pl.pblocks * pl.qchunks;
+  const int groups = (int)std::min<size_t>(kHessGroups, (nparts + kBlock - 1) / kBlock);
+  int rc = c->part.ensure(nparts * kHessPart * sizeof(double));
+  if (!rc) rc = c->scratch2.ensure(sizeof(double) * kHessPart * groups);
+  if (!rc) rc = c->hstage.ensure(sizeof(double) * kHessPart * groups);
+  if (!rc) rc = transform_sources(c, ps, iso);
+  if (rc) return rc;
+  double *parts = (double *)c->part.ptr, *gres = (double *)c->scratch2.ptr;
+  PairTimer timer(c);
+  if (iso)
+    launch_pairs_c<Rec3I, kModeHess, 2>(pl, (const Rec3I *)c->dSrecI, c->nS, (const Rec3I *)c->dTrecI, c->nT,
+                                          parts, ps.t, c->stream);
+  else
+    launch_pairs_c<Rec3F, kModeHess, 2>(pl, (const Rec3F *)c->dSrecF, c->nS, (const Rec3F *)c->dTrecF, c->nT,
+                                          parts, ps.t, c->stream);
+  timer.stop();
+  hipLaunchKernelGGL(k_reduce_hess, dim3(groups), dim3(kBlock), 0, c->stream, parts, (int)nparts, gres);
+  DUGMA_HIP(hipGetLastError());
+  DUGMA_HIP(hipMemcpyAsync(c->hstage.ptr, gres, sizeof(double) * kHessPart * groups, hipMemcpyDeviceToHost,
+                           c->stream));
+  DUGMA_HIP(hipStreamSynchronize(c->stream));
+  timer.finish();
+  double sum[kHessUsed];
+  for (int a = 0; a < kHessUsed; ++a) sum[a] = 0.0;
+  const double *hs = (const double *)c->hstage.ptr;
+  for (int g = 0; g < groups; ++g)
+    for (int a = 0; a < kHessUsed; ++a) sum[a] += hs[kHessPart * g + a];
+  // unscale: E ~ s^3, and a factor s per translation index
+  const double cn = gauss_norm(3), sc = c->scale, sD = sc * sc * sc;
+  res[0] = cn * sD * sum[0];
+  for (int a = 1; a < 7; ++a) res[a] = cn * sD * sum[a] * (a > 3 ? sc : 1.0);
+  int q = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j, ++q) H[q] = cn * sD * sum[kPosePart + q] * (i >= 3 ? sc : 1.0) * (j >= 3 ? sc : 1.0);
+  return DUGMA_OK;
+}
+
 static int check_pose_args(dugma_ctx *c, const double *R, const double *t) {
   if (!c) return fail(DUGMA_ERR_ARG, "null context");
   if (!R || !t) return fail(DUGMA_ERR_ARG, "null R or t");
@@ -730,6 +895,28 @@ extern "C" int dugma_energy_grad(dugma_ctx *ctx, const double *R,
   const int nd = ctx->D == 3 ? 6 : 3;
   if (gpose)
     for (int a = 0; a < nd; ++a) gpose[a] = res[1 + a];
+  return DUGMA_OK;
+}
+
+extern "C" int dugma_energy_grad_hess(dugma_ctx *ctx, const double *R, const double *t, int cov_mode, double *E,
+                                      double *gpose, double *H) {
+  int rc = check_pose_args(ctx, R, t);
+  if (rc) return rc;
+  if (ctx->D != 3) return fail(DUGMA_ERR_ARG, "dugma_energy_grad_hess: 3-D contexts only");
+  bool iso;
+  rc = resolve_iso(ctx, cov_mode, &iso);
+  if (rc) return rc;
+  double res[7], h[21];
+  rc = pose_hess_pass(ctx, R, t, iso, res, h);
+  if (rc) return rc;
+  if (E) *E = res[0];
+  if (gpose)
+    for (int a = 0; a < 6; ++a) gpose[a] = res[1 + a];
+  if (H) {
+    int q = 0;
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j, ++q) H[6 * i + j] = H[6 * j + i] = h[q];
+  }
   return DUGMA_OK;
 }
 

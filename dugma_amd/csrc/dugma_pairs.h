@@ -168,6 +168,20 @@ __device__ __forceinline__ void pair_eval(const Rec3I &p, const Rec3I &q, float 
     acc[2] = fmaf(einv, dy, acc[2]);
     acc[3] = fmaf(einv, dz, acc[3]);
     if (GRAD == 2) acc[4] = fmaf(einv, fmaf(d2 * inv, kLn4f, -3.0f), acc[4]);
+    if (GRAD == 3) {
+      // B = sum e (w w^T - S^-1) in two parts (the ln 4 restored with the
+      // gradient's): [4..9] sum (e / s'^2) ln 4 d d^T, upper triangle row by
+      // row, and [10] sum e / s'
+      const float cw = einv * inv * kLn4f;
+      const float cx = cw * dx, cy = cw * dy, cz = cw * dz;
+      acc[4] = fmaf(cx, dx, acc[4]);
+      acc[5] = fmaf(cx, dy, acc[5]);
+      acc[6] = fmaf(cx, dz, acc[6]);
+      acc[7] = fmaf(cy, dy, acc[7]);
+      acc[8] = fmaf(cy, dz, acc[8]);
+      acc[9] = fmaf(cz, dz, acc[9]);
+      acc[10] += einv;
+    }
   }
 }
 

@@ -8,6 +8,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -460,6 +461,47 @@ static void energy_step(int D, const double *R, const double *t, const double *s
 
 }  // namespace dugma
 
+// The inverse of the Hessian of f = -log E in the driver's parameters
+// q = [w L, dt], from E, its gradient g and Hessian He in (w, dt) (row-major
+// 6 x 6): H_f = -He / E + g g^T / E^2, a factor 1/L per rotation index.
+// False unless H_f is positive definite (Cholesky pivots above 1e-12 of its
+// largest diagonal entry).
+static bool newton_inverse_q(double E, const double *g, const double *He, double L, double *Hinv) {
+  double H[6][6], dmax = 0.0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b)
+      H[a][b] = (-He[6 * a + b] / E + g[a] * g[b] / (E * E)) * (a < 3 ? 1.0 / L : 1.0) * (b < 3 ? 1.0 / L : 1.0);
+  for (int a = 0; a < 6; ++a) dmax = std::max(dmax, H[a][a]);
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return false;
+  for (int j = 0; j < 6; ++j) {
+    double d = H[j][j];
+    for (int k = 0; k < j; ++k) d -= H[j][k] * H[j][k];
+    if (!(d > 1e-12 * dmax)) return false;
+    d = std::sqrt(d);
+    H[j][j] = d;
+    for (int i = j + 1; i < 6; ++i) {
+      double v = H[i][j];
+      for (int k = 0; k < j; ++k) v -= H[i][k] * H[j][k];
+      H[i][j] = v / d;
+    }
+  }
+  for (int c = 0; c < 6; ++c) {
+    double y[6], x[6];
+    for (int i = 0; i < 6; ++i) {
+      double v = i == c ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= H[i][k] * y[k];
+      y[i] = v / H[i][i];
+    }
+    for (int i = 5; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < 6; ++k) v -= H[k][i] * x[k];
+      x[i] = v / H[i][i];
+    }
+    for (int i = 0; i < 6; ++i) Hinv[6 * i + c] = x[i];
+  }
+  return true;
+}
+
 extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const double *t0,
                                      const dugma_register_energy_opts *opts, double *R, double *t,
                                      int *iterations, double *E_out) {
@@ -497,10 +539,20 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
   if (!(w0 > 0) || !std::isfinite(w0)) w0 = 1e-3 * L;
   const double step_max = 0.25 * L;
 
-  double cR[9], ct[3], E, g[6];
+  // 3-D: each pass also sums the pose Hessian (dugma_energy_grad_hess), and
+  // an accepted pose takes the Newton direction where -log E's Hessian is
+  // positive definite; else, and in 2-D, BFGS.  DUGMA_REG_NEWTON=0 in the
+  // environment keeps BFGS alone (for comparisons).
+  const char *nv = getenv("DUGMA_REG_NEWTON");
+  const bool newton = D == 3 && !(nv && nv[0] == '0');
+  auto eval = [&](const double *Rx, const double *tx, double *Ex, double *gx, double *Hx) -> int {
+    return newton ? dugma_energy_grad_hess(ctx, Rx, tx, op.cov_mode, Ex, gx, Hx)
+                  : dugma_energy_grad(ctx, Rx, tx, op.cov_mode, Ex, gx);
+  };
+  double cR[9], ct[3], E, g[6], He[36];
   memcpy(cR, R0, sizeof(double) * DD);
   memcpy(ct, t0, sizeof(double) * D);
-  int rc = dugma_energy_grad(ctx, cR, ct, op.cov_mode, &E, g);
+  int rc = eval(cR, ct, &E, g, He);
   if (rc) return rc;
   if (!(E > 0) || !std::isfinite(E))
     return fail(DUGMA_ERR_STATE, "dugma_register_energy: the energy is zero at the start pose");
@@ -515,6 +567,7 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
   };
   reset();
   bool scaled = false;
+  if (newton && newton_inverse_q(E, g, He, L, H)) scaled = true;
   int it = 0;
   while (it < op.max_iter) {
     double d[6], slope = 0.0, dn = 0.0;
@@ -552,12 +605,12 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
       if (std::sqrt(pa) < op.tol_rot && pt < op.tol_trans) break;
     }
     const double f = -std::log(E);
-    double s[6], Rn[9], tn[3], En = 0.0, gn[6];
+    double s[6], Rn[9], tn[3], En = 0.0, gn[6], Hen[36];
     bool ok = false;
     for (int k = 0; k < 40 && !ok; ++k) {
       for (int a = 0; a < np; ++a) s[a] = alpha * d[a];
       energy_step(D, cR, ct, s, L, Rn, tn);
-      rc = dugma_energy_grad(ctx, Rn, tn, op.cov_mode, &En, gn);
+      rc = eval(Rn, tn, &En, gn, Hen);
       if (rc) return rc;
       if (En > 0 && std::isfinite(En) && -std::log(En) <= f + 1e-4 * alpha * slope)
         ok = true;
@@ -591,6 +644,11 @@ extern "C" int dugma_register_energy(dugma_ctx *ctx, const double *R0, const dou
       for (int a = 0; a < np; ++a)
         for (int b = 0; b < np; ++b)
           H[a * np + b] += -r * (Hy[a] * s[b] + s[a] * Hy[b]) + (r * r * yHy + r) * s[a] * s[b];
+    }
+    double Hn[36];
+    if (newton && newton_inverse_q(En, gn, Hen, L, Hn)) {
+      memcpy(H, Hn, sizeof(H));
+      scaled = true;
     }
     memcpy(cR, Rn, sizeof(double) * DD);
     memcpy(ct, tn, sizeof(double) * D);

@@ -184,6 +184,16 @@ int dugma_energy_grad_points(dugma_ctx *ctx, const double *R, const double *t,
                              double *gX, double *gXC, double *gY, double *gYC,
                              double *gdT);
 
+/* 3-D: dugma_energy_grad plus the 6 x 6 Hessian H of E in the same pose
+ * parameters (row-major, symmetric), from one pass that also sums, per
+ * transformed source x', J^T B J + the second-order turn of x' - t (J =
+ * [-[x'-t]x | I], B = sum_i e (S^-1 d d^T S^-1 - S^-1)).  Exact for isotropic
+ * covariances; with full ones the turn of the sources' covariances is left
+ * out of the second order (the translation block stays exact).  For Newton
+ * steps (dugma_register_energy).  No reference counterpart. */
+int dugma_energy_grad_hess(dugma_ctx *ctx, const double *R, const double *t,
+                           int cov_mode, double *E, double *gpose, double *H);
+
 /* "Plain" gradient: central finite differences of dugma_energy over the pose
  * (2*dofs + 1 forward launches), step h.  Same gpose layout. */
 int dugma_energy_grad_fd(dugma_ctx *ctx, const double *R, const double *t,

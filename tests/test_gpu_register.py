@@ -219,3 +219,83 @@ def test_register_energy_matches_independent_minimiser(oracle, D):
         g[:nr] /= L
         return np.linalg.norm(g)
     assert scaled(o1) <= 1e-3 * scaled(o0), (scaled(o1), scaled(o0))
+
+
+def _left(R, t, p):
+    """The pose after the left increment p = [w, dt]: exp([w]x) R, t + dt."""
+    w = np.asarray(p[:3], float)
+    th = np.linalg.norm(w)
+    K = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    A = np.sin(th) / th if th > 1e-12 else 1.0
+    B = (1.0 - np.cos(th)) / th ** 2 if th > 1e-12 else 0.5
+    return (np.eye(3) + A * K + B * K @ K) @ R, np.asarray(t, float) + np.asarray(p[3:], float)
+
+
+@pytest.mark.parametrize("cov", ["iso", "full"])
+def test_pose_hessian_matches_gradient_differences(cov):
+    """dugma_energy_grad_hess at a converged pose (gradient ~ 0: there the
+    Hessian does not depend on how pose increments compose) against central
+    differences of dugma_energy_grad's own gradient, translation parameters
+    times the clouds' RMS radius: every entry for isotropic covariances (the
+    Hessian is exact there), the translation block for surface covariances
+    (the turn of the covariances is left out of the rest), 2e-3 of the
+    largest entry."""
+    d = W.config3(n=1500, m=1400)
+    R0 = W.random_rotation(np.random.default_rng(5), 4.0) @ d["R_G"]
+    t0 = np.ravel(d["t_G"]) + np.array([0.02, -0.015, 0.02])
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        if cov == "full":
+            dg.update_cov_knn(h)  # surface covariances on both clouds
+        # the same energy and gradient as the pose pass (summed in another order)
+        E1, g1, _ = dg.energy_grad_hess(h, R0, t0, cov=cov)
+        E2, g2 = dg.energy_grad(h, R0, t0, cov=cov)
+        assert abs(E1 / E2 - 1) < 1e-10
+        assert np.abs(g1 - g2).max() < 1e-8 * np.abs(g2).max()
+        R, t, it, E = dg.register_energy(h, R0, t0, cov=cov)
+        E1, g, H = dg.energy_grad_hess(h, R, t, cov=cov)
+        Y = np.asarray(d["Y"]).T
+        L = np.sqrt(((Y - Y.mean(0)) ** 2).sum(1).mean())
+        steps = np.array([1e-4, 1e-4, 1e-4, 1e-4 * L, 1e-4 * L, 1e-4 * L])
+        Hfd = np.zeros((6, 6))
+        for a in range(6):
+            p = np.zeros(6)
+            p[a] = steps[a]
+            gp = dg.energy_grad(h, *_left(R, t, p), cov=cov)[1]
+            gm = dg.energy_grad(h, *_left(R, t, -p), cov=cov)[1]
+            Hfd[:, a] = (gp - gm) / (2 * steps[a])
+    finally:
+        dg.fr_m(h)
+    S = np.diag([1.0, 1.0, 1.0, L, L, L])
+    Hs, Fs = S @ H @ S, S @ (0.5 * (Hfd + Hfd.T)) @ S
+    assert np.allclose(H, H.T)
+    if cov == "iso":
+        err = np.abs(Hs - Fs).max() / np.abs(Fs).max()
+    else:
+        err = np.abs(Hs[3:, 3:] - Fs[3:, 3:]).max() / np.abs(Fs[3:, 3:]).max()
+    print("pose Hessian (%s) vs gradient differences: %.2e of the largest entry" % (cov, err))
+    assert err < 2e-3, (Hs, Fs)
+    assert np.all(np.linalg.eigvalsh(Hs) < 0)  # a maximum of E
+
+
+def test_register_energy_newton_and_bfgs_agree(monkeypatch):
+    """The registration with Newton directions (the default in 3-D) and with
+    BFGS alone (DUGMA_REG_NEWTON=0) end at the same pose, the Newton run in
+    no more steps."""
+    d = W.config3(n=1500, m=1400)
+    R0 = W.random_rotation(np.random.default_rng(7), 6.0) @ d["R_G"]
+    t0 = np.ravel(d["t_G"]) + np.array([-0.02, 0.01, 0.015])
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        Rn, tn, itn, En = dg.register_energy(h, R0, t0)
+        monkeypatch.setenv("DUGMA_REG_NEWTON", "0")
+        Rb, tb, itb, Eb = dg.register_energy(h, R0, t0)
+    finally:
+        dg.fr_m(h)
+    Y = np.asarray(d["Y"]).T
+    L = np.sqrt(((Y - Y.mean(0)) ** 2).sum(1).mean())
+    print("register_energy: Newton %d steps, BFGS %d steps" % (itn, itb))
+    assert W.rotation_error_deg(Rn, Rb) < 5e-3
+    assert np.linalg.norm(tn - tb) < 5e-5 * L
+    assert abs(En / Eb - 1) < 1e-7
+    assert itn <= itb
