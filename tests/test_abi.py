@@ -55,6 +55,7 @@ def test_argument_errors_are_status_codes(lib):
                         20000000, 1, 3, ctypes.byref(out), None)
     assert rc == -4
     assert lib.dugma_energy(None, None, None, 0, None) == -1
+    assert lib.dugma_energy_grad_hess(None, None, None, 0, None, None, None) == -1
     assert lib.dugma_update_sig(None, None, None, 1.0, 1.0, 3.0, None) == -1
     assert lib.dugma_compi(None, None, None, 1.0, 1.0, 3.0, 1.0, None) == -1
     assert lib.dugma_knn(None, 0, 8, None, None, None, None) == -1

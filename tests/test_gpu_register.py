@@ -278,6 +278,16 @@ def test_pose_hessian_matches_gradient_differences(cov):
     assert np.all(np.linalg.eigvalsh(Hs) < 0)  # a maximum of E
 
 
+def test_pose_hessian_is_3d_only():
+    d = W.config5_2d(n=200, m=180)
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        with pytest.raises(dg.DugmaError, match="3-D"):
+            dg.energy_grad_hess(h, np.eye(3), np.zeros(3))
+    finally:
+        dg.fr_m(h)
+
+
 def test_register_energy_newton_and_bfgs_agree(monkeypatch):
     """The registration with Newton directions (the default in 3-D) and with
     BFGS alone (DUGMA_REG_NEWTON=0) end at the same pose, the Newton run in
