@@ -508,8 +508,12 @@ template <class Rec, int MODE, int COMP>
 static void launch_pairs_c(const Plan &pl, const Rec *P, int nP, const Rec *Q, int nQ,
                            double *out, const double *t, hipStream_t s) {
   // the XCD tile order for clouds beyond one L2, except in the per-point
-  // passes (measured 2% slower with it there)
-  const int xcd_map = MODE != kModePoint && (size_t)(nP + nQ) * sizeof(Rec) > kXcdMapBytes ? 1 : 0;
+  // passes (measured 2% slower with it there); DUGMA_XCD_ORDER=0 in the
+  // environment keeps the 2-D order (the sums are the same, bit for bit)
+  const int xcd_map = MODE != kModePoint && (size_t)(nP + nQ) * sizeof(Rec) > kXcdMapBytes &&
+                              env_int("DUGMA_XCD_ORDER", 1) != 0
+                          ? 1
+                          : 0;
   const dim3 grid = tile_grid(pl.pblocks, pl.qchunks, xcd_map), block(kBlock);
   const double t0 = t[0], t1 = t[1], t2 = (RecTraits<Rec>::D == 3) ? t[2] : 0.0;
   if (pl.T == 2)

@@ -438,3 +438,26 @@ def test_one_point_per_lane_matches_oracle(oracle, case):
             else:
                 os.environ[k] = v
         dg.fr_m(h)
+
+
+@pytest.mark.parametrize("cov", ["iso", "full"])
+def test_xcd_tile_order_gives_the_same_sums(monkeypatch, cov):
+    """Clouds beyond 4 MiB of records run their energy and pose passes in the
+    XCD-aware tile order.  Every tile writes its partial to its own slot and
+    the partials are reduced in a fixed order, so the results equal the 2-D
+    order's (DUGMA_XCD_ORDER=0) bit for bit, which also shows that every
+    tile of the padded grid runs exactly once."""
+    d = W.config3(n=90000, m=80000)  # 32-B isotropic / 64-B full records: beyond 4 MiB
+    R, t = d["R_G"], np.ravel(d["t_G"])
+    ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
+    try:
+        out = {}
+        for v in ("1", "0"):
+            monkeypatch.setenv("DUGMA_XCD_ORDER", v)
+            E0 = dg.energy(h, R, t, cov=cov)
+            E1, g = dg.energy_grad(h, R, t, cov=cov)
+            out[v] = (E0, E1, g)
+    finally:
+        dg.fr_m(h)
+    assert out["1"][0] == out["0"][0] and out["1"][1] == out["0"][1]
+    np.testing.assert_array_equal(out["1"][2], out["0"][2])
