@@ -243,9 +243,11 @@ def bench_full_6k(hip, dg, d2, steps, warmup, plain=True):
 
 
 def bench_register_energy(dg, d):
-    """Registration on the convolved energy (BFGS, one fused pass per trial
-    pose) on the 200K x 200K clouds, from 5 degrees about a fixed axis and
-    1 cm per axis off the ground truth."""
+    """Registration on the convolved energy (Newton steps from each pass's
+    pose Hessian, BFGS where that is not positive definite; one fused pass
+    per trial pose) on the 200K x 200K clouds, from 5 degrees about a fixed
+    axis and 1 cm per axis off the ground truth; beside it the same with
+    BFGS alone (DUGMA_REG_NEWTON=0)."""
     from dugma_amd import workloads as W
     ok, h = dg.init(d["X"], d["X_C"], d["Y"], d["Y_C"])
     try:
@@ -256,12 +258,21 @@ def bench_register_energy(dg, d):
         t1 = time.perf_counter()
         R, t, it, E = dg.register_energy(h, R0, t0)
         sec = time.perf_counter() - t1
+        os.environ["DUGMA_REG_NEWTON"] = "0"
+        try:
+            t2 = time.perf_counter()
+            Rb, tb, itb, Eb = dg.register_energy(h, R0, t0)
+            secb = time.perf_counter() - t2
+        finally:
+            del os.environ["DUGMA_REG_NEWTON"]
     finally:
         dg.fr_m(h)
-    log("register_energy 200Kx200K: %d steps in %.3f s" % (it, sec))
+    log("register_energy 200Kx200K: %d steps in %.3f s (BFGS alone: %d steps in %.3f s)" % (it, sec, itb, secb))
     return {"seconds": round(sec, 4), "steps": it, "start_rotation_error_deg": round(W.rotation_error_deg(R0, d["R_G"]), 4),
             "rotation_error_deg": round(W.rotation_error_deg(R, d["R_G"]), 4),
-            "translation_error": float(np.linalg.norm(t - np.ravel(d["t_G"])))}
+            "translation_error": float(np.linalg.norm(t - np.ravel(d["t_G"]))),
+            "bfgs_alone": {"seconds": round(secb, 4), "steps": itb,
+                           "rotation_error_deg": round(W.rotation_error_deg(Rb, d["R_G"]), 4)}}
 
 
 def bench_plain_large(hip, dg, d, big, gradients=2):
