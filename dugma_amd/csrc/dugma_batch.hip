@@ -56,7 +56,8 @@
 //                       by the target-side identity, in the source frame;
 //                       a wave skips each block of 64 sources whose terms
 //                       are all below 7e-13 of their peaks (kFarExp)
-//   k_batchf_update     sums the items, rotates the gradient back, Armijo
+//   k_batchf_fold       sums each running trial's items (one workgroup each)
+//   k_batchf_update     one thread per trial: rotates the gradient back, Armijo
 //                       test, BFGS update, next candidate pose; at an
 //                       accepted pose the direction comes from the pass's
 //                       own approximate Hessian (Newton) when that is
@@ -945,17 +946,38 @@ __device__ bool newton_inverse(const double *st, const double *cR, const double 
 // pose, then the optimiser's next move (dugma_register_energy's logic, and
 // a trial also stops when its next quasi-Newton step is below the
 // tolerances).
-__global__ __launch_bounds__(256) void k_batchf_update(
-    const double *__restrict__ part, const int *__restrict__ titem, int k0, int k1,
-    BRef *__restrict__ ref, int *__restrict__ done, unsigned *__restrict__ queue, RParams rp) {
+// One workgroup per running trial: its items' kRefStat statistics staged
+// in LDS a block of items at a time, thread a summing statistic a in item
+// order, into fold[k]; re-arms the queue.
+__global__ __launch_bounds__(kUpd) void k_batchf_fold(const double *__restrict__ part,
+                                                      const int *__restrict__ titem, int k0, int k1,
+                                                      const int *__restrict__ done,
+                                                      unsigned *__restrict__ queue, double *__restrict__ fold) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0u;
+  const int k = k0 + blockIdx.x;
+  if (k >= k1 || done[k]) return;
+  __shared__ double s_part[kUpdItems * kStat];
+  const int i0 = titem[k], i1 = titem[k + 1];
+  double acc = 0.0;
+  for (int ib = i0; ib < i1; ib += kUpdItems) {
+    const int nb = min(kUpdItems, i1 - ib);
+    for (int e = threadIdx.x; e < nb * kStat; e += kUpd) s_part[e] = part[(size_t)ib * kStat + e];
+    __syncthreads();
+    if (threadIdx.x < kRefStat)
+      for (int i = 0; i < nb; ++i) acc += s_part[i * kStat + threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x < kRefStat) fold[(size_t)k * kStat + threadIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_batchf_update(const double *__restrict__ fold, int k0, int k1,
+                                                       BRef *__restrict__ ref, int *__restrict__ done,
+                                                       RParams rp) {
   const int k = k0 + blockIdx.x * 256 + threadIdx.x;
   if (k >= k1 || done[k]) return;
   BRef r = ref[k];
   double st[kRefStat];
-  for (int a = 0; a < kRefStat; ++a) st[a] = 0.0;
-  for (int i = titem[k]; i < titem[k + 1]; ++i)
-    for (int a = 0; a < kRefStat; ++a) st[a] += part[(size_t)i * kStat + a];
+  for (int a = 0; a < kRefStat; ++a) st[a] = fold[(size_t)k * kStat + a];
   const double E = st[0];
   // gradient (rotation, translation) back from the source frame: g = cR g'
   double g[6];
@@ -1685,8 +1707,10 @@ static int launch_refine(dugma_batch *b, int l, int k0, int k1, const RParams &r
       default: launch_pairsf<4>(b, l, grid, order, ni); break;
     }
   }
-  hipLaunchKernelGGL(k_batchf_update, dim3((k1 - k0 + 255) / 256), dim3(256), 0, b->stream, L.dpart, L.dtitem, k0,
-                     k1, b->dref, b->ddone, b->dqueue, rp);
+  hipLaunchKernelGGL(k_batchf_fold, dim3(k1 - k0), dim3(kUpd), 0, b->stream, L.dpart, L.dtitem, k0, k1, b->ddone,
+                     b->dqueue, b->dfold);
+  hipLaunchKernelGGL(k_batchf_update, dim3((k1 - k0 + 255) / 256), dim3(256), 0, b->stream, b->dfold, k0, k1,
+                     b->dref, b->ddone, rp);
   DUGMA_HIP(hipGetLastError());
   return DUGMA_OK;
 }
